@@ -1,0 +1,377 @@
+"""MI355X implementation of the Falcons interpolation compression path.
+
+The Python surface mirrors pyfalcons (tools/pyfalcons/src/pyfalcons/falcons.pyx,
+pyConfig.pyx, pyConfigEnums.py): ``falconsConfig``, ``falcons.compress``,
+``falcons.decompress``, ``falcons.verify`` and the enum classes, with the same argument meaning and
+error behaviour (ValueError where the C++ API raises std::invalid_argument). Every call goes
+through the C-ABI of ``libfalcons_mi355x.so`` (declared in include/falcons_mi355x.h), whose
+kernels run on the GPU. There is no CPU fallback: when the library is not built, importing this
+package fails.
+
+Beyond pyfalcons, arrays already on the GPU (torch tensors on a ROCm device) are compressed in
+place on the device, and ``Context.compress_ex`` exposes the pre-zstd buffer and device timings.
+"""
+import ctypes as C
+import os
+from enum import IntEnum
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfalcons_mi355x.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it with `make -C falcons_amd/csrc` "
+                      "(or __graft_entry__.build())")
+
+_L = C.CDLL(LIB_PATH)
+
+FZ_OK, FZ_ERR_INVALID_ARGUMENT, FZ_ERR_UNSUPPORTED, FZ_ERR_GPU, FZ_ERR_RUNTIME = range(5)
+
+
+class fz_config(C.Structure):
+    _fields_ = [
+        ("N", C.c_int32), ("dims", C.c_uint64 * 4), ("num", C.c_uint64),
+        ("cmpr_algo", C.c_int32), ("eb_mode", C.c_int32),
+        ("abs_eb", C.c_double), ("rel_eb", C.c_double), ("psnr_eb", C.c_double), ("l2norm_eb", C.c_double),
+        ("lorenzo", C.c_int32), ("lorenzo2", C.c_int32), ("regression", C.c_int32), ("regression2", C.c_int32),
+        ("openmp", C.c_int32), ("quantbin_cnt", C.c_int32), ("block_size", C.c_int32),
+        ("pred_dim", C.c_int32), ("data_type", C.c_int32),
+        ("interp_algo", C.c_int32), ("interp_direction", C.c_int32), ("interp_anchor_stride", C.c_int32),
+        ("interp_alpha", C.c_double), ("interp_beta", C.c_double),
+    ]
+
+
+class fz_opts(C.Structure):
+    _fields_ = [
+        ("simd", C.c_int32), ("copy_back", C.c_int32),
+        ("prezstd", C.POINTER(C.c_uint8)), ("prezstd_cap", C.c_size_t), ("prezstd_len", C.c_size_t),
+        ("t_device_ms", C.c_double), ("t_zstd_ms", C.c_double), ("t_total_ms", C.c_double),
+        ("ncodes", C.c_uint64), ("nunpred", C.c_uint64), ("prezstd_bytes", C.c_uint64),
+        ("method", C.c_int32), ("use_delta", C.c_int32),
+    ]
+
+
+_u8p = C.POINTER(C.c_uint8)
+_L.fz_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+_L.fz_ctx_create.restype = C.c_int
+_L.fz_ctx_destroy.argtypes = [C.c_void_p]
+_L.fz_last_error.argtypes = [C.c_void_p]
+_L.fz_last_error.restype = C.c_char_p
+_L.fz_opts_init.argtypes = [C.POINTER(fz_opts)]
+_L.fz_config_init.argtypes = [C.POINTER(fz_config), C.c_int, C.POINTER(C.c_uint64)]
+_L.fz_config_size_est.argtypes = [C.POINTER(fz_config)]
+_L.fz_config_size_est.restype = C.c_size_t
+_L.fz_config_save.argtypes = [C.POINTER(fz_config), _u8p]
+_L.fz_config_save.restype = C.c_size_t
+_L.fz_read_config.argtypes = [_u8p, C.c_size_t, C.POINTER(fz_config), C.c_char_p, C.c_size_t]
+_L.fz_read_config.restype = C.c_int
+_L.fz_compress_size_bound.argtypes = [C.POINTER(fz_config), C.c_int]
+_L.fz_compress_size_bound.restype = C.c_size_t
+_L.fz_compress.argtypes = [C.c_void_p, C.POINTER(fz_config), C.c_int, C.c_void_p, C.c_int, _u8p, C.c_size_t,
+                           C.POINTER(C.c_size_t), C.POINTER(fz_opts)]
+_L.fz_compress.restype = C.c_int
+_L.fz_decompress.argtypes = [C.c_void_p, _u8p, C.c_size_t, C.c_int, C.c_void_p, C.c_size_t, C.c_int,
+                             C.POINTER(fz_config), C.POINTER(fz_opts)]
+_L.fz_decompress.restype = C.c_int
+
+EXPORTED_SYMBOLS = ("fz_ctx_create", "fz_ctx_destroy", "fz_last_error", "fz_opts_init", "fz_config_init",
+                    "fz_config_size_est", "fz_config_save", "fz_read_config", "fz_compress_size_bound",
+                    "fz_compress", "fz_decompress")
+
+
+class falconsErrorBoundMode(IntEnum):
+    ABS = 0
+    REL = 1
+    PSNR = 2
+    L2NORM = 3
+    ABS_AND_REL = 4
+    ABS_OR_REL = 5
+
+
+class falconsAlgorithm(IntEnum):
+    LORENZO_REG = 0
+    INTERP_LORENZO = 1
+    INTERP = 2
+    NOPRED = 3
+    LOSSLESS = 4
+
+
+class falconsInterpAlgorithm(IntEnum):
+    LINEAR = 0
+    CUBIC = 1
+
+
+class FalconsError(RuntimeError):
+    pass
+
+
+class FalconsUnsupported(FalconsError):
+    pass
+
+
+def _raise(rc, msg):
+    if rc == FZ_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if rc == FZ_ERR_UNSUPPORTED:
+        raise FalconsUnsupported(msg)
+    raise FalconsError(msg)
+
+
+class falconsConfig:
+    """Falcons::Config (include/Falcons/utils/Config.hpp) as seen from Python."""
+
+    def __init__(self, *args):
+        self.conf = fz_config()
+        self.setDims(*args if args else (1,))
+
+    def setDims(self, *args):
+        if not args:
+            raise ValueError("At least one dimension required")
+        for a in args:
+            if int(a) != a or int(a) < 1:
+                raise ValueError(f"Dimension must be positive integer, got {a}")
+        keep = {k: getattr(self.conf, k) for k, _ in fz_config._fields_ if k not in ("N", "dims", "num",
+                                                                                   "block_size", "pred_dim")}
+        fresh = self.conf.N == 0
+        dims = (C.c_uint64 * 4)(*[int(a) for a in args][:4])
+        _L.fz_config_init(C.byref(self.conf), len(args), dims)
+        if not fresh:
+            for k, v in keep.items():
+                setattr(self.conf, k, v)
+        return self.num_elements
+
+    @property
+    def dims(self):
+        return tuple(int(self.conf.dims[i]) for i in range(self.conf.N))
+
+    @property
+    def num_elements(self):
+        return int(self.conf.num)
+
+    @property
+    def ndim(self):
+        return int(self.conf.N)
+
+    def _prop(name, cast=float):
+        def g(self):
+            return cast(getattr(self.conf, name))
+
+        def s(self, v):
+            setattr(self.conf, name, int(v) if cast is not float else float(v))
+
+        return property(g, s)
+
+    absErrorBound = _prop("abs_eb")
+    relErrorBound = _prop("rel_eb")
+    psnrErrorBound = _prop("psnr_eb")
+    l2normErrorBound = _prop("l2norm_eb")
+    errorBoundMode = _prop("eb_mode", int)
+    cmprAlgo = _prop("cmpr_algo", int)
+    interpAlgo = _prop("interp_algo", int)
+    interpDirection = _prop("interp_direction", int)
+    interpAnchorStride = _prop("interp_anchor_stride", int)
+    interpAlpha = _prop("interp_alpha")
+    interpBeta = _prop("interp_beta")
+    lorenzo = _prop("lorenzo", bool)
+    lorenzo2 = _prop("lorenzo2", bool)
+    regression = _prop("regression", bool)
+    regression2 = _prop("regression2", bool)
+    openmp = _prop("openmp", bool)
+    quantbinCnt = _prop("quantbin_cnt", int)
+    blockSize = _prop("block_size", int)
+    dataType = _prop("data_type", int)
+    del _prop
+
+    def size_est(self):
+        return int(_L.fz_config_size_est(C.byref(self.conf)))
+
+    def save(self):
+        buf = (C.c_uint8 * 256)()
+        n = _L.fz_config_save(C.byref(self.conf), buf)
+        return bytes(buf[:n])
+
+
+def read_config(stream):
+    """Falcons_read_config: parse and validate a stream header (ValueError when invalid)."""
+    b = np.frombuffer(bytes(stream), dtype=np.uint8) if not isinstance(stream, np.ndarray) else stream
+    b = np.ascontiguousarray(b, dtype=np.uint8)
+    cfg = falconsConfig(1)
+    err = C.create_string_buffer(256)
+    rc = _L.fz_read_config(b.ctypes.data_as(_u8p), b.size, C.byref(cfg.conf), err, 256)
+    if rc:
+        raise ValueError(err.value.decode())
+    return cfg
+
+
+_DT = {np.dtype(np.float32): 0, np.dtype(np.float64): 1}
+
+
+def _is_device_tensor(x):
+    return type(x).__module__.startswith("torch") and getattr(x, "is_cuda", False)
+
+
+class Context:
+    """One GPU: stream, device workspace. Methods are not re-entrant on one context."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = _L.fz_ctx_create(int(device), C.byref(h))
+        if rc:
+            raise FalconsError(f"cannot create a GPU context on device {device}")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            _L.fz_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self):
+        return _L.fz_last_error(self.h).decode()
+
+    def compress_ex(self, data, config, simd=1, want_prezstd=False, copy_back=True):
+        """Compress a numpy array (host) or a torch GPU tensor (device, overwritten in place).
+        Returns (stream bytes as uint8 ndarray, fz_opts with timings, pre-zstd bytes or None)."""
+        on_dev = _is_device_tensor(data)
+        if on_dev:
+            import torch
+            dt = {torch.float32: 0, torch.float64: 1}.get(data.dtype)
+            if dt is None or not data.is_contiguous():
+                raise TypeError("device input must be a contiguous float32/float64 tensor")
+            ptr, nbytes, size = data.data_ptr(), data.numel() * data.element_size(), data.numel()
+        else:
+            if data.dtype not in _DT:
+                raise TypeError(f"Unsupported dtype: {data.dtype}. Supported: float32, float64")
+            if not data.flags["C_CONTIGUOUS"] or not data.flags["WRITEABLE"]:
+                raise ValueError("compression needs a C-contiguous writable array")
+            dt, ptr, nbytes, size = _DT[data.dtype], data.ctypes.data, data.nbytes, data.size
+        if size == 0:
+            raise ValueError("Cannot compress an empty array")
+        if config.num_elements != size:
+            config.setDims(*data.shape)
+        cap = int(_L.fz_compress_size_bound(C.byref(config.conf), dt))
+        out = np.empty(cap, dtype=np.uint8)
+        opts = fz_opts()
+        _L.fz_opts_init(C.byref(opts))
+        opts.simd = int(simd)
+        opts.copy_back = 1 if copy_back else 0
+        pre = None
+        if want_prezstd:
+            pre = np.empty(nbytes * 3 + (4 << 20), dtype=np.uint8)
+            opts.prezstd = pre.ctypes.data_as(_u8p)
+            opts.prezstd_cap = pre.size
+        olen = C.c_size_t(0)
+        rc = _L.fz_compress(self.h, C.byref(config.conf), dt, C.c_void_p(ptr), 1 if on_dev else 0,
+                            out.ctypes.data_as(_u8p), cap, C.byref(olen), C.byref(opts))
+        if rc:
+            _raise(rc, self._err())
+        if want_prezstd:
+            pre = pre[: opts.prezstd_len].tobytes()
+        return out[: olen.value], opts, pre
+
+    def decompress_ex(self, stream, dtype, out=None):
+        """Decompress into a new numpy array, or into `out` (numpy array or torch GPU tensor)."""
+        b = np.ascontiguousarray(np.frombuffer(bytes(stream), dtype=np.uint8)
+                                 if not isinstance(stream, np.ndarray) else stream, dtype=np.uint8)
+        dt = _DT[np.dtype(dtype)]
+        cfg = falconsConfig(1)
+        opts = fz_opts()
+        _L.fz_opts_init(C.byref(opts))
+        if out is None:
+            hdr = read_config(b)
+            out = np.empty(hdr.num_elements, dtype=dtype)
+        on_dev = _is_device_tensor(out)
+        ptr = out.data_ptr() if on_dev else out.ctypes.data
+        cap = out.numel() if on_dev else out.size
+        rc = _L.fz_decompress(self.h, b.ctypes.data_as(_u8p), b.size, dt, C.c_void_p(ptr), cap, 1 if on_dev else 0,
+                              C.byref(cfg.conf), C.byref(opts))
+        if rc:
+            _raise(rc, self._err())
+        return out, cfg, opts
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class falcons:
+    """pyfalcons.falcons: compress / decompress / verify (falcons.pyx:31-333)."""
+
+    _SUPPORTED_DTYPES = (np.dtype(np.float32), np.dtype(np.float64))
+
+    @staticmethod
+    def compress(data, config, copy=False):
+        if _is_device_tensor(data):
+            out, _, _ = default_context().compress_ex(data, config)
+            return out, data.numel() * data.element_size() / float(out.size)
+        if data.dtype not in falcons._SUPPORTED_DTYPES:
+            raise TypeError(f"Unsupported dtype: {data.dtype}. Supported: float32, float64")
+        if data.size == 0:
+            raise ValueError("Cannot compress an empty array")
+        if not isinstance(config, falconsConfig):
+            raise TypeError(f"config must be falconsConfig, got {type(config)}")
+        if copy:
+            data = np.array(data, order="C", copy=True)
+        else:
+            if not data.flags["C_CONTIGUOUS"]:
+                raise ValueError("copy=False requires a C-contiguous array; pass copy=True")
+            if not data.flags["WRITEABLE"]:
+                raise ValueError("copy=False requires a writable array; pass copy=True")
+        out, _, _ = default_context().compress_ex(data, config)
+        return out, data.nbytes / float(out.size)
+
+    @staticmethod
+    def decompress(compressed, dtype, shape, config=None):
+        compressed = np.asarray(compressed)
+        if compressed.dtype != np.uint8:
+            raise TypeError(f"Compressed data must be uint8, got {compressed.dtype}")
+        if np.dtype(dtype) not in falcons._SUPPORTED_DTYPES:
+            raise TypeError(f"Unsupported dtype: {dtype}")
+        shape = tuple(int(s) for s in shape)
+        if any(s <= 0 for s in shape):
+            raise ValueError("Shape dimensions must be positive integers")
+        hdr = read_config(compressed)
+        if int(np.prod(shape)) != hdr.num_elements:
+            raise ValueError("Requested shape does not match the compressed element count")
+        out = np.empty(hdr.num_elements, dtype=dtype)
+        out, cfg, _ = default_context().decompress_ex(compressed, dtype, out)
+        return out.reshape(shape), cfg
+
+    @staticmethod
+    def verify(src_data, dec_data):
+        if np.shape(src_data) != np.shape(dec_data):
+            raise ValueError("Shape mismatch between src_data and dec_data")
+        if src_data.dtype != dec_data.dtype:
+            raise ValueError("Dtype mismatch between src_data and dec_data")
+        if src_data.size == 0:
+            raise ValueError("Cannot verify empty arrays")
+        source = src_data.astype(np.float64)
+        data_range = float(np.max(source) - np.min(source))
+        diff = source - dec_data.astype(np.float64)
+        max_diff = float(np.max(np.abs(diff)))
+        mse = float(np.mean(diff ** 2))
+        if mse == 0:
+            nrmse, psnr = 0.0, float("inf")
+        elif data_range == 0:
+            nrmse, psnr = float("inf"), -float("inf")
+        else:
+            nrmse = float(np.sqrt(mse) / data_range)
+            psnr = float(-20.0 * np.log10(nrmse))
+        return max_diff, psnr, nrmse
+
+
+__all__ = ["falcons", "falconsConfig", "falconsErrorBoundMode", "falconsAlgorithm", "falconsInterpAlgorithm",
+           "Context", "read_config", "FalconsError", "FalconsUnsupported", "LIB_PATH", "EXPORTED_SYMBOLS"]

@@ -1,0 +1,785 @@
+// C-ABI and host pipeline of the MI355X Falcons interpolation path.
+//
+// Host work is limited to what the reference keeps outside the hot loops: the stream header
+// (Config::save/load), the level/pass plan (pure index arithmetic on the dimensions), the
+// per-level ratio table, the final zstd stage and the ratio<3 lossless test
+// (FalconsDispatcher.hpp:121-133). Everything that touches every element runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/falcons_mi355x.h"
+#include "fz_common.h"
+#include "fz_kernels.h"
+#include "fz_tuner.h"
+#include "fz_zstd.h"
+
+using namespace fz;
+
+namespace {
+
+constexpr uint32_t kMagic = 0xF342F310u;  // version.hpp.in:10
+constexpr uint32_t kDataVer = 0x01000000u;  // versionInt("1.0.0"), CMakeLists.txt:7
+constexpr unsigned long long kZstdContentSizeUnknown = 0ULL - 1;  // zstd.h ZSTD_CONTENTSIZE_UNKNOWN
+constexpr unsigned long long kZstdContentSizeError = 0ULL - 2;    // zstd.h ZSTD_CONTENTSIZE_ERROR
+
+struct ArgError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct Unsupported : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct fz_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevArena arena;
+    std::string err;
+};
+
+// ------------------------------------------------------------------ Config header
+extern "C" void fz_config_init(fz_config* c, int N, const uint64_t* dims) {
+    memset(c, 0, sizeof *c);
+    int n = 0;
+    uint64_t num = 1;
+    for (int i = 0; i < N; i++)
+        if (dims[i] > 1) { c->dims[n++] = dims[i]; num *= dims[i]; }
+    if (n == 0) { c->dims[0] = 1; n = 1; }
+    c->N = n;
+    c->num = num;
+    c->pred_dim = n;
+    c->block_size = (n == 1 ? 128 : (n == 2 ? 16 : 6));
+    c->cmpr_algo = 1;
+    c->eb_mode = 0;
+    c->abs_eb = 1e-3;
+    c->lorenzo = 1;
+    c->regression = 1;
+    c->quantbin_cnt = 65536;
+    c->interp_algo = 1;
+    c->interp_anchor_stride = -1;
+    c->interp_alpha = 1.25;
+    c->interp_beta = 2.0;
+}
+
+namespace {
+template <typename V>
+void put(uint8_t*& p, V v) {
+    memcpy(p, &v, sizeof v);
+    p += sizeof v;
+}
+template <typename V>
+V get(const uint8_t*& p) {
+    V v;
+    memcpy(&v, p, sizeof v);
+    p += sizeof v;
+    return v;
+}
+}  // namespace
+
+extern "C" size_t fz_config_save(const fz_config* c, uint8_t* out) {
+    uint8_t* p = out + 1;
+    put<uint32_t>(p, kMagic);
+    put<uint32_t>(p, kDataVer);
+    put<int8_t>(p, (int8_t)c->N);
+    uint64_t mx = 0;
+    for (int i = 0; i < c->N; i++) mx = std::max<uint64_t>(mx, c->dims[i]);
+    uint8_t bw = 0;
+    while (mx > 0) { mx >>= 1; bw++; }
+    put<uint8_t>(p, bw);
+    const size_t nbytes = ((size_t)c->N * bw + 7) / 8;
+    memset(p, 0, nbytes);
+    for (int i = 0; i < c->N; i++)
+        for (int j = 0; j < bw; j++) {
+            const size_t b = (size_t)i * bw + j;
+            if ((c->dims[i] >> j) & 1) p[b / 8] |= (uint8_t)(1u << (b % 8));
+        }
+    p += nbytes;
+    put<uint64_t>(p, c->num);
+    put<uint8_t>(p, (uint8_t)c->cmpr_algo);
+    put<uint8_t>(p, (uint8_t)c->eb_mode);
+    switch (c->eb_mode) {
+        case 0: put<double>(p, c->abs_eb); break;
+        case 1: put<double>(p, c->rel_eb); break;
+        case 2: put<double>(p, c->psnr_eb); break;
+        case 3: put<double>(p, c->l2norm_eb); break;
+        case 4:
+        case 5: put<double>(p, c->abs_eb); put<double>(p, c->rel_eb); break;
+        default: break;
+    }
+    put<uint8_t>(p, (uint8_t)(((c->lorenzo & 1) << 7) | ((c->lorenzo2 & 1) << 6) | ((c->regression & 1) << 5) |
+                              ((c->regression2 & 1) << 4) | ((c->openmp & 1) << 3)));
+    put<uint8_t>(p, (uint8_t)c->data_type);
+    put<int32_t>(p, c->quantbin_cnt);
+    put<int32_t>(p, c->block_size);
+    put<uint8_t>(p, (uint8_t)c->pred_dim);
+    const size_t sz = (size_t)(p - out);
+    out[0] = (uint8_t)sz;
+    return sz;
+}
+
+extern "C" size_t fz_config_size_est(const fz_config* c) {
+    uint8_t tmp[256];
+    return fz_config_save(c, tmp);
+}
+
+namespace {
+// Config::load (Config.hpp:372-454) then the checks of Falcons_read_config (falcons.hpp:122-141).
+size_t load_config(const uint8_t* in, size_t avail, fz_config* c) {
+    if (!in || avail < 1) throw ArgError("Truncated Falcons configuration");
+    const size_t hs = in[0];
+    if (hs < 1 || hs > avail) throw ArgError("Truncated Falcons configuration");
+    const uint8_t *p = in + 1, *end = in + hs;
+    auto need = [&](size_t k) {
+        if ((size_t)(end - p) < k) throw ArgError("Truncated Falcons configuration field");
+    };
+    need(4); const uint32_t magic = get<uint32_t>(p);
+    need(4); const uint32_t ver = get<uint32_t>(p);
+    need(1); const int8_t N = get<int8_t>(p);
+    need(1); const uint8_t bw = get<uint8_t>(p);
+    if (N < 1 || N > 4 || bw < 1 || bw > 64) throw ArgError("Invalid Falcons dimensions");
+    const size_t db = ((size_t)N * bw + 7) / 8;
+    if ((size_t)(end - p) < db) throw ArgError("Truncated Falcons dimensions");
+    memset(c, 0, sizeof *c);
+    c->N = N;
+    for (int i = 0; i < N; i++) {
+        uint64_t v = 0;
+        for (int j = 0; j < bw; j++) {
+            const size_t b = (size_t)i * bw + j;
+            v |= (uint64_t)((p[b / 8] >> (b % 8)) & 1u) << j;
+        }
+        c->dims[i] = v;
+    }
+    p += db;
+    need(8); c->num = get<uint64_t>(p);
+    uint64_t expected = 1;
+    for (int i = 0; i < N; i++) {
+        if (c->dims[i] == 0 || c->dims[i] > UINT64_MAX / expected) throw ArgError("Invalid Falcons element count");
+        expected *= c->dims[i];
+    }
+    if (c->num != expected) throw ArgError("Falcons dimensions do not match element count");
+    need(1); c->cmpr_algo = get<uint8_t>(p);
+    need(1); c->eb_mode = get<uint8_t>(p);
+    if (c->eb_mode > 5 || c->cmpr_algo > 4) throw ArgError("Invalid Falcons algorithm or error mode");
+    switch (c->eb_mode) {
+        case 0: need(8); c->abs_eb = get<double>(p); break;
+        case 1: need(8); c->rel_eb = get<double>(p); break;
+        case 2: need(8); c->psnr_eb = get<double>(p); break;
+        case 3: need(8); c->l2norm_eb = get<double>(p); break;
+        default: need(16); c->abs_eb = get<double>(p); c->rel_eb = get<double>(p); break;
+    }
+    c->quantbin_cnt = 65536;
+    c->lorenzo = 1;
+    c->regression = 1;
+    if (p < end) {
+        const uint8_t f = get<uint8_t>(p);
+        c->lorenzo = (f >> 7) & 1; c->lorenzo2 = (f >> 6) & 1; c->regression = (f >> 5) & 1;
+        c->regression2 = (f >> 4) & 1; c->openmp = (f >> 3) & 1;
+    }
+    if (p < end) c->data_type = get<uint8_t>(p);
+    if (p < end) { need(4); c->quantbin_cnt = get<int32_t>(p); }
+    if (p < end) { need(4); c->block_size = get<int32_t>(p); }
+    if (p < end) { need(1); c->pred_dim = get<uint8_t>(p); }
+    if (magic != kMagic) throw ArgError("magic number mismatch, the input data is not compressed by Falcons");
+    if (ver != kDataVer) {
+        char m[96];
+        snprintf(m, sizeof m, "Please use Falcons v%u.%u.%u to decompress the data", ver >> 24, (ver >> 16) & 0xFF,
+                 (ver >> 8) & 0xFF);
+        throw ArgError(m);
+    }
+    if (avail <= hs) throw ArgError("Missing Falcons compressed payload");
+    c->interp_algo = 1;
+    c->interp_anchor_stride = -1;
+    c->interp_alpha = 1.25;
+    c->interp_beta = 2.0;
+    return hs;
+}
+}  // namespace
+
+extern "C" int fz_read_config(const uint8_t* cmp, size_t len, fz_config* out, char* err, size_t errlen) {
+    try {
+        load_config(cmp, len, out);
+        return FZ_OK;
+    } catch (const std::exception& e) {
+        if (err && errlen) snprintf(err, errlen, "%s", e.what());
+        return FZ_ERR_INVALID_ARGUMENT;
+    }
+}
+
+extern "C" size_t fz_compress_size_bound(const fz_config* c, int dtype) {
+    return 4096 + fz_config_size_est(c) + fz::zstd().compressBound(c->num * (dtype == FZ_DOUBLE ? 8 : 4));
+}
+
+// ------------------------------------------------------------------ level / pass plan
+namespace {
+
+struct Plan {
+    int N = 0;
+    uint64_t D[3] = {1, 1, 1};  // slowest first, N entries used
+    uint64_t anchor = 0;
+    int levels = 0;
+    uint64_t ncodes = 0, nanchors = 0;
+    uint64_t acnt[3] = {1, 1, 1}, astr[3] = {0, 0, 0};
+    std::vector<PassDesc> passes;
+};
+
+PassDesc rows_pass(int cubic, uint64_t n, uint64_t sp, uint64_t no, uint64_t so, uint64_t len, uint64_t sk, int skip,
+                   int is2d, int level) {
+    PassDesc p{};
+    p.type = 0; p.cubic = cubic; p.skip = skip; p.is2d = is2d; p.level = level - 1;
+    p.n = n; p.sp = sp; p.no = no; p.so = so; p.len = len; p.sk = sk;
+    p.ncodes = n > 1 ? no * (n / 2) * len : 0;
+    return p;
+}
+PassDesc lines_pass(int cubic, uint64_t n, uint64_t sp, uint64_t n1, uint64_t s1, uint64_t n2, uint64_t s2, int skip,
+                    int level) {
+    PassDesc p{};
+    p.type = 1; p.cubic = cubic; p.skip = skip; p.level = level - 1;
+    p.n = n; p.sp = sp; p.n1 = n1; p.s1 = s1; p.n2 = n2; p.s2 = s2;
+    p.ncodes = n > 1 ? n1 * n2 * (n / 2) : 0;
+    return p;
+}
+
+// InterpolationDecomposition::init (IDH:286-339) and the pass sequence of interpolation()
+// (IDH:2294-2510) for the serial build, TUNING::DISABLED.
+Plan make_plan(int N, const uint64_t* dims, long anchor_in, int cubic, int dir) {
+    Plan P;
+    P.N = N;
+    for (int i = 0; i < N; i++) P.D[i] = dims[i];
+    int L = -1;
+    bool use_anchor = false;
+    uint64_t anchor = anchor_in < 0 ? 0 : (uint64_t)anchor_in;
+    for (int i = 0; i < N; i++) {
+        const double lg = std::ceil(std::log2((double)dims[i]));
+        if (L < lg) L = (int)lg;
+        if (dims[i] > anchor) use_anchor = true;
+    }
+    if (!use_anchor) anchor = 0;
+    if (anchor > 0) {
+        const int m = (int)std::log2((double)anchor) + 1;
+        if (m <= L) L = m;
+    }
+    if (N == 3 && dir != 0 && dir != 5) throw Unsupported("3-D interpolation direction must be 0 or 5");
+    if (N == 2 && dir != 0 && dir != 1) throw Unsupported("2-D interpolation direction must be 0 or 1");
+    P.anchor = anchor;
+    uint64_t code = 0;
+    if (anchor == 0) {
+        code = 1;
+    } else {
+        uint64_t off[3] = {1, 1, 1};
+        if (N == 3) { off[0] = dims[1] * dims[2]; off[1] = dims[2]; }
+        if (N == 2) { off[0] = dims[1]; }
+        // anchor grid in slowest-first order, padded to 3 axes
+        for (int i = 0; i < 3; i++) { P.acnt[i] = 1; P.astr[i] = 0; }
+        for (int i = 0; i < N; i++) {
+            P.acnt[3 - N + i] = (dims[i] - 1) / anchor + 1;
+            P.astr[3 - N + i] = anchor * off[i];
+        }
+        P.nanchors = P.acnt[0] * P.acnt[1] * P.acnt[2];
+        L--;
+    }
+    P.levels = L;
+    for (int level = L; level > 0; level--) {
+        const uint64_t st = 1ull << (level - 1);
+        const int last = level == 1;
+        std::vector<PassDesc> lv;
+        if (N == 1) {
+            lv.push_back(lines_pass(cubic, (dims[0] - 1) / st + 1, st, 1, 0, 1, 0, last, level));
+        } else if (N == 2) {
+            const uint64_t D0 = dims[0], D1 = dims[1];
+            if (dir == 0) {
+                lv.push_back(rows_pass(cubic, (D0 - 1) / st + 1, st * D1, 1, 0, (D1 - 1) / (2 * st) + 1, 2 * st, 0, 1, level));
+                lv.push_back(lines_pass(cubic, (D1 - 1) / st + 1, st, (D0 - 1) / st + 1, st * D1, 1, 0, last, level));
+            } else {
+                lv.push_back(lines_pass(cubic, (D1 - 1) / st + 1, st, (D0 - 1) / (2 * st) + 1, 2 * st * D1, 1, 0, 0, level));
+                lv.push_back(rows_pass(cubic, (D0 - 1) / st + 1, st * D1, 1, 0, (D1 - 1) / st + 1, st, last, 1, level));
+            }
+        } else {
+            const uint64_t Dz = dims[0], Dy = dims[1], Dx = dims[2], Sz = Dy * Dx, Sy = Dx;
+            if (dir == 0) {
+                lv.push_back(rows_pass(cubic, (Dz - 1) / st + 1, st * Sz, (Dy - 1) / (2 * st) + 1, 2 * st * Sy,
+                                       (Dx - 1) / (2 * st) + 1, 2 * st, 0, 0, level));
+                lv.push_back(rows_pass(cubic, (Dy - 1) / st + 1, st * Sy, (Dz - 1) / st + 1, st * Sz,
+                                       (Dx - 1) / (2 * st) + 1, 2 * st, 0, 0, level));
+                lv.push_back(lines_pass(cubic, (Dx - 1) / st + 1, st, (Dz - 1) / st + 1, st * Sz, (Dy - 1) / st + 1,
+                                        st * Sy, last, level));
+            } else {
+                lv.push_back(lines_pass(cubic, (Dx - 1) / st + 1, st, (Dz - 1) / (2 * st) + 1, 2 * st * Sz,
+                                        (Dy - 1) / (2 * st) + 1, 2 * st * Sy, 0, level));
+                lv.push_back(rows_pass(cubic, (Dy - 1) / st + 1, st * Sy, (Dz - 1) / (2 * st) + 1, 2 * st * Sz,
+                                       (Dx - 1) / st + 1, st, 0, 0, level));
+                lv.push_back(rows_pass(cubic, (Dz - 1) / st + 1, st * Sz, (Dy - 1) / st + 1, st * Sy, (Dx - 1) / st + 1,
+                                       st, last, 0, level));
+            }
+        }
+        for (auto& p : lv) {
+            p.code_base = code;
+            code += p.ncodes;
+            P.passes.push_back(p);
+        }
+    }
+    P.ncodes = code;
+    return P;
+}
+
+// Level ratio table min(alpha^(level-1), beta) with the host's libm pow (IDH:186-192); mode 0 keeps
+// the bound, 1 halves it from level 3 (alpha < 0), 2 divides by the ratio (alpha >= 1).
+int ratio_table(double alpha, double beta, int levels, std::vector<double>& r) {
+    r.assign(std::max(levels, 1), 1.0);
+    if (alpha < 0) return 1;
+    if (alpha >= 1) {
+        for (int l = 1; l <= levels; l++) {
+            double c = std::pow(alpha, l - 1);
+            if (c > beta) c = beta;
+            r[l - 1] = c;
+        }
+        return 2;
+    }
+    return 0;
+}
+
+void zstd_frame(const uint8_t* src, size_t n, std::vector<uint8_t>& out) {
+    // Lossless_zstd::compress (Lossless_zstd.hpp:29-46): [size_t n][zstd level 3 frame], or the raw
+    // bytes when FALCONS_NOZSTD is set.
+    const Zstd& Z = zstd();
+    out.resize(8 + (getenv("FALCONS_NOZSTD") ? n : Z.compressBound(n)));
+    memcpy(out.data(), &n, 8);
+    if (getenv("FALCONS_NOZSTD")) {
+        memcpy(out.data() + 8, src, n);
+        return;
+    }
+    const size_t r = Z.compress(out.data() + 8, out.size() - 8, src, n, 3);
+    if (Z.isError(r)) throw std::runtime_error(std::string("Zstd compression failed: ") + Z.getErrorName(r));
+    out.resize(8 + r);
+}
+
+std::vector<uint8_t> zstd_unframe(const uint8_t* src, size_t n) {
+    if (!src || n < 8) throw ArgError("Truncated Falcons lossless payload");
+    uint64_t dl;
+    memcpy(&dl, src, 8);
+    const size_t pl = n - 8;
+    const bool raw = getenv("FALCONS_NOZSTD") != nullptr;
+    std::vector<uint8_t> out;
+    if (raw) {
+        if (dl > pl) throw ArgError("Truncated Falcons raw payload");
+        out.assign(src + 8, src + 8 + dl);
+        return out;
+    }
+    const Zstd& Z = zstd();
+    const unsigned long long fs = Z.getFrameContentSize(src + 8, pl);
+    if (fs == kZstdContentSizeError || (fs != kZstdContentSizeUnknown && fs != dl))
+        throw ArgError("Invalid Falcons Zstd payload size");
+    out.resize(dl ? dl : 1);
+    const size_t r = Z.decompress(out.data(), dl, src + 8, pl);
+    if (Z.isError(r) || r != dl) throw ArgError("Invalid or truncated Falcons Zstd payload");
+    out.resize(dl);
+    return out;
+}
+
+void check_supported(const fz_config& c, int dtype) {
+    if (dtype != FZ_FLOAT && dtype != FZ_DOUBLE) throw Unsupported("only float and double fields are on this path");
+    if (c.N < 1 || c.N > 3) throw Unsupported("4-D fields are not on this path");
+    if (c.openmp) throw Unsupported("the OpenMP stream format is not produced by this path");
+    if (c.quantbin_cnt != 65536) throw Unsupported("quantbinCnt other than 65536 is not on this path");
+}
+
+struct Timer {
+    hipStream_t st;
+    hipEvent_t a, b;
+    void start() { FZ_HIP(hipEventRecord(a, st)); }
+    double stop() {
+        FZ_HIP(hipEventRecord(b, st));
+        FZ_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        FZ_HIP(hipEventElapsedTime(&ms, a, b));
+        return ms;
+    }
+};
+
+// The compression pipeline on the GPU. d_data is device memory holding the field; it is
+// overwritten with the reconstructed values.
+void gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, const void* host_data, int simd,
+                  std::vector<uint8_t>& prebuf, fz_opts* o) {
+    hipStream_t st = ctx->stream;
+    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    const uint64_t n = c.num;
+    DevArena& ws = ctx->arena;
+    ws.reset();
+    const size_t need = n * es * 2 + n * 2 + n * 8 + (n * 3 + (1 << 20)) + select_escapes_scratch(n) +
+                        entropy_encode_workspace(n) + (8u << 20);
+    ws.reserve(need);
+    Timer tm{st, ctx->ev0, ctx->ev1};
+    double* eb_dev = ws.take<double>(1);
+    void* mm_scratch = ws.take<uint8_t>(minmax_scratch_bytes());
+    // error bound (calAbsErrorBound, Statistic.hpp:216-240)
+    double psnr_factor = 0;
+    if (c.eb_mode == 2) psnr_factor = std::pow(10, (c.psnr_eb + 10 * std::log10(1 - 2.0 / 3.0 * 0.99)) / -20);
+    if (c.eb_mode == 3) {
+        c.abs_eb = std::sqrt(3.0 / (double)n) * c.l2norm_eb;
+        c.eb_mode = 0;
+    }
+    tm.start();
+    if (c.eb_mode == 0) {
+        FZ_HIP(hipMemcpyAsync(eb_dev, &c.abs_eb, 8, hipMemcpyHostToDevice, st));
+    } else {
+        abs_error_bound(dtype, d_data, n, c.eb_mode, c.abs_eb, c.rel_eb, psnr_factor, mm_scratch, eb_dev, st);
+    }
+    double eb_host = c.abs_eb;
+    const bool need_host_eb = c.eb_mode != 0 || c.cmpr_algo == 1;
+    if (need_host_eb) {
+        FZ_HIP(hipMemcpyAsync(&eb_host, eb_dev, 8, hipMemcpyDeviceToHost, st));
+        FZ_HIP(hipStreamSynchronize(st));
+    }
+    c.abs_eb = eb_host;
+    c.eb_mode = 0;
+    if (c.abs_eb == 0) throw Unsupported("zero bound selects the lossless mode");  // handled by the caller
+    if (c.interp_anchor_stride < 0) {
+        static const int def[4] = {4096, 128, 32, 16};
+        c.interp_anchor_stride = def[c.N - 1];
+    }
+    if (c.cmpr_algo == 1) {
+        // ALGO_INTERP_LORENZO: parameter search on sampled blocks (FalconsAlgoInterp.hpp:273-507)
+        if (!host_data) throw Unsupported("ALGO_INTERP_LORENZO needs the field on the host");
+        const int r = tune_interp(c, dtype, host_data);
+        if (r == 2) throw Unsupported("ALGO_INTERP_LORENZO on 1-D data needs the Lorenzo/regression path");
+        c.cmpr_algo = 2;
+    }
+    if (c.cmpr_algo != 2) throw Unsupported("only the interpolation algorithms are on this path");
+    const Plan P = make_plan(c.N, c.dims, c.interp_anchor_stride, c.interp_algo, c.interp_direction);
+    std::vector<double> ratio;
+    const int rmode = ratio_table(c.interp_alpha, c.interp_beta, P.levels, ratio);
+    double* ratio_dev = ws.take<double>(ratio.size());
+    LevelEb* lv = ws.take<LevelEb>(std::max(P.levels, 1));
+    FZ_HIP(hipMemcpyAsync(ratio_dev, ratio.data(), ratio.size() * 8, hipMemcpyHostToDevice, st));
+    if (P.levels > 0) level_constants(eb_dev, ratio_dev, rmode, P.levels, lv, st);
+    int16_t* codes = ws.take<int16_t>(P.ncodes + 16);
+    void* escval = ws.take<uint8_t>((P.ncodes + 16) * es);
+    void* unpred = ws.take<uint8_t>((n + 16) * es);
+    uint64_t* esc_idx = ws.take<uint64_t>(P.ncodes + 1);
+    uint64_t* nesc_dev = ws.take<uint64_t>(1);
+    const size_t sel_bytes = select_escapes_scratch(P.ncodes);
+    void* sel_tmp = ws.take<uint8_t>(sel_bytes + 1);
+    if (P.anchor) interp_anchors(dtype, false, d_data, unpred, P.acnt, P.astr, st);
+    else interp_first(dtype, false, d_data, codes, escval, eb_dev, nullptr, 0, nullptr, st);
+    const int avx2 = simd ? 1 : 0;
+    for (const auto& p : P.passes) interp_pass(dtype, false, p, d_data, codes, escval, lv, avx2, nullptr, 0, nullptr, st);
+    select_escapes(codes, P.ncodes, esc_idx, nesc_dev, sel_tmp, sel_bytes, st);
+    gather_escapes(dtype, esc_idx, nesc_dev, P.ncodes, escval, static_cast<uint8_t*>(unpred) + P.nanchors * es, st);
+    uint8_t* enc = ws.take<uint8_t>(P.ncodes * 3 + (1 << 20));
+    EntropyStats es_stats;
+    const size_t enc_len = entropy_encode(codes, P.ncodes, enc, P.ncodes * 3 + (1 << 20), ws, st, &es_stats);
+    uint64_t nesc = 0;
+    FZ_HIP(hipMemcpyAsync(&nesc, nesc_dev, 8, hipMemcpyDeviceToHost, st));
+    const double tdev = tm.stop();
+    const uint64_t nun = P.nanchors + nesc;
+    // pre-zstd buffer: decomposition.save, nCodes, encoder.save + payload (FGC:61-83, IDH:241-252,
+    // LinearQuantizer.hpp:142-152)
+    const size_t head = 8 * c.N + 4 + 4 + 4 + 8 + 8 + 8 + 1 + 8 + 4 + 4;
+    prebuf.resize(head + nun * es + 8 + enc_len);
+    uint8_t* p = prebuf.data();
+    uint64_t max_dim = 1;
+    for (int i = 0; i < c.N; i++) { put<uint64_t>(p, c.dims[i]); max_dim = std::max<uint64_t>(max_dim, c.dims[i]); }
+    put<uint32_t>(p, (uint32_t)max_dim);
+    put<int32_t>(p, c.interp_algo);
+    put<int32_t>(p, c.interp_direction);
+    put<uint64_t>(p, P.anchor);
+    put<double>(p, c.interp_alpha);
+    put<double>(p, c.interp_beta);
+    put<uint8_t>(p, 2);
+    put<double>(p, c.abs_eb);
+    put<int32_t>(p, c.quantbin_cnt / 2);
+    put<int32_t>(p, (int32_t)nun);
+    FZ_HIP(hipMemcpyAsync(p, unpred, nun * es, hipMemcpyDeviceToHost, st));
+    p += nun * es;
+    put<uint64_t>(p, P.ncodes);
+    FZ_HIP(hipMemcpyAsync(p, enc, enc_len, hipMemcpyDeviceToHost, st));
+    FZ_HIP(hipStreamSynchronize(st));
+    if (o) {
+        o->t_device_ms = tdev;
+        o->ncodes = P.ncodes;
+        o->nunpred = nun;
+        o->prezstd_bytes = prebuf.size();
+        o->method = es_stats.method;
+        o->use_delta = es_stats.use_delta;
+    }
+}
+
+void set_err(fz_ctx* ctx, const char* m) {
+    if (ctx) ctx->err = m;
+}
+
+}  // namespace
+
+extern "C" void fz_opts_init(fz_opts* o) {
+    memset(o, 0, sizeof *o);
+    o->simd = 1;
+    o->copy_back = 1;
+}
+
+extern "C" int fz_ctx_create(int device, fz_ctx** out) {
+    try {
+        auto* c = new fz_ctx();
+        c->device = device;
+        FZ_HIP(hipSetDevice(device));
+        FZ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        FZ_HIP(hipEventCreate(&c->ev0));
+        FZ_HIP(hipEventCreate(&c->ev1));
+        *out = c;
+        return FZ_OK;
+    } catch (const std::exception& e) {
+        *out = nullptr;
+        fprintf(stderr, "fz_ctx_create: %s\n", e.what());
+        return FZ_ERR_GPU;
+    }
+}
+
+extern "C" void fz_ctx_destroy(fz_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    c->arena.release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char* fz_last_error(const fz_ctx* c) { return c ? c->err.c_str() : ""; }
+
+#define FZ_GUARD(...)                                                    \
+    try {                                                                 \
+        __VA_ARGS__                                                       \
+    } catch (const ArgError& e) {                                         \
+        set_err(ctx, e.what());                                           \
+        return FZ_ERR_INVALID_ARGUMENT;                                   \
+    } catch (const Unsupported& e) {                                      \
+        set_err(ctx, e.what());                                           \
+        return FZ_ERR_UNSUPPORTED;                                        \
+    } catch (const GpuError& e) {                                         \
+        set_err(ctx, e.what());                                           \
+        return FZ_ERR_GPU;                                                \
+    } catch (const std::exception& e) {                                   \
+        set_err(ctx, e.what());                                           \
+        return FZ_ERR_RUNTIME;                                            \
+    }
+
+extern "C" int fz_compress(fz_ctx* ctx, const fz_config* conf, int dtype, void* data, int data_on_device, uint8_t* out,
+                           size_t cap, size_t* out_len, fz_opts* opts) {
+    FZ_GUARD({
+        const double t0 = now_ms();
+        fz_opts local;
+        fz_opts_init(&local);
+        fz_opts* o = opts ? opts : &local;
+        FZ_HIP(hipSetDevice(ctx->device));
+        fz_config c = *conf;
+        if (cap < fz_compress_size_bound(&c, dtype)) throw ArgError("The buffer for compressed data is not large enough.");
+        check_supported(c, dtype);
+        const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+        const uint64_t n = c.num;
+        // Falcons_compress: the payload starts after the header size estimated from the caller's
+        // configuration; the final header is written at the front afterwards (falcons.hpp:59-96).
+        const size_t est = fz_config_size_est(&c);
+        memset(out, 0, est);
+        uint8_t* payload = out + est;
+        // device copy of the field
+        void* d_data = data;
+        void* d_owned = nullptr;
+        if (!data_on_device) {
+            FZ_HIP(hipMalloc(&d_owned, n * es));
+            FZ_HIP(hipMemcpyAsync(d_owned, data, n * es, hipMemcpyHostToDevice, ctx->stream));
+            d_data = d_owned;
+        }
+        struct Free {
+            void* p;
+            ~Free() { if (p) (void)hipFree(p); }
+        } guard{d_owned};
+        std::vector<uint8_t> host_copy;
+        const void* host_view = data_on_device ? nullptr : data;
+        if (c.cmpr_algo == 1 && data_on_device) {
+            host_copy.resize(n * es);
+            FZ_HIP(hipMemcpyAsync(host_copy.data(), data, n * es, hipMemcpyDeviceToHost, ctx->stream));
+            FZ_HIP(hipStreamSynchronize(ctx->stream));
+            host_view = host_copy.data();
+        }
+        size_t plen = 0;
+        bool lossless = c.cmpr_algo == 4;
+        std::vector<uint8_t> framed;
+        double tz = 0;
+        if (!lossless) {
+            // a zero absolute bound (after REL conversion) selects the lossless mode
+            // (FalconsDispatcher.hpp:29-31); an ABS bound of zero is known up front
+            if (c.eb_mode == 0 && c.abs_eb == 0) {
+                lossless = true;
+            } else {
+                std::vector<uint8_t> pre;
+                try {
+                    gpu_compress(ctx, c, dtype, d_data, host_view, o->simd, pre, o);
+                } catch (const Unsupported& u) {
+                    if (std::string(u.what()) != "zero bound selects the lossless mode") throw;
+                    lossless = true;
+                }
+                if (!lossless) {
+                    if (o->prezstd) {
+                        o->prezstd_len = pre.size();
+                        if (pre.size() <= o->prezstd_cap) memcpy(o->prezstd, pre.data(), pre.size());
+                    }
+                    const double tz0 = now_ms();
+                    zstd_frame(pre.data(), pre.size(), framed);
+                    tz += now_ms() - tz0;
+                    plen = framed.size();
+                }
+            }
+        }
+        if (!lossless && !data_on_device && o->copy_back) {
+            FZ_HIP(hipMemcpyAsync(data, d_data, n * es, hipMemcpyDeviceToHost, ctx->stream));
+            FZ_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        if (lossless) {
+            c.cmpr_algo = 4;
+            std::vector<uint8_t> src(n * es);
+            FZ_HIP(hipMemcpyAsync(src.data(), d_data, n * es, hipMemcpyDeviceToHost, ctx->stream));
+            FZ_HIP(hipStreamSynchronize(ctx->stream));
+            const double tz0 = now_ms();
+            zstd_frame(src.data(), src.size(), framed);
+            tz += now_ms() - tz0;
+            plen = framed.size();
+            if (c.eb_mode != 0) c.eb_mode = 0;
+        } else if ((double)(n * es) / 1.0 / (double)plen < 3) {
+            // ratio below 3: zstd of the reconstructed field may win (FalconsDispatcher.hpp:121-133)
+            std::vector<uint8_t> src(n * es), z;
+            FZ_HIP(hipMemcpyAsync(src.data(), d_data, n * es, hipMemcpyDeviceToHost, ctx->stream));
+            FZ_HIP(hipStreamSynchronize(ctx->stream));
+            const double tz0 = now_ms();
+            zstd_frame(src.data(), src.size(), z);
+            tz += now_ms() - tz0;
+            if (z.size() < plen && z.size() <= cap - est) {
+                c.cmpr_algo = 4;
+                framed.swap(z);
+                plen = framed.size();
+            }
+        }
+        if (plen > cap - est) throw ArgError("The buffer for compressed data is not large enough.");
+        memcpy(payload, framed.data(), plen);
+        const size_t hs = fz_config_save(&c, out);
+        *out_len = hs + plen;
+        o->t_zstd_ms = tz;
+        o->t_total_ms = now_ms() - t0;
+        return FZ_OK;
+    })
+}
+
+extern "C" int fz_decompress(fz_ctx* ctx, const uint8_t* cmp, size_t len, int dtype, void* out, size_t out_cap_elems,
+                             int out_on_device, fz_config* conf_out, fz_opts* opts) {
+    FZ_GUARD({
+        const double t0 = now_ms();
+        fz_opts local;
+        fz_opts_init(&local);
+        fz_opts* o = opts ? opts : &local;
+        FZ_HIP(hipSetDevice(ctx->device));
+        fz_config c;
+        const size_t hs = load_config(cmp, len, &c);
+        if (conf_out) *conf_out = c;
+        if (!out || c.num > out_cap_elems) throw ArgError("The decompression buffer is too small");
+        if (c.openmp) throw Unsupported("the OpenMP stream format is not produced by this path");
+        if (dtype != FZ_FLOAT && dtype != FZ_DOUBLE) throw Unsupported("only float and double fields are on this path");
+        const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+        const uint64_t n = c.num;
+        hipStream_t st = ctx->stream;
+        const double tz0 = now_ms();
+        std::vector<uint8_t> buf = zstd_unframe(cmp + hs, len - hs);
+        o->t_zstd_ms = now_ms() - tz0;
+        if (c.cmpr_algo == 4) {
+            if (buf.size() != n * es) throw std::runtime_error("Decompressed data size does not match the original data size");
+            FZ_HIP(hipMemcpyAsync(out, buf.data(), n * es, out_on_device ? hipMemcpyHostToDevice : hipMemcpyHostToHost, st));
+            FZ_HIP(hipStreamSynchronize(st));
+            o->t_total_ms = now_ms() - t0;
+            return FZ_OK;
+        }
+        if (c.cmpr_algo != 2) throw Unsupported("only the interpolation algorithms are on this path");
+        if (c.N > 3) throw Unsupported("4-D fields are not on this path");
+        // decomposition.load (IDH:256-267, LinearQuantizer.hpp:154-176)
+        const uint8_t* p = buf.data();
+        const uint8_t* bend = buf.data() + buf.size();
+        auto need = [&](size_t k) {
+            if ((size_t)(bend - p) < k) throw ArgError("Truncated Falcons payload");
+        };
+        uint64_t dims[4];
+        need(8 * c.N + 45);
+        for (int i = 0; i < c.N; i++) dims[i] = get<uint64_t>(p);
+        (void)get<uint32_t>(p);
+        const int32_t algo = get<int32_t>(p);
+        const int32_t dir = get<int32_t>(p);
+        const uint64_t anchor = get<uint64_t>(p);
+        const double alpha = get<double>(p), beta = get<double>(p);
+        if (get<uint8_t>(p) != 2) throw ArgError("LinearQuantizer uid mismatch");
+        const double eb = get<double>(p);
+        (void)get<int32_t>(p);
+        const int32_t nun = get<int32_t>(p);
+        need((size_t)nun * es + 8);
+        const uint8_t* unpred_h = p;
+        p += (size_t)nun * es;
+        const uint64_t ncodes = get<uint64_t>(p);
+        const uint8_t* enc_h = p;
+        const size_t enc_len = (size_t)(bend - p);
+        uint64_t pdims[4];
+        for (int i = 0; i < c.N; i++) pdims[i] = dims[i];
+        const Plan P = make_plan(c.N, pdims, (long)anchor, algo, dir);
+        if (P.ncodes != ncodes) throw ArgError("code count does not match the field shape");
+        DevArena& ws = ctx->arena;
+        ws.reset();
+        ws.reserve(n * es + ncodes * 2 + ncodes * 8 + (size_t)nun * es + enc_len * 2 + select_escapes_scratch(ncodes) +
+                   entropy_decode_workspace(ncodes, enc_len) + (16u << 20));
+        void* d_out = out_on_device ? out : ws.take<uint8_t>(n * es);
+        uint8_t* d_enc = ws.take<uint8_t>(enc_len + 16);
+        void* d_unpred = ws.take<uint8_t>((size_t)nun * es + 16);
+        FZ_HIP(hipMemcpyAsync(d_enc, enc_h, enc_len, hipMemcpyHostToDevice, st));
+        if (nun) FZ_HIP(hipMemcpyAsync(d_unpred, unpred_h, (size_t)nun * es, hipMemcpyHostToDevice, st));
+        double* eb_dev = ws.take<double>(1);
+        FZ_HIP(hipMemcpyAsync(eb_dev, &eb, 8, hipMemcpyHostToDevice, st));
+        std::vector<double> ratio;
+        const int rmode = ratio_table(alpha, beta, P.levels, ratio);
+        double* ratio_dev = ws.take<double>(ratio.size());
+        FZ_HIP(hipMemcpyAsync(ratio_dev, ratio.data(), ratio.size() * 8, hipMemcpyHostToDevice, st));
+        LevelEb* lv = ws.take<LevelEb>(std::max(P.levels, 1));
+        int16_t* codes = ws.take<int16_t>(ncodes + 16);
+        uint64_t* esc_idx = ws.take<uint64_t>(ncodes + 1);
+        uint64_t* nesc_dev = ws.take<uint64_t>(1);
+        const size_t sel_bytes = select_escapes_scratch(ncodes);
+        void* sel_tmp = ws.take<uint8_t>(sel_bytes + 1);
+        FZ_HIP(hipStreamSynchronize(st));
+        Timer tm{st, ctx->ev0, ctx->ev1};
+        tm.start();
+        entropy_decode(enc_h, d_enc, enc_len, ncodes, codes, ws, st);
+        select_escapes(codes, ncodes, esc_idx, nesc_dev, sel_tmp, sel_bytes, st);
+        uint64_t nesc = 0;
+        FZ_HIP(hipMemcpyAsync(&nesc, nesc_dev, 8, hipMemcpyDeviceToHost, st));
+        FZ_HIP(hipStreamSynchronize(st));
+        if (P.nanchors + nesc != (uint64_t)nun) throw ArgError("unpredictable count does not match the code stream");
+        if (P.levels > 0) level_constants(eb_dev, ratio_dev, rmode, P.levels, lv, st);
+        const void* esc_vals = static_cast<const uint8_t*>(d_unpred) + P.nanchors * es;
+        if (P.anchor) interp_anchors(dtype, true, d_out, d_unpred, P.acnt, P.astr, st);
+        else interp_first(dtype, true, d_out, codes, nullptr, eb_dev, esc_idx, nesc, esc_vals, st);
+        for (const auto& ps : P.passes)
+            interp_pass(dtype, true, ps, d_out, codes, nullptr, lv, 1, esc_idx, nesc, esc_vals, st);
+        o->t_device_ms = tm.stop();
+        if (!out_on_device) {
+            FZ_HIP(hipMemcpyAsync(out, d_out, n * es, hipMemcpyDeviceToHost, st));
+            FZ_HIP(hipStreamSynchronize(st));
+        }
+        o->ncodes = ncodes;
+        o->nunpred = (uint64_t)nun;
+        o->t_total_ms = now_ms() - t0;
+        return FZ_OK;
+    })
+}

@@ -1,0 +1,1087 @@
+// Entropy coder of the Falcons interpolation path on gfx950: the RleFseEncoder<int16_t> stream
+// format (include/Falcons/encoder/RleFseEncoder.hpp) produced and consumed on the GPU.
+//
+//  method 0: runs of the zero code and the other codes as Elias-gamma classes (bytes) plus
+//            MSB-first mantissa bits; the class bytes go through FSE in 64 K blocks.
+//  method 1: order-1 tANS: per segment, 4 chunks, each one tANS chain whose table is switched per
+//            symbol by the context of the previous code; codes with |code| >= 128 escape to a
+//            gamma/FSE side stream.
+//
+// Data-parallel steps (sampling, delta, escape selection, context histograms, run/value extraction,
+// mantissa bit scatter, block framing, run expansion, prefix sums) run across the whole chip. The
+// tANS chains themselves are sequential state machines; each chain is run by one lane here and all
+// chains of a call run concurrently.
+#include <hipcub/hipcub.hpp>
+
+#include "fz_common.h"
+#include "fz_fse.h"
+#include "fz_kernels.h"
+
+namespace fz {
+
+constexpr int kO1Ctx = 15;               // RleFseEncoder.hpp:344
+constexpr int kO1Sym = 256;              // :347 (255 = escape to the side stream)
+constexpr unsigned kO1Log = 13;          // :345
+constexpr int kO1K = 4;                  // :479
+constexpr uint64_t kO1Seg = 8ull << 20;  // :577
+constexpr uint32_t kFblk = 65536;        // :75
+
+// Context bucket of the previous code (RleFseEncoder.hpp:354-385 in closed form): 0 for 0,
+// magnitudes 1..4 one bucket each, then 5..7, 8..11 and >= 12; times two for the sign.
+__host__ __device__ inline int o1_ctx(int code) {
+    if (code == 0) return 0;
+    const int sgn = code < 0;
+    const int a = sgn ? -code : code;
+    int m;
+    if (a <= 4) m = a - 1;
+    else if (a <= 7) m = 4;
+    else if (a <= 11) m = 5;
+    else m = 6;
+    return 1 + 2 * m + sgn;
+}
+// Zigzag symbol clamped to 255 (escape).
+__host__ __device__ inline int o1_sym(int code) {
+    const uint32_t zz = ((uint32_t)code << 1) ^ (uint32_t)(code >> 31);
+    return zz < 255 ? (int)zz : 255;
+}
+
+// ---------------------------------------------------------------- small device helpers
+struct BitW {  // LSB-first FSE bitstream writer (zfse/bitstream.h)
+    uint8_t* buf;
+    uint64_t pos;
+    uint64_t acc;
+    uint32_t n;
+};
+__device__ inline void bw_add(BitW& w, uint64_t v, uint32_t nb) {
+    if (!nb) return;
+    w.acc |= (v & ((1ull << nb) - 1)) << w.n;
+    w.n += nb;
+    while (w.n >= 8) {
+        w.buf[w.pos++] = (uint8_t)w.acc;
+        w.acc >>= 8;
+        w.n -= 8;
+    }
+}
+__device__ inline uint64_t bw_close(BitW& w) {
+    bw_add(w, 1, 1);
+    if (w.n) {
+        w.buf[w.pos++] = (uint8_t)w.acc;
+        w.acc = 0;
+        w.n = 0;
+    }
+    return w.pos;
+}
+__device__ inline void fse_encode(BitW& w, uint32_t& state, const CTable& ct, unsigned sym) {
+    const uint32_t nbOut = (uint32_t)(((uint64_t)state + ct.deltaNbBits[sym]) >> 16);
+    bw_add(w, state, nbOut);
+    state = ct.stateTable[(state >> nbOut) + ct.deltaFindState[sym]];
+}
+
+struct BitR {  // backward reader of such a stream; bits below the start read as zero
+    const uint8_t* buf;
+    int64_t pos;
+};
+__device__ inline bool br_init(BitR& r, const uint8_t* buf, uint64_t size) {
+    r.buf = buf;
+    if (size < 1 || buf[size - 1] == 0) { r.pos = 0; return false; }
+    r.pos = (int64_t)(size - 1) * 8 + (int64_t)highbit32(buf[size - 1]);
+    return true;
+}
+__device__ inline uint32_t br_read(BitR& r, uint32_t nb) {
+    if (!nb) return 0;
+    const int64_t lo = r.pos - nb;
+    r.pos = lo;
+    uint32_t v = 0;
+    const int64_t b0 = lo >> 3;  // arithmetic shift: floor for negatives
+    const int sh = (int)(lo - b0 * 8);
+    uint64_t w = 0;
+    for (int j = 0; j < 4; j++) {
+        const int64_t b = b0 + j;
+        if (b >= 0) w |= (uint64_t)r.buf[b] << (8 * j);
+    }
+    v = (uint32_t)((w >> sh) & ((1ull << nb) - 1));
+    return v;
+}
+
+// ---------------------------------------------------------------- preprocess (RleFse:850-895)
+struct PreCounts {
+    unsigned long long z, zd, cnt, cntd;
+};
+__global__ void k_preprocess(const int16_t* bins, uint64_t n, uint64_t NW, uint64_t WL, uint64_t step, PreCounts* out) {
+    __shared__ unsigned long long sz[256], szd[256], sc[256], scd[256];
+    const uint64_t w = threadIdx.x;
+    unsigned long long z = 0, zd = 0, c = 0, cd = 0;
+    // windows are taken in order and the scan stops at the first one that does not fit
+    if (w < NW) {
+        bool fits = true;
+        for (uint64_t v = 0; v <= w; v++)
+            if (v * step + WL > n) { fits = false; break; }
+        if (fits) {
+            const uint64_t off = w * step;
+            int16_t prev = bins[off];
+            for (uint64_t t = 0; t < WL; t++) {
+                const int16_t x = bins[off + t];
+                z += (x == 0);
+                if (t > 0 && x == prev) zd++;
+                prev = x;
+            }
+            c = WL;
+            cd = WL - 1;
+        }
+    }
+    sz[threadIdx.x] = z; szd[threadIdx.x] = zd; sc[threadIdx.x] = c; scd[threadIdx.x] = cd;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        PreCounts r{0, 0, 0, 0};
+        for (int i = 0; i < 256; i++) { r.z += sz[i]; r.zd += szd[i]; r.cnt += sc[i]; r.cntd += scd[i]; }
+        *out = r;
+    }
+}
+
+__global__ void k_delta(const int16_t* __restrict__ in, int16_t* __restrict__ out, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = i ? (int16_t)(in[i] - in[i - 1]) : in[0];
+}
+
+static inline unsigned nblk(uint64_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------- FSE block (workgroup-wide)
+// FSE_compress of one block of at most 64 K bytes (fse_compress.c:632-694), run by one workgroup:
+// the histogram is shared, the table build and the two-state encode run on lane 0.
+struct FseLds {
+    uint32_t count[256];
+    int16_t norm[256];
+    uint8_t spread[1u << kFseMaxTableLog];
+    CTable ct;
+    uint64_t result;
+};
+
+__device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_t* dst, FseLds& L) {
+    for (int s = threadIdx.x; s < 256; s += blockDim.x) L.count[s] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&L.count[src[i]], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t r;
+        if (n <= 1) {
+            r = 0;
+        } else {
+            unsigned maxSym = 255;
+            while (!L.count[maxSym]) maxSym--;
+            uint32_t maxCount = 0;
+            for (unsigned s = 0; s <= maxSym; s++) maxCount = L.count[s] > maxCount ? L.count[s] : maxCount;
+            if (maxCount == n) r = 1;
+            else if (maxCount == 1 || maxCount < (n >> 7)) r = 0;
+            else {
+                const unsigned tl = optimal_table_log(kFseDefaultTableLog, n, maxSym);
+                if (normalize_count(L.norm, tl, L.count, n, maxSym) == kFseErr) {
+                    r = kFseErr;
+                } else {
+                    const uint64_t nc = write_ncount(dst, L.norm, maxSym, tl);
+                    build_ctable(&L.ct, L.norm, maxSym, tl, L.spread);
+                    BitW w{dst + nc, 0, 0, 0};
+                    uint32_t ip = n, s1, s2;
+                    if (n & 1) {
+                        s1 = init_state2(&L.ct, src[--ip]);
+                        s2 = init_state2(&L.ct, src[--ip]);
+                        fse_encode(w, s1, L.ct, src[--ip]);
+                    } else {
+                        s2 = init_state2(&L.ct, src[--ip]);
+                        s1 = init_state2(&L.ct, src[--ip]);
+                    }
+                    if ((n - 2) & 2) {
+                        fse_encode(w, s2, L.ct, src[--ip]);
+                        fse_encode(w, s1, L.ct, src[--ip]);
+                    }
+                    while (ip > 0) {
+                        fse_encode(w, s2, L.ct, src[--ip]);
+                        fse_encode(w, s1, L.ct, src[--ip]);
+                        fse_encode(w, s2, L.ct, src[--ip]);
+                        fse_encode(w, s1, L.ct, src[--ip]);
+                    }
+                    bw_add(w, s2, tl);
+                    bw_add(w, s1, tl);
+                    const uint64_t bs = bw_close(w);
+                    const uint64_t total = nc + bs;
+                    r = (total >= n - 1) ? 0 : total;
+                }
+            }
+        }
+        L.result = r;
+    }
+    __syncthreads();
+    return L.result;
+}
+
+// Frames one class-byte block as in RleFse:150-176: [0][raw] | [2][sym] | [1][u32 len][FSE bytes].
+// tmp holds the FSE attempt; returns the framed length written to out.
+__device__ uint64_t frame_block_wg(const uint8_t* src, uint32_t len, uint8_t* tmp, uint8_t* out, FseLds& L) {
+    const uint64_t r = fse_block_compress_wg(src, len, tmp, L);
+    uint64_t sz;
+    if (r == kFseErr || r == 0 || r >= len) {
+        if (threadIdx.x == 0) out[0] = 0;
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) out[1 + i] = src[i];
+        sz = 1 + len;
+    } else if (r == 1) {
+        if (threadIdx.x == 0) { out[0] = 2; out[1] = src[0]; }
+        sz = 2;
+    } else {
+        if (threadIdx.x == 0) {
+            out[0] = 1;
+            const uint32_t rr = (uint32_t)r;
+            out[1] = (uint8_t)rr; out[2] = (uint8_t)(rr >> 8); out[3] = (uint8_t)(rr >> 16); out[4] = (uint8_t)(rr >> 24);
+        }
+        for (uint32_t i = threadIdx.x; i < r; i += blockDim.x) out[5 + i] = tmp[i];
+        sz = 5 + r;
+    }
+    __syncthreads();
+    return sz;
+}
+
+// One workgroup per 64 K block of a class-byte array; framed blocks land in fixed slots.
+__global__ void __launch_bounds__(256) k_fse_blocks(const uint8_t* src, uint64_t n, uint8_t* tmp, uint8_t* slots,
+                                                    uint64_t* sizes) {
+    __shared__ FseLds L;
+    const uint64_t b = blockIdx.x;
+    const uint64_t off = b * kFblk;
+    const uint32_t len = (uint32_t)min<uint64_t>(kFblk, n - off);
+    const uint64_t sz = frame_block_wg(src + off, len, tmp + b * (kFblk + 1024), slots + b * (kFblk + 1024), L);
+    if (threadIdx.x == 0) sizes[b] = sz;
+}
+
+// Copies variable-size slots to their packed offsets (exclusive prefix of sizes, computed by the host).
+__global__ void k_pack_slots(const uint8_t* slots, uint64_t slot_stride, const uint64_t* sizes, const uint64_t* offs,
+                             uint8_t* out) {
+    const uint64_t b = blockIdx.y;
+    const uint64_t sz = sizes[b];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sz; i += (uint64_t)gridDim.x * blockDim.x)
+        out[offs[b] + i] = slots[b * slot_stride + i];
+}
+
+// FSE_decompress of one block (fse_decompress.c:254-282) by lane 0 of a workgroup.
+struct FseDecLds {
+    int16_t norm[256];
+    uint8_t spread[1u << kFseMaxTableLog];
+    DEntry dt[1u << kFseMaxTableLog];
+};
+__device__ bool fse_block_decompress(const uint8_t* src, uint64_t srcSize, uint8_t* dst, uint32_t n, FseDecLds& L) {
+    unsigned maxSym = 255, tl = 0;
+    const uint64_t hl = read_ncount(L.norm, &maxSym, &tl, src, srcSize);
+    if (hl == kFseErr) return false;
+    if (!build_dtable(L.dt, L.norm, maxSym, tl, L.spread)) return false;
+    BitR r;
+    if (!br_init(r, src + hl, srcSize - hl)) return false;
+    uint32_t st[2];
+    st[0] = br_read(r, tl);
+    st[1] = br_read(r, tl);
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t& s = st[i & 1];
+        const DEntry e = L.dt[s];
+        dst[i] = e.symbol;
+        s = e.newState + br_read(r, e.nbBits);
+    }
+    return true;
+}
+
+// Decodes framed class-byte blocks; block offsets were found by the host.
+__global__ void k_fse_dec_blocks(const uint8_t* in, const uint64_t* offs, uint64_t n, uint8_t* out, int* err) {
+    __shared__ FseDecLds L;
+    const uint64_t b = blockIdx.x;
+    const uint64_t o = offs[b];
+    const uint32_t len = (uint32_t)min<uint64_t>(kFblk, n - b * kFblk);
+    const uint8_t flag = in[o];
+    uint8_t* dst = out + b * kFblk;
+    if (flag == 0) {
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = in[o + 1 + i];
+    } else if (flag == 2) {
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = in[o + 1];
+    } else if (threadIdx.x == 0) {
+        const uint32_t rr = (uint32_t)in[o + 1] | ((uint32_t)in[o + 2] << 8) | ((uint32_t)in[o + 3] << 16) |
+                            ((uint32_t)in[o + 4] << 24);
+        if (!fse_block_decompress(in + o + 5, rr, dst, len, L)) *err = 1;
+    }
+}
+
+// ---------------------------------------------------------------- method 0 (RleFse:928-982)
+struct NonZero {
+    const int16_t* b;
+    __device__ __forceinline__ bool operator()(const uint64_t& i) const { return b[i] != 0; }
+};
+
+__device__ inline int floor_log2_64(uint64_t v) { return 63 - __clzll((long long)v); }
+
+// Run j precedes value j; the last run exists when the stream ends on the zero code.
+__global__ void k_runs_values(const int16_t* bins, uint64_t n, const uint64_t* pos, uint64_t nv, uint64_t nr,
+                              uint8_t* rcl, uint8_t* vcl, uint64_t* rbits, uint64_t* vbits) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nr) {
+        const uint64_t start = j ? pos[j - 1] + 1 : 0;
+        const uint64_t end = j < nv ? pos[j] : n;
+        const uint64_t L = end - start;
+        const int k = floor_log2_64(L + 1);
+        rcl[j] = (uint8_t)k;
+        rbits[j] = (uint64_t)k;
+    }
+    if (j < nv) {
+        const int32_t f = bins[pos[j]];
+        const uint32_t zz = ((uint32_t)f << 1) ^ (uint32_t)(f >> 31);
+        const int k = floor_log2_64((uint64_t)zz + 1);
+        vcl[j] = (uint8_t)k;
+        vbits[j] = (uint64_t)k;
+    }
+}
+
+// MSB-first mantissa field (low k bits of v) at bit offset off of a big-endian bit string.
+__device__ inline void put_msb_field(uint32_t* words, uint64_t off, uint64_t v, int k) {
+    if (k <= 0) return;
+    // bit p of the stream lives in byte p/8 at bit 7 - p%8; bytes map to little-endian words
+    uint64_t bit = off;
+    int left = k;
+    while (left > 0) {
+        const uint64_t byte = bit >> 3;
+        const int inb = (int)(bit & 7);
+        const int take = min(left, 8 - inb);
+        const uint32_t chunk = (uint32_t)((v >> (left - take)) & ((1u << take) - 1));
+        const uint32_t bits8 = chunk << (8 - inb - take);
+        atomicOr(&words[byte >> 2], bits8 << (8 * (byte & 3)));
+        left -= take;
+        bit += take;
+    }
+}
+
+__global__ void k_scatter_mantissas(const int16_t* bins, uint64_t n, const uint64_t* pos, uint64_t nv, uint64_t nr,
+                                    const uint64_t* roff, const uint64_t* voff, const uint8_t* rcl, const uint8_t* vcl,
+                                    uint32_t* rman, uint32_t* vman) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nr) {
+        const uint64_t start = j ? pos[j - 1] + 1 : 0;
+        const uint64_t end = j < nv ? pos[j] : n;
+        put_msb_field(rman, roff[j], end - start + 1, rcl[j]);
+    }
+    if (j < nv) {
+        const int32_t f = bins[pos[j]];
+        const uint32_t zz = ((uint32_t)f << 1) ^ (uint32_t)(f >> 31);
+        put_msb_field(vman, voff[j], (uint64_t)zz + 1, vcl[j]);
+    }
+}
+
+// ---------------------------------------------------------------- method 1 (RleFse:582-751)
+struct IsO1Escape {
+    const int16_t* b;
+    __device__ __forceinline__ bool operator()(const uint64_t& i) const { return o1_sym(b[i]) == 255; }
+};
+
+// Exact per-context histogram of a segment (N < 2^20): the context resets to that of 0 at each
+// chunk start (RleFse:611-628).
+__global__ void __launch_bounds__(256) k_o1_hist_exact(const int16_t* bins, uint64_t n, uint64_t cs, uint32_t* cnt) {
+    __shared__ uint32_t h[kO1Ctx * kO1Sym];
+    for (int i = threadIdx.x; i < kO1Ctx * kO1Sym; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const int c = (i % cs == 0) ? 0 : o1_ctx(bins[i - 1]);
+        atomicAdd(&h[c * kO1Sym + o1_sym(bins[i])], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kO1Ctx * kO1Sym; i += blockDim.x)
+        if (h[i]) atomicAdd(&cnt[i], h[i]);
+}
+
+// Windowed sample histogram (RleFse:641-658): 512-code windows every `step` codes, the context of
+// each code taken from its true predecessor (0 at the segment start).
+__global__ void __launch_bounds__(256) k_o1_hist_sample(const int16_t* bins, uint64_t n, uint64_t step, uint64_t wlen,
+                                                        uint32_t* cnt) {
+    __shared__ uint32_t h[kO1Ctx * kO1Sym];
+    for (int i = threadIdx.x; i < kO1Ctx * kO1Sym; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const uint64_t w = (uint64_t)blockIdx.x * step;
+    if (w < n) {
+        const uint64_t we = min(w + wlen, n);
+        for (uint64_t i = w + threadIdx.x; i < we; i += blockDim.x) {
+            const int c = i ? o1_ctx(bins[i - 1]) : 0;
+            atomicAdd(&h[c * kO1Sym + o1_sym(bins[i])], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kO1Ctx * kO1Sym; i += blockDim.x)
+        if (h[i]) atomicAdd(&cnt[i], h[i]);
+}
+
+__global__ void k_add_one(uint32_t* cnt, int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) cnt[i] += 1;
+}
+
+// o1_normalize (RleFse:410-477) and FSE_buildCTable for each context; one workgroup per context.
+__global__ void k_o1_tables(const uint32_t* cnt, uint16_t* nf, CTable* ct, uint8_t* spread_scratch) {
+    const int c = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    const uint32_t* h = cnt + c * kO1Sym;
+    uint16_t* f = nf + c * kO1Sym;
+    uint64_t total = 0;
+    int nonzero = 0;
+    for (int s = 0; s < kO1Sym; s++) { total += h[s]; nonzero += h[s] != 0; }
+    const int M = 1 << kO1Log;
+    bool done = false;
+    int16_t norm[kO1Sym];
+    if (total == 0) {
+        for (int s = 0; s < kO1Sym; s++) f[s] = 0;
+        f[0] = (uint16_t)M;
+        done = true;
+    } else if (nonzero > 1) {
+        if (normalize_count(norm, kO1Log, h, total, kO1Sym - 1) != kFseErr) {
+            for (int s = 0; s < kO1Sym; s++) f[s] = (uint16_t)norm[s];
+            done = true;
+        }
+    }
+    if (!done) {
+        int sum = 0, maxf = 0, argmax = 0;
+        for (int s = 0; s < kO1Sym; s++) {
+            if (h[s]) {
+                int v = (int)((uint64_t)h[s] * (uint64_t)M / total);
+                if (v == 0) v = 1;
+                f[s] = (uint16_t)v;
+                sum += v;
+                if (v > maxf) { maxf = v; argmax = s; }
+            } else {
+                f[s] = 0;
+            }
+        }
+        int e = M - sum;
+        if (e > 0) {
+            f[argmax] = (uint16_t)(f[argmax] + e);
+        } else if (e < 0) {
+            e = -e;
+            int take = (maxf - 1 < e) ? (maxf - 1) : e;
+            f[argmax] = (uint16_t)(f[argmax] - take);
+            e -= take;
+            while (e > 0) {
+                int mi = -1;
+                uint32_t mv = 1;
+                for (int s = 0; s < kO1Sym; s++)
+                    if (f[s] > mv) { mv = f[s]; mi = s; }
+                if (mi < 0) break;
+                take = (int)f[mi] - 1;
+                if (take > e) take = e;
+                f[mi] = (uint16_t)(f[mi] - take);
+                e -= take;
+            }
+        }
+    }
+    for (int s = 0; s < kO1Sym; s++) norm[s] = (int16_t)f[s];
+    build_ctable(&ct[c], norm, kO1Sym - 1, kO1Log, spread_scratch + (size_t)c * (1u << kO1Log));
+}
+
+// One tANS chain (one chunk of a segment), encoded backwards with a per-symbol context switch
+// (RleFse:681-727). chains[] = {segment lo, chunk lo, chunk hi}.
+__global__ void k_o1_encode(const int16_t* bins, const uint64_t* chains, int nchains, const CTable* ct_all,
+                            uint8_t* bufs, uint64_t buf_stride, uint64_t* blen) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nchains) return;
+    const uint64_t seg = chains[4 * q + 3];
+    const CTable* ct = ct_all + seg * kO1Ctx;
+    const uint64_t lo = chains[4 * q + 1], hi = chains[4 * q + 2];
+    if (hi <= lo) { blen[q] = 0; return; }
+    BitW w{bufs + (uint64_t)q * buf_stride, 0, 0, 0};
+    uint64_t idx = hi - 1;
+    int prev = (idx > lo) ? bins[idx - 1] : 0;
+    uint32_t st = init_state2(&ct[o1_ctx(prev)], o1_sym(bins[idx]));
+    for (uint64_t i = hi - 1; i-- > lo;) {
+        const int pc = (i > lo) ? bins[i - 1] : 0;
+        fse_encode(w, st, ct[o1_ctx(pc)], o1_sym(bins[i]));
+    }
+    bw_add(w, st, kO1Log);
+    blen[q] = bw_close(w);
+}
+
+// Side stream of escaped codes (RleFse:483-532), one workgroup per segment.
+__global__ void __launch_bounds__(256) k_o1_ovf(const int16_t* ovf_all, const uint64_t* ovf_off, uint8_t* vcl_buf,
+                                                uint8_t* vman_buf, uint8_t* tmp_buf, uint8_t* out_buf, uint64_t stride,
+                                                uint64_t* out_len) {
+    __shared__ FseLds L;
+    __shared__ uint64_t s_vbp, s_svcl;
+    const int g = blockIdx.x;
+    const uint64_t o = ovf_off[g], n = ovf_off[g + 1] - o;
+    const int16_t* ovf = ovf_all + o;
+    uint8_t* out = out_buf + g * stride;
+    uint8_t* vcl = vcl_buf + g * stride;
+    uint8_t* vman = vman_buf + g * stride;
+    uint8_t* tmp = tmp_buf + g * stride;
+    if (n == 0) {
+        if (threadIdx.x == 0) { out[0] = 0; out_len[g] = 1; }
+        return;
+    }
+    if (threadIdx.x == 0) {
+        uint64_t bp = 0, acc = 0;
+        int an = 0;
+        for (uint64_t i = 0; i < n; i++) {
+            const int32_t f = ovf[i];
+            const uint32_t zz = ((uint32_t)f << 1) ^ (uint32_t)(f >> 31);
+            const uint64_t vv = (uint64_t)zz + 1;
+            const int vk = floor_log2_64(vv);
+            vcl[i] = (uint8_t)vk;
+            acc = (acc << vk) | (vv & ((1ull << vk) - 1));
+            an += vk;
+            while (an >= 8) { an -= 8; vman[bp++] = (uint8_t)(acc >> an); }
+        }
+        if (an > 0) vman[bp++] = (uint8_t)(acc << (8 - an));
+        s_vbp = bp;
+    }
+    __syncthreads();
+    // class bytes through the framed FSE blocks, written after the 17-byte head
+    uint64_t p = 0;
+    uint8_t* body = out + 17;
+    for (uint64_t off = 0; off < n; off += kFblk) {
+        const uint32_t len = (uint32_t)min<uint64_t>(kFblk, n - off);
+        p += frame_block_wg(vcl + off, len, tmp, body + p, L);
+    }
+    if (threadIdx.x == 0) s_svcl = p;
+    __syncthreads();
+    const uint64_t vbp = s_vbp, svcl = s_svcl;
+    const uint64_t raw = n * 2, cm = svcl + vbp + 16;
+    if (cm >= raw) {
+        if (threadIdx.x == 0) out[0] = 0;
+        const uint8_t* src = reinterpret_cast<const uint8_t*>(ovf);
+        for (uint64_t i = threadIdx.x; i < raw; i += blockDim.x) out[1 + i] = src[i];
+        if (threadIdx.x == 0) out_len[g] = 1 + raw;
+    } else {
+        if (threadIdx.x == 0) {
+            out[0] = 1;
+            for (int b = 0; b < 8; b++) { out[1 + b] = (uint8_t)(svcl >> (8 * b)); out[9 + b] = (uint8_t)(vbp >> (8 * b)); }
+        }
+        for (uint64_t i = threadIdx.x; i < vbp; i += blockDim.x) body[svcl + i] = vman[i];
+        if (threadIdx.x == 0) out_len[g] = 17 + svcl + vbp;
+    }
+}
+
+__global__ void k_gather_i16(const int16_t* src, const uint64_t* idx, uint64_t n, int16_t* out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = src[idx[i]];
+}
+
+// ---------------------------------------------------------------- method 1 decode
+// nf may sit at any byte offset of the stream, so it is read byte by byte.
+__global__ void k_o1_dtables(const uint8_t* nf, DEntry* dt, uint8_t* spread_scratch, int* err) {
+    const int c = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    int16_t norm[kO1Sym];
+    for (int s = 0; s < kO1Sym; s++) {
+        const uint8_t* b = nf + 2 * (c * kO1Sym + s);
+        norm[s] = (int16_t)(uint16_t)(b[0] | (b[1] << 8));
+    }
+    if (!build_dtable(dt + (size_t)c * (1u << kO1Log), norm, kO1Sym - 1, kO1Log,
+                      spread_scratch + (size_t)c * (1u << kO1Log)))
+        *err = 1;
+}
+
+// chains[] = {stream offset, stream bytes, out lo, out hi, ovf offset, dtable set}
+__global__ void k_o1_decode(const uint8_t* in, const uint64_t* chains, int nchains, const DEntry* dt_all,
+                            const int16_t* ovf, int16_t* out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nchains) return;
+    const uint64_t* c = chains + 6 * q;
+    const uint64_t lo = c[2], hi = c[3];
+    if (hi <= lo || c[1] == 0) return;
+    const DEntry* dt = dt_all + c[5] * kO1Ctx * (1u << kO1Log);
+    BitR r;
+    br_init(r, in + c[0], c[1]);
+    uint32_t st = br_read(r, kO1Log);
+    uint64_t ovi = c[4];
+    int prev = 0;
+    for (uint64_t i = lo; i < hi; i++) {
+        const DEntry e = dt[(size_t)o1_ctx(prev) * (1u << kO1Log) + st];
+        st = e.newState + br_read(r, e.nbBits);
+        const unsigned s = e.symbol;
+        const int code = (s == 255) ? ovf[ovi++] : (int)((s >> 1) ^ (~(s & 1) + 1));
+        out[i] = (int16_t)code;
+        prev = code;
+    }
+}
+
+__global__ void k_ovf_dec(const uint8_t* vcl, const uint8_t* vman, uint64_t n, int16_t* ovf) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t rp = 0, acc = 0;
+    int an = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        const int vk = vcl[i];
+        while (an < vk) { acc = (acc << 8) | vman[rp++]; an += 8; }
+        an -= vk;
+        const uint64_t vv = (1ull << vk) | ((acc >> an) & ((1ull << vk) - 1));
+        const uint32_t zz = (uint32_t)(vv - 1);
+        ovf[i] = (int16_t)(int32_t)((zz >> 1) ^ (~(zz & 1) + 1));
+    }
+}
+
+// ---------------------------------------------------------------- method 0 decode
+__device__ inline uint64_t get_msb_field(const uint8_t* buf, uint64_t off, int k) {
+    uint64_t v = 0;
+    for (int j = 0; j < k; j++) {
+        const uint64_t b = off + j;
+        v = (v << 1) | ((buf[b >> 3] >> (7 - (b & 7))) & 1u);
+    }
+    return v;
+}
+__global__ void k_runs_decode(const uint8_t* rcl, const uint64_t* roff, const uint8_t* rman, uint64_t nr,
+                              uint64_t* runlen) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nr) return;
+    const int k = rcl[j];
+    runlen[j] = ((1ull << k) | get_msb_field(rman, roff[j], k)) - 1;
+}
+__global__ void k_values_place(const uint8_t* vcl, const uint64_t* voff, const uint8_t* vman, uint64_t nv,
+                               const uint64_t* runsum, uint64_t n, int16_t* out) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nv) return;
+    const int k = vcl[j];
+    const uint64_t vv = (1ull << k) | get_msb_field(vman, voff[j], k);
+    const uint32_t zz = (uint32_t)(vv - 1);
+    const int32_t f = (int32_t)((zz >> 1) ^ (~(zz & 1) + 1));
+    const uint64_t p = runsum[j] + j;  // zeros before value j plus the j values before it
+    if (p < n) out[p] = (int16_t)f;
+}
+
+struct ToU64 {
+    __device__ __forceinline__ uint64_t operator()(const uint8_t& x) const { return x; }
+};
+
+struct Add16 {
+    __device__ __forceinline__ int16_t operator()(const int16_t& a, const int16_t& b) const { return (int16_t)(a + b); }
+};
+
+// ---------------------------------------------------------------- host orchestration
+static void sync(hipStream_t st) { FZ_HIP(hipStreamSynchronize(st)); }
+template <typename T>
+static void d2h(T* h, const T* d, size_t n, hipStream_t st) {
+    FZ_HIP(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, st));
+    sync(st);
+}
+template <typename T>
+static void h2d(T* d, const T* h, size_t n, hipStream_t st) {
+    FZ_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+size_t entropy_encode_workspace(uint64_t n) {
+    // delta copy, positions, class bytes, mantissas, block slots, scan scratch and o1 chain buffers
+    const uint64_t blocks = n / kFblk + 2;
+    size_t b = 0;
+    b += n * 2 + 4096;                   // delta
+    b += n * 8 + 4096;                   // positions
+    b += 2 * (n + 4096);                 // rcl, vcl
+    b += 2 * (n * 8 + 4096);             // bit counts
+    b += 2 * (n * 8 + 4096);             // bit offsets
+    b += (n * 3 + 4096) * 2;             // mantissas
+    b += 4 * blocks * (kFblk + 1024);    // block tmp + slots (two arrays)
+    b += n * 2 * 2 + (1 << 20);          // o1 chain buffers (2 bytes per symbol + slack)
+    b += 16 * kO1Ctx * sizeof(CTable) + 16 * kO1Ctx * (1u << kO1Log);
+    b += 3 * (n * 2 + 4 * (kFblk + 1024)) + (1 << 20);  // ovf
+    b += 64u << 20;                      // cub scratch
+    return b;
+}
+
+// Scratch for cub calls is sized per call from the arena.
+static void* cub_scratch(DevArena& ws, size_t bytes) { return ws.take<uint8_t>(bytes ? bytes : 1); }
+
+static void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, DevArena& ws, hipStream_t st) {
+    if (n == 0) return;
+    size_t bytes = 0;
+    FZ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int64_t)n, st));
+    void* tmp = cub_scratch(ws, bytes);
+    FZ_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int64_t)n, st));
+}
+
+// fse_enc over a class-byte array (RleFse:150-176); returns the framed length written to out.
+static uint64_t fse_enc_blocks(const uint8_t* src, uint64_t n, uint8_t* out, DevArena& ws, hipStream_t st) {
+    if (n == 0) return 0;
+    const uint64_t nb = (n + kFblk - 1) / kFblk;
+    uint8_t* tmp = ws.take<uint8_t>(nb * (kFblk + 1024));
+    uint8_t* slots = ws.take<uint8_t>(nb * (kFblk + 1024));
+    uint64_t* sizes = ws.take<uint64_t>(nb);
+    uint64_t* offs = ws.take<uint64_t>(nb);
+    hipLaunchKernelGGL(k_fse_blocks, dim3((unsigned)nb), dim3(256), 0, st, src, n, tmp, slots, sizes);
+    FZ_HIP(hipGetLastError());
+    std::vector<uint64_t> hs(nb), ho(nb);
+    d2h(hs.data(), sizes, nb, st);
+    uint64_t acc = 0;
+    for (uint64_t b = 0; b < nb; b++) { ho[b] = acc; acc += hs[b]; }
+    h2d(offs, ho.data(), nb, st);
+    hipLaunchKernelGGL(k_pack_slots, dim3(64, (unsigned)nb), dim3(256), 0, st, slots, (uint64_t)(kFblk + 1024), sizes,
+                       offs, out);
+    FZ_HIP(hipGetLastError());
+    return acc;
+}
+
+static void put64(uint8_t*& p, uint64_t v) { memcpy(p, &v, 8); p += 8; }
+
+static size_t encode_method0(const int16_t* bins, uint64_t n, uint8_t* out, DevArena& ws, hipStream_t st) {
+    uint64_t* pos = ws.take<uint64_t>(n + 1);
+    uint64_t* nsel = ws.take<uint64_t>(1);
+    size_t bytes = 0;
+    hipcub::CountingInputIterator<uint64_t> it(0);
+    NonZero pred{bins};
+    FZ_HIP(hipcub::DeviceSelect::If(nullptr, bytes, it, pos, nsel, (int64_t)n, pred, st));
+    void* tmp = cub_scratch(ws, bytes);
+    FZ_HIP(hipcub::DeviceSelect::If(tmp, bytes, it, pos, nsel, (int64_t)n, pred, st));
+    uint64_t nv = 0;
+    d2h(&nv, nsel, 1, st);
+    uint64_t last_pos = 0;
+    if (nv) d2h(&last_pos, pos + nv - 1, 1, st);
+    const uint64_t nr = (nv == 0) ? 1 : (last_pos == n - 1 ? nv : nv + 1);
+    uint8_t* rcl = ws.take<uint8_t>(nr + 1);
+    uint8_t* vcl = ws.take<uint8_t>(nv + 1);
+    uint64_t* rbits = ws.take<uint64_t>(nr + 1);
+    uint64_t* vbits = ws.take<uint64_t>(nv + 1);
+    uint64_t* roff = ws.take<uint64_t>(nr + 1);
+    uint64_t* voff = ws.take<uint64_t>(nv + 1);
+    hipLaunchKernelGGL(k_runs_values, dim3(nblk(std::max(nr, nv))), dim3(256), 0, st, bins, n, pos, nv, nr, rcl, vcl,
+                       rbits, vbits);
+    FZ_HIP(hipGetLastError());
+    exclusive_scan_u64(rbits, roff, nr, ws, st);
+    exclusive_scan_u64(vbits, voff, nv, ws, st);
+    uint64_t rlast[2] = {0, 0}, vlast[2] = {0, 0};
+    d2h(&rlast[0], roff + nr - 1, 1, st);
+    d2h(&rlast[1], rbits + nr - 1, 1, st);
+    if (nv) { d2h(&vlast[0], voff + nv - 1, 1, st); d2h(&vlast[1], vbits + nv - 1, 1, st); }
+    const uint64_t rbp = (rlast[0] + rlast[1] + 7) / 8, vbp = nv ? (vlast[0] + vlast[1] + 7) / 8 : 0;
+    uint32_t* rman = ws.take<uint32_t>(rbp / 4 + 2);
+    uint32_t* vman = ws.take<uint32_t>(vbp / 4 + 2);
+    FZ_HIP(hipMemsetAsync(rman, 0, (rbp / 4 + 2) * 4, st));
+    FZ_HIP(hipMemsetAsync(vman, 0, (vbp / 4 + 2) * 4, st));
+    hipLaunchKernelGGL(k_scatter_mantissas, dim3(nblk(std::max(nr, nv))), dim3(256), 0, st, bins, n, pos, nv, nr, roff,
+                       voff, rcl, vcl, rman, vman);
+    FZ_HIP(hipGetLastError());
+    // class streams go after the 48-byte head
+    uint8_t* body = out + 48;
+    const uint64_t s_rcl = fse_enc_blocks(rcl, nr, body, ws, st);
+    const uint64_t s_vcl = fse_enc_blocks(vcl, nv, body + s_rcl, ws, st);
+    FZ_HIP(hipMemcpyAsync(body + s_rcl + s_vcl, rman, rbp, hipMemcpyDeviceToDevice, st));
+    FZ_HIP(hipMemcpyAsync(body + s_rcl + s_vcl + rbp, vman, vbp, hipMemcpyDeviceToDevice, st));
+    uint8_t head[48], *p = head;
+    put64(p, nr); put64(p, nv); put64(p, s_rcl); put64(p, s_vcl); put64(p, rbp); put64(p, vbp);
+    FZ_HIP(hipMemcpyAsync(out, head, 48, hipMemcpyHostToDevice, st));
+    sync(st);
+    return 48 + s_rcl + s_vcl + rbp + vbp;
+}
+
+static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, DevArena& ws, hipStream_t st) {
+    uint64_t S = n / kO1Seg;
+    if (S < 1) S = 1;
+    if (S > 16) S = 16;
+    const uint64_t seg = (n + S - 1) / S;
+    // escapes (sym 255) over the whole array, in index order
+    uint64_t* epos = ws.take<uint64_t>(n + 1);
+    uint64_t* nsel = ws.take<uint64_t>(1);
+    {
+        size_t bytes = 0;
+        hipcub::CountingInputIterator<uint64_t> it(0);
+        IsO1Escape pred{bins};
+        FZ_HIP(hipcub::DeviceSelect::If(nullptr, bytes, it, epos, nsel, (int64_t)n, pred, st));
+        void* tmp = cub_scratch(ws, bytes);
+        FZ_HIP(hipcub::DeviceSelect::If(tmp, bytes, it, epos, nsel, (int64_t)n, pred, st));
+    }
+    uint64_t nesc = 0;
+    d2h(&nesc, nsel, 1, st);
+    std::vector<uint64_t> hpos(nesc);
+    if (nesc) d2h(hpos.data(), epos, nesc, st);
+    int16_t* ovf = ws.take<int16_t>(nesc + 1);
+    if (nesc) hipLaunchKernelGGL(k_gather_i16, dim3(nblk(nesc)), dim3(256), 0, st, bins, epos, nesc, ovf);
+    // histograms and tables per segment
+    uint32_t* cnt = ws.take<uint32_t>(S * kO1Ctx * kO1Sym);
+    uint16_t* nf = ws.take<uint16_t>(S * kO1Ctx * kO1Sym);
+    CTable* ct = ws.take<CTable>(S * kO1Ctx);
+    uint8_t* spread = ws.take<uint8_t>(S * kO1Ctx * (1u << kO1Log));
+    FZ_HIP(hipMemsetAsync(cnt, 0, S * kO1Ctx * kO1Sym * 4, st));
+    std::vector<uint64_t> chains;  // {seg lo, chunk lo, chunk hi, segment}
+    std::vector<uint64_t> novf(S * kO1K), ovf_off(S + 1);
+    std::vector<uint64_t> seg_n(S);
+    uint64_t e = 0;
+    for (uint64_t g = 0; g < S; g++) {
+        const uint64_t lo = g * seg, hi = std::min(lo + seg, n), sn = hi - lo;
+        seg_n[g] = sn;
+        const uint64_t cs = (sn + kO1K - 1) / kO1K;
+        const int16_t* sb = bins + lo;
+        uint32_t* c = cnt + g * kO1Ctx * kO1Sym;
+        if (sn < (1u << 20)) {
+            hipLaunchKernelGGL(k_o1_hist_exact, dim3(std::min<uint64_t>(1024, nblk(sn))), dim3(256), 0, st, sb, sn, cs, c);
+        } else {
+            uint64_t step = sn / 2048;
+            if (step < 512) step = 512;
+            const uint64_t nw = (sn + step - 1) / step;
+            hipLaunchKernelGGL(k_o1_hist_sample, dim3((unsigned)nw), dim3(256), 0, st, sb, sn, step, (uint64_t)512, c);
+            hipLaunchKernelGGL(k_add_one, dim3(nblk(kO1Ctx * kO1Sym)), dim3(256), 0, st, c, kO1Ctx * kO1Sym);
+        }
+        FZ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_o1_tables, dim3(kO1Ctx), dim3(64), 0, st, c, nf + g * kO1Ctx * kO1Sym,
+                           ct + g * kO1Ctx, spread + g * kO1Ctx * (1u << kO1Log));
+        FZ_HIP(hipGetLastError());
+        ovf_off[g] = e;
+        for (int k = 0; k < kO1K; k++) {
+            const uint64_t clo = lo + k * cs, chi = std::min(clo + cs, hi);
+            chains.insert(chains.end(), {lo, clo, std::max(chi, clo), g});
+            uint64_t cntk = 0;
+            while (e < nesc && hpos[e] < chi) { e++; cntk++; }
+            novf[g * kO1K + k] = (chi > clo) ? cntk : 0;
+        }
+    }
+    ovf_off[S] = e;
+    const int nch = (int)(S * kO1K);
+    uint64_t* dchains = ws.take<uint64_t>(chains.size());
+    h2d(dchains, chains.data(), chains.size(), st);
+    uint64_t maxcs = 0;
+    for (uint64_t g = 0; g < S; g++) maxcs = std::max(maxcs, (seg_n[g] + kO1K - 1) / kO1K);
+    const uint64_t bstride = maxcs * 2 + 4096;
+    uint8_t* bufs = ws.take<uint8_t>(nch * bstride);
+    uint64_t* blen = ws.take<uint64_t>(nch);
+    hipLaunchKernelGGL(k_o1_encode, dim3(nblk(nch, 64)), dim3(64), 0, st, bins, dchains, nch, ct, bufs, bstride, blen);
+    FZ_HIP(hipGetLastError());
+    // escape side streams
+    uint64_t maxo = 0;
+    for (uint64_t g = 0; g < S; g++) maxo = std::max(maxo, ovf_off[g + 1] - ovf_off[g]);
+    const uint64_t ostride = 2 * maxo + 17 + 2 * ((maxo + kFblk - 1) / kFblk + 1) * (kFblk + 1024) + 256;
+    uint8_t* ovcl = ws.take<uint8_t>(S * ostride);
+    uint8_t* ovman = ws.take<uint8_t>(S * ostride);
+    uint8_t* otmp = ws.take<uint8_t>(S * ostride);
+    uint8_t* oout = ws.take<uint8_t>(S * ostride);
+    uint64_t* olen = ws.take<uint64_t>(S);
+    uint64_t* dovf_off = ws.take<uint64_t>(S + 1);
+    h2d(dovf_off, ovf_off.data(), S + 1, st);
+    hipLaunchKernelGGL(k_o1_ovf, dim3((unsigned)S), dim3(256), 0, st, ovf, dovf_off, ovcl, ovman, otmp, oout, ostride, olen);
+    FZ_HIP(hipGetLastError());
+    std::vector<uint64_t> hblen(nch), holen(S);
+    d2h(hblen.data(), blen, nch, st);
+    d2h(holen.data(), olen, S, st);
+    // assemble: [u64 S] then per segment [n][K][blen x4][novf x4][nf][streams][ovf block]
+    uint8_t* p = out;
+    {
+        uint64_t s = S;
+        FZ_HIP(hipMemcpyAsync(p, &s, 8, hipMemcpyHostToDevice, st));
+        sync(st);
+        p += 8;
+    }
+    for (uint64_t g = 0; g < S; g++) {
+        uint8_t head[8 * 10], *h = head;
+        put64(h, seg_n[g]);
+        put64(h, kO1K);
+        for (int k = 0; k < kO1K; k++) put64(h, hblen[g * kO1K + k]);
+        for (int k = 0; k < kO1K; k++) put64(h, novf[g * kO1K + k]);
+        FZ_HIP(hipMemcpyAsync(p, head, sizeof head, hipMemcpyHostToDevice, st));
+        sync(st);
+        p += sizeof head;
+        FZ_HIP(hipMemcpyAsync(p, nf + g * kO1Ctx * kO1Sym, kO1Ctx * kO1Sym * 2, hipMemcpyDeviceToDevice, st));
+        p += kO1Ctx * kO1Sym * 2;
+        for (int k = 0; k < kO1K; k++) {
+            const uint64_t q = g * kO1K + k;
+            FZ_HIP(hipMemcpyAsync(p, bufs + q * bstride, hblen[q], hipMemcpyDeviceToDevice, st));
+            p += hblen[q];
+        }
+        FZ_HIP(hipMemcpyAsync(p, oout + g * ostride, holen[g], hipMemcpyDeviceToDevice, st));
+        p += holen[g];
+    }
+    sync(st);
+    return (size_t)(p - out);
+}
+
+size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out_cap, DevArena& ws, hipStream_t st,
+                      EntropyStats* stats) {
+    (void)out_cap;
+    // preprocess_encode: 256 windows of 32 codes (RleFse:850-895)
+    uint64_t NW = 256, WL = 32;
+    if (n < NW * WL) { WL = n / NW; if (!WL) WL = 1; }
+    uint64_t step = (n > NW * WL) ? (n - WL) / NW : WL;
+    if (!step) step = 1;
+    PreCounts* pc = ws.take<PreCounts>(1);
+    hipLaunchKernelGGL(k_preprocess, dim3(1), dim3(256), 0, st, codes, n, NW, WL, step, pc);
+    FZ_HIP(hipGetLastError());
+    PreCounts h{};
+    d2h(&h, pc, 1, st);
+    const double p0 = h.cnt ? 1.0 * (double)h.z / (double)h.cnt : 1.0;
+    const double dz = (h.cnt && h.cntd) ? (1.0 * (double)h.zd / (double)h.cntd - p0) : 0.0;
+    const int use_delta = dz > 0.005;
+    int method = (p0 >= 0.80) ? 0 : 1;
+    if (n < (uint64_t)20 * kO1Ctx * kO1Sym) method = 0;
+    if (stats) { stats->method = method; stats->use_delta = use_delta; }
+    const uint8_t save[5] = {(uint8_t)(method | (use_delta ? 0x80 : 0)), 0, 0, 0, 0};
+    FZ_HIP(hipMemcpyAsync(out, save, 5, hipMemcpyHostToDevice, st));
+    sync(st);
+    const int16_t* bins = codes;
+    if (use_delta && n > 0) {
+        int16_t* d = ws.take<int16_t>(n);
+        hipLaunchKernelGGL(k_delta, dim3(nblk(n)), dim3(256), 0, st, codes, d, n);
+        FZ_HIP(hipGetLastError());
+        bins = d;
+    }
+    const size_t body = (method == 1) ? encode_method1(bins, n, out + 5, ws, st) : encode_method0(bins, n, out + 5, ws, st);
+    return 5 + body;
+}
+
+size_t entropy_decode_workspace(uint64_t n, size_t len) {
+    return n * 8 * 3 + n * 2 * 2 + len * 2 + 16 * kO1Ctx * (sizeof(DEntry) + 1) * (1u << kO1Log) + (96u << 20);
+}
+
+static uint64_t get64(const uint8_t*& p) {
+    uint64_t v;
+    memcpy(&v, p, 8);
+    p += 8;
+    return v;
+}
+
+// Block offsets of a framed class-byte stream, found by walking the block headers.
+static uint64_t frame_offsets(const uint8_t* h, uint64_t n, std::vector<uint64_t>& offs) {
+    uint64_t rp = 0, dp = 0;
+    offs.clear();
+    while (dp < n) {
+        const uint64_t len = std::min<uint64_t>(kFblk, n - dp);
+        offs.push_back(rp);
+        const uint8_t flag = h[rp];
+        if (flag == 0) rp += 1 + len;
+        else if (flag == 2) rp += 2;
+        else { uint32_t rr; memcpy(&rr, h + rp + 1, 4); rp += 5 + rr; }
+        dp += len;
+    }
+    return rp;
+}
+
+static void fse_dec_blocks(const uint8_t* hsrc, const uint8_t* dsrc, uint64_t n, uint8_t* out, int* err, DevArena& ws,
+                           hipStream_t st, uint64_t* consumed) {
+    std::vector<uint64_t> offs;
+    const uint64_t used = frame_offsets(hsrc, n, offs);
+    if (consumed) *consumed = used;
+    if (n == 0) return;
+    uint64_t* doffs = ws.take<uint64_t>(offs.size());
+    h2d(doffs, offs.data(), offs.size(), st);
+    hipLaunchKernelGGL(k_fse_dec_blocks, dim3((unsigned)offs.size()), dim3(256), 0, st, dsrc, doffs, n, out, err);
+    FZ_HIP(hipGetLastError());
+}
+
+void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
+                    hipStream_t st) {
+    (void)len;
+    const int method = host[0] & 0x7F, use_delta = (host[0] & 0x80) != 0;
+    int* err = ws.take<int>(1);
+    FZ_HIP(hipMemsetAsync(err, 0, 4, st));
+    const uint8_t* hp = host + 5;
+    int16_t* out = use_delta ? ws.take<int16_t>(n + 1) : codes;
+    if (method == 1) {
+        uint64_t S = get64(hp);
+        if (!S) S = 1;
+        const uint64_t seg = (n + S - 1) / S;
+        std::vector<uint64_t> chains;
+        std::vector<uint64_t> nf_off(S), ovf_sizes(S), ovf_base(S);
+        uint64_t total_ovf = 0;
+        struct OvfPiece { uint64_t n, vcl_off, vman_off, nraw_off, mode; };
+        std::vector<OvfPiece> pieces(S);
+        for (uint64_t g = 0; g < S; g++) {
+            const uint64_t lo = g * seg, hi = std::min(lo + seg, n), sn = hi - lo;
+            (void)get64(hp);
+            const int K = (int)get64(hp);
+            if (K < 1 || K > 16) throw std::runtime_error("corrupt o1 tANS stream");
+            uint64_t blen[16], nvk[16];
+            for (int k = 0; k < K; k++) blen[k] = get64(hp);
+            for (int k = 0; k < K; k++) nvk[k] = get64(hp);
+            nf_off[g] = (uint64_t)(hp - host);
+            hp += kO1Ctx * kO1Sym * 2;
+            const uint64_t cs = sn ? (sn + K - 1) / K : 0;
+            uint64_t ovfo = 0;
+            for (int k = 0; k < K; k++) {
+                const uint64_t clo = lo + k * cs, chi = std::min(clo + cs, hi);
+                chains.insert(chains.end(), {(uint64_t)(hp - host), blen[k], clo, std::max(clo, chi), total_ovf + ovfo, g});
+                hp += blen[k];
+                ovfo += nvk[k];
+            }
+            OvfPiece pc{ovfo, 0, 0, 0, 0};
+            pc.mode = *hp++;
+            if (ovfo) {
+                if (pc.mode == 0) {
+                    pc.nraw_off = (uint64_t)(hp - host);
+                    hp += ovfo * 2;
+                } else {
+                    const uint64_t svcl = get64(hp), vbp = get64(hp);
+                    pc.vcl_off = (uint64_t)(hp - host);
+                    hp += svcl;
+                    pc.vman_off = (uint64_t)(hp - host);
+                    hp += vbp;
+                }
+            }
+            pieces[g] = pc;
+            ovf_base[g] = total_ovf;
+            total_ovf += ovfo;
+        }
+        int16_t* ovf = ws.take<int16_t>(total_ovf + 1);
+        for (uint64_t g = 0; g < S; g++) {
+            const OvfPiece& pc = pieces[g];
+            if (!pc.n) continue;
+            if (pc.mode == 0) {
+                FZ_HIP(hipMemcpyAsync(ovf + ovf_base[g], dev + pc.nraw_off, pc.n * 2, hipMemcpyDeviceToDevice, st));
+            } else {
+                uint8_t* vcl = ws.take<uint8_t>(pc.n);
+                fse_dec_blocks(host + pc.vcl_off, dev + pc.vcl_off, pc.n, vcl, err, ws, st, nullptr);
+                hipLaunchKernelGGL(k_ovf_dec, dim3(1), dim3(1), 0, st, vcl, dev + pc.vman_off, pc.n, ovf + ovf_base[g]);
+                FZ_HIP(hipGetLastError());
+            }
+        }
+        DEntry* dt = ws.take<DEntry>(S * kO1Ctx * (1u << kO1Log));
+        uint8_t* spread = ws.take<uint8_t>(S * kO1Ctx * (1u << kO1Log));
+        for (uint64_t g = 0; g < S; g++) {
+            hipLaunchKernelGGL(k_o1_dtables, dim3(kO1Ctx), dim3(64), 0, st, dev + nf_off[g],
+                               dt + g * kO1Ctx * (1u << kO1Log), spread + g * kO1Ctx * (1u << kO1Log), err);
+        }
+        const int nch = (int)(chains.size() / 6);
+        uint64_t* dch = ws.take<uint64_t>(chains.size());
+        h2d(dch, chains.data(), chains.size(), st);
+        hipLaunchKernelGGL(k_o1_decode, dim3(nblk(nch, 64)), dim3(64), 0, st, dev, dch, nch, dt, ovf, out);
+        FZ_HIP(hipGetLastError());
+    } else {
+        const uint64_t nr = get64(hp), nv = get64(hp), s_rcl = get64(hp), s_vcl = get64(hp), rbp = get64(hp),
+                       vbp = get64(hp);
+        const uint64_t o_rcl = (uint64_t)(hp - host), o_vcl = o_rcl + s_rcl, o_rman = o_vcl + s_vcl,
+                       o_vman = o_rman + rbp;
+        (void)vbp;
+        uint8_t* rcl = ws.take<uint8_t>(nr + 1);
+        uint8_t* vcl = ws.take<uint8_t>(nv + 1);
+        fse_dec_blocks(host + o_rcl, dev + o_rcl, nr, rcl, err, ws, st, nullptr);
+        if (nv) fse_dec_blocks(host + o_vcl, dev + o_vcl, nv, vcl, err, ws, st, nullptr);
+        // bit offsets of the mantissas
+        uint64_t* rk = ws.take<uint64_t>(nr + 1);
+        uint64_t* vk = ws.take<uint64_t>(nv + 1);
+        uint64_t* roff = ws.take<uint64_t>(nr + 1);
+        uint64_t* voff = ws.take<uint64_t>(nv + 1);
+        {
+            hipcub::TransformInputIterator<uint64_t, ToU64, const uint8_t*> ri(rcl, ToU64{});
+            hipcub::TransformInputIterator<uint64_t, ToU64, const uint8_t*> vi(vcl, ToU64{});
+            size_t b1 = 0, b2 = 0;
+            FZ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, ri, roff, (int64_t)nr, st));
+            void* t1 = cub_scratch(ws, b1);
+            FZ_HIP(hipcub::DeviceScan::ExclusiveSum(t1, b1, ri, roff, (int64_t)nr, st));
+            if (nv) {
+                FZ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, vi, voff, (int64_t)nv, st));
+                void* t2 = cub_scratch(ws, b2);
+                FZ_HIP(hipcub::DeviceScan::ExclusiveSum(t2, b2, vi, voff, (int64_t)nv, st));
+            }
+        }
+        (void)rk; (void)vk;
+        uint64_t* runlen = ws.take<uint64_t>(nr + 1);
+        uint64_t* runsum = ws.take<uint64_t>(nr + 1);
+        hipLaunchKernelGGL(k_runs_decode, dim3(nblk(nr)), dim3(256), 0, st, rcl, roff, dev + o_rman, nr, runlen);
+        FZ_HIP(hipGetLastError());
+        {
+            size_t b = 0;
+            FZ_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, b, runlen, runsum, (int64_t)nr, st));
+            void* t = cub_scratch(ws, b);
+            FZ_HIP(hipcub::DeviceScan::InclusiveSum(t, b, runlen, runsum, (int64_t)nr, st));
+        }
+        FZ_HIP(hipMemsetAsync(out, 0, n * 2, st));
+        if (nv) hipLaunchKernelGGL(k_values_place, dim3(nblk(nv)), dim3(256), 0, st, vcl, voff, dev + o_vman, nv,
+                                   runsum, n, out);
+        FZ_HIP(hipGetLastError());
+    }
+    if (use_delta && n > 0) {
+        size_t b = 0;
+        FZ_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, b, out, codes, Add16{}, (int64_t)n, st));
+        void* t = cub_scratch(ws, b);
+        FZ_HIP(hipcub::DeviceScan::InclusiveScan(t, b, out, codes, Add16{}, (int64_t)n, st));
+    }
+    int herr = 0;
+    d2h(&herr, err, 1, st);
+    if (herr) throw std::runtime_error("corrupt entropy stream");
+}
+
+}  // namespace fz

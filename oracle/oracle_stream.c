@@ -6,7 +6,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
-#include <zstd.h>
+#include "oracle_zstd.h"
 
 #include "falcons_oracle.h"
 #include "oracle_internal.h"
@@ -205,9 +205,11 @@ static size_t fo_lossless(const uint8_t *src, size_t n, uint8_t *dst, size_t cap
         memcpy(dst + 8, src, n);
         return n + 8;
     }
-    if (cap - 8 < ZSTD_compressBound(n)) return 0;
-    size_t r = ZSTD_compress(dst + 8, cap - 8, src, n, 3);
-    if (ZSTD_isError(r)) { set_err("Zstd compression failed"); return 0; }
+    const fo_zstd_api *Z = fo_zstd();
+    if (!Z) { set_err("cannot load libzstd"); return 0; }
+    if (cap - 8 < Z->compressBound(n)) return 0;
+    size_t r = Z->compress(dst + 8, cap - 8, src, n, 3);
+    if (Z->isError(r)) { set_err("Zstd compression failed"); return 0; }
     return r + 8;
 }
 
@@ -221,8 +223,9 @@ static uint8_t *fo_unlossless(const uint8_t *src, size_t n, size_t *outlen) {
     if (raw) {
         if (dl > pl) { set_err("Truncated Falcons raw payload"); return NULL; }
     } else {
-        unsigned long long fs = ZSTD_getFrameContentSize(src + 8, pl);
-        if (fs == ZSTD_CONTENTSIZE_ERROR || (fs != ZSTD_CONTENTSIZE_UNKNOWN && fs != dl)) {
+        if (!fo_zstd()) { set_err("cannot load libzstd"); return NULL; }
+        unsigned long long fs = fo_zstd()->getFrameContentSize(src + 8, pl);
+        if (fs == FO_ZSTD_CONTENTSIZE_ERROR || (fs != FO_ZSTD_CONTENTSIZE_UNKNOWN && fs != dl)) {
             set_err("Invalid Falcons Zstd payload size");
             return NULL;
         }
@@ -231,8 +234,8 @@ static uint8_t *fo_unlossless(const uint8_t *src, size_t n, size_t *outlen) {
     if (raw) {
         memcpy(dst, src + 8, dl);
     } else {
-        size_t r = ZSTD_decompress(dst, dl, src + 8, pl);
-        if (ZSTD_isError(r) || r != dl) { free(dst); set_err("Invalid or truncated Falcons Zstd payload"); return NULL; }
+        size_t r = fo_zstd()->decompress(dst, dl, src + 8, pl);
+        if (fo_zstd()->isError(r) || r != dl) { free(dst); set_err("Invalid or truncated Falcons Zstd payload"); return NULL; }
     }
     *outlen = dl;
     return dst;
@@ -241,7 +244,7 @@ static uint8_t *fo_unlossless(const uint8_t *src, size_t n, size_t *outlen) {
 size_t fo_size_bound(const fo_config *c, int dtype) {
     uint8_t tmp[256];
     size_t es = fo_config_save(c, tmp);
-    return 4096 + es + ZSTD_compressBound(c->num * (dtype ? 8 : 4));
+    return 4096 + es + (fo_zstd() ? fo_zstd()->compressBound(c->num * (dtype ? 8 : 4)) : 0);
 }
 
 /* Falcons_compress_Interp + FalconsGenericCompressor::compress: the buffer handed to the lossless
@@ -323,7 +326,7 @@ size_t fo_compress(const fo_config *conf_in, int dtype, void *data, uint8_t *out
     if (c.cmpr_algo == 4) {
         plen = fo_lossless((const uint8_t *)data, c.num * es, payload, pcap);
     } else if ((double)(c.num * es) / 1.0 / (double)plen < 3) {
-        size_t zc = ZSTD_compressBound(c.num * es) + 8;
+        size_t zc = fo_zstd()->compressBound(c.num * es) + 8;
         uint8_t *z = (uint8_t *)malloc(zc);
         size_t zl = fo_lossless((const uint8_t *)data, c.num * es, z, zc);
         if (zl && zl < plen && zl <= pcap) {

@@ -5,7 +5,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
-#include <zstd.h>
+#include "oracle_zstd.h"
 
 #include "falcons_oracle.h"
 #include "oracle_internal.h"
@@ -173,9 +173,10 @@ DEF_TUNE(double, f64, fma)
 
 static size_t tn_lossless_size(const uint8_t *src, size_t n) {
     if (getenv("FALCONS_NOZSTD")) return n + 8;
-    size_t cap = ZSTD_compressBound(n);
+    const fo_zstd_api *Z = fo_zstd();
+    size_t cap = Z->compressBound(n);
     uint8_t *dst = (uint8_t *)malloc(cap ? cap : 1);
-    size_t r = ZSTD_compress(dst, cap, src, n, 3);
+    size_t r = Z->compress(dst, cap, src, n, 3);
     free(dst);
     return r + 8;
 }

@@ -1,0 +1,74 @@
+"""GPU path against the oracle on the same seeded inputs: the buffer handed to zstd, the final stream
+and the decompressed field must be identical bit for bit."""
+import numpy as np
+import pytest
+
+from falcons_amd import synth
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import falcons_amd
+    return falcons_amd.Context(0)
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return Oracle()
+
+
+def _cfg_pair(oracle, shape, **kw):
+    import falcons_amd
+    c = falcons_amd.falconsConfig(*shape)
+    oc = oracle.config(shape)
+    for k, v in kw.items():
+        setattr(c.conf, k, v)
+        setattr(oc, k, v)
+    return c, oc
+
+
+CASES = [
+    # shape, dtype, algo(2 interp), interp_algo, direction, eb_mode, eb, anchor
+    ((33, 33, 33), np.float32, 1, 0, 0, 1e-3, -1),
+    ((33, 33, 33), np.float32, 0, 0, 0, 1e-3, -1),
+    ((17, 40, 65), np.float32, 1, 5, 0, 1e-3, -1),
+    ((17, 40, 65), np.float32, 0, 5, 1, 1e-3, -1),
+    ((64, 48, 100), np.float32, 1, 0, 1, 1e-4, -1),
+    ((64, 48, 100), np.float64, 1, 0, 0, 1e-6, -1),
+    ((20, 30, 31), np.float64, 0, 5, 0, 1e-5, -1),
+    ((5, 6, 7), np.float32, 1, 0, 0, 1e-3, -1),
+    ((2, 3, 5), np.float32, 0, 0, 0, 1e-3, -1),
+    ((300, 257), np.float32, 1, 0, 0, 1e-3, -1),
+    ((300, 257), np.float32, 0, 1, 1, 1e-3, -1),
+    ((129, 130), np.float64, 1, 1, 0, 1e-7, -1),
+    ((5000,), np.float32, 1, 0, 0, 1e-3, -1),
+    ((4097,), np.float64, 0, 0, 1, 1e-4, -1),
+    ((40, 40, 40), np.float32, 1, 0, 0, 1e-3, 0),
+    ((32, 32, 32), np.float32, 1, 0, 0, 1e-3, -1),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{np.dtype(c[1]).name}-a{c[2]}d{c[3]}m{c[4]}" for c in CASES])
+@pytest.mark.parametrize("simd", [1, 0])
+def test_stream_parity(ctx, oracle, case, simd):
+    shape, dtype, ialgo, direction, mode, eb, anchor = case
+    data = synth.sinusoids(shape, seed=7, dtype=dtype)
+    kw = dict(cmpr_algo=2, eb_mode=mode, abs_eb=eb, rel_eb=eb, interp_algo=ialgo, interp_direction=direction,
+              interp_anchor_stride=anchor)
+    c, oc = _cfg_pair(oracle, shape, **kw)
+    oracle.L.fo_set_simd(simd)
+    o_stream, o_mut, o_pre = oracle.compress(data, oc, want_pre=True)
+    g_data = data.copy()
+    g_stream, opts, g_pre = ctx.compress_ex(g_data, c, simd=simd, want_prezstd=True)
+    assert len(g_pre) == len(o_pre)
+    assert g_pre == o_pre
+    assert bytes(g_stream) == o_stream
+    assert np.array_equal(g_data.view(np.uint8), o_mut.view(np.uint8))
+    dec, _, _ = ctx.decompress_ex(g_stream, dtype)
+    o_dec, _ = oracle.decompress(o_stream, dtype)
+    assert np.array_equal(dec.reshape(shape).view(np.uint8), o_dec.reshape(shape).view(np.uint8))
+    bound = eb if mode == 0 else eb * float(data.max() - data.min())
+    assert np.abs(dec.reshape(shape).astype(np.float64) - data.astype(np.float64)).max() <= bound
