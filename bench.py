@@ -1,0 +1,215 @@
+"""Throughput of the interpolation -> quantize -> entropy-code path on one MI355X.
+
+Workload: a 512^3 float32 Gaussian random field with power spectrum |k|^-3 (seeded FFT synthesis,
+zero mean, unit variance), REL error bound 1e-3, ALGO_INTERP, the plain per-call GPU path.
+
+One step = one compress (field resident in HBM -> pre-zstd buffer resident in HBM) plus one
+decompress (pre-zstd buffer in HBM -> field in HBM) through the C-ABI device entry points, plus the
+device-to-device restore of the input that compression overwrites with its reconstruction.
+value = 2 * N * sizeof(float) / (wall time per step), in GB/s (1e9 B/s).
+
+The host zstd stage is timed separately (end-to-end API figures) and the CPU oracle is timed on the
+host cores over the same field; its pre-zstd buffer and stream are compared byte for byte with the
+GPU's, so every run re-checks byte identity at full size.
+"""
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--side", type=int, default=512, help="cube side of the field")
+    ap.add_argument("--rel", type=float, default=1e-3)
+    ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timing and full-size byte check")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end host API timing")
+    ap.add_argument("--seed", type=int, default=2)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the path runs on one MI355X; "
+              "independent fields shard across GPUs with no exchange)", file=sys.stderr)
+        sys.exit(2)
+    import falcons_amd as fa
+    from falcons_amd import synth
+
+    shape = (a.side,) * 3
+    t = time.time()
+    field = synth.power_law(shape, slope=-3.0, seed=a.seed, dtype=np.float32)
+    log(f"field {shape} float32 k^-3 seed {a.seed} generated in {time.time() - t:.1f} s")
+    n = field.size
+    nbytes = field.nbytes
+
+    cfg = fa.falconsConfig(*shape)
+    cfg.cmprAlgo = fa.falconsAlgorithm.INTERP
+    cfg.errorBoundMode = fa.falconsErrorBoundMode.REL
+    cfg.relErrorBound = a.rel
+
+    ctx = fa.Context(0)
+    d_orig = ctx.alloc(nbytes)
+    d_work = ctx.alloc(nbytes)
+    d_out = ctx.alloc(nbytes)
+    d_orig.upload(field)
+    pre_cap = ctx.prezstd_bound(cfg, np.float32)
+    d_pre = ctx.alloc(pre_cap)
+
+    def restore():
+        fa._L.fz_memcpy(ctx.h, d_work.ptr, d_orig.ptr, nbytes, 2)
+
+    # first call: headers for the device-resident decompress, and the checks
+    restore()
+    run_cfg = cfg.copy()
+    pre_len, copts = ctx.compress_device(run_cfg, np.float32, d_work, d_pre)
+    h_pre = d_pre.download(pre_len)
+    recon = d_work.download(nbytes, np.float32)
+    ctx.decompress_device(run_cfg, np.float32, h_pre, d_pre, d_out)
+    dec = d_out.download(nbytes, np.float32)
+    # compression writes the reconstruction back except at escapes and on the last pass of level 1
+    src = field.reshape(-1)
+    if not np.all((recon.view(np.uint32) == dec.view(np.uint32)) | (recon.view(np.uint32) == src.view(np.uint32))):
+        raise SystemExit("the input written back by compression disagrees with the decompressed field")
+    eb = run_cfg.absErrorBound
+    err = float(np.max(np.abs(dec.astype(np.float64) - field.reshape(-1).astype(np.float64))))
+    if not err <= eb:
+        raise SystemExit(f"max error {err} above the bound {eb}")
+    log(f"pre-zstd {pre_len} B method {copts.method} delta {copts.use_delta} codes {copts.ncodes} "
+        f"unpred {copts.nunpred}; max err {err:.4g} <= {eb:.4g}")
+
+    for _ in range(a.warmup):
+        restore()
+        c = cfg.copy()
+        ctx.compress_device(c, np.float32, d_work, d_pre)
+        ctx.decompress_device(c, np.float32, h_pre, d_pre, d_out)
+    ctx.synchronize()
+
+    tc, td = [], []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        restore()
+        c = cfg.copy()
+        plen, co = ctx.compress_device(c, np.float32, d_work, d_pre)
+        do = ctx.decompress_device(c, np.float32, h_pre, d_pre, d_out)
+        tc.append(co.t_device_ms)
+        td.append(do.t_device_ms)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    if plen != pre_len:
+        raise SystemExit("pre-zstd length changed between steps")
+    ms_step = wall * 1e3 / a.steps
+    value = 2.0 * nbytes / (ms_step * 1e-3) / 1e9
+    tc_m, td_m = float(np.median(tc)), float(np.median(td))
+    comp_gbps = nbytes / (tc_m * 1e-3) / 1e9
+    decomp_gbps = nbytes / (td_m * 1e-3) / 1e9
+    log(f"step {ms_step:.2f} ms; compress device {tc_m:.2f} ms ({comp_gbps:.1f} GB/s), "
+        f"decompress device {td_m:.2f} ms ({decomp_gbps:.1f} GB/s)")
+
+    # whole-call algorithmic bytes (compress: N*s in + C_dev out; decompress: C_dev in + N*s out)
+    alg_c = nbytes + pre_len
+    alg_d = pre_len + nbytes
+    pipe = {
+        "compress_GBps_alg": alg_c / (tc_m * 1e-3) / 1e9,
+        "decompress_GBps_alg": alg_d / (td_m * 1e-3) / 1e9,
+    }
+    pipe["compress_frac"] = pipe["compress_GBps_alg"] / HBM_PEAK_GBPS
+    pipe["decompress_frac"] = pipe["decompress_GBps_alg"] / HBM_PEAK_GBPS
+
+    e2e = None
+    stream = None
+    if not a.no_e2e:
+        host = field.copy()
+        t = time.perf_counter()
+        stream, eo, _ = ctx.compress_ex(host, cfg.copy(), copy_back=False)
+        e2e_c = time.perf_counter() - t
+        t = time.perf_counter()
+        out, _, _ = ctx.decompress_ex(stream, np.float32)
+        e2e_d = time.perf_counter() - t
+        if not np.array_equal(out.reshape(-1).view(np.uint32), dec.view(np.uint32)):
+            raise SystemExit("host-API decompression differs from the device path")
+        e2e = {"compress_GBps": nbytes / e2e_c / 1e9, "decompress_GBps": nbytes / e2e_d / 1e9,
+               "zstd_ms": eo.t_zstd_ms, "stream_bytes": int(stream.size), "ratio": nbytes / stream.size}
+        log(f"end-to-end host API: compress {e2e['compress_GBps']:.2f} GB/s, decompress "
+            f"{e2e['decompress_GBps']:.2f} GB/s, zstd {eo.t_zstd_ms:.1f} ms, ratio {e2e['ratio']:.3f}")
+
+    cpu = None
+    parity = None
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import Oracle
+
+        orc = Oracle()
+        oc = orc.config(shape, cmpr_algo=2, eb_mode=1, rel_eb=a.rel)
+        src = field.copy()
+        t = time.perf_counter()
+        o_stream, _, o_pre = orc.compress(src, oc, want_pre=True)
+        t_oc = time.perf_counter() - t
+        t = time.perf_counter()
+        o_dec, _ = orc.decompress(o_stream, np.float32)
+        t_od = time.perf_counter() - t
+        parity = {
+            "prezstd_identical": bytes(h_pre) == o_pre,
+            "decompressed_identical": bool(np.array_equal(o_dec.reshape(-1).view(np.uint32), dec.view(np.uint32))),
+            "prezstd_sha256": hashlib.sha256(bytes(h_pre)).hexdigest()[:16],
+        }
+        if stream is not None:
+            parity["stream_identical"] = bytes(stream) == o_stream
+        cpu = {"value": 2.0 * nbytes / (t_oc + t_od) / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"the full {a.side}^3 field, one compress + one decompress through the C oracle "
+                         f"(serial AVX2-build semantics, host zstd included): compress {t_oc:.2f} s, "
+                         f"decompress {t_od:.2f} s"}
+        log(f"cpu oracle: compress {t_oc:.2f} s decompress {t_od:.2f} s; parity {parity}")
+        if not all(v for k, v in parity.items() if k.endswith("identical")):
+            raise SystemExit(f"GPU output differs from the oracle at full size: {parity}")
+
+    line = {
+        "metric": "compress/decompress GB/s on 1 MI355X, byte-identical; batched p50/p99 latency",
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{a.side}^3 float32 k^-3 Gaussian random field, REL {a.rel:g}, ALGO_INTERP, "
+                               "plain per-call GPU path, device-resident compress+decompress round trip",
+                   "field_bytes": nbytes, "prezstd_bytes": pre_len, "entropy_method": copts.method},
+        "compress_GBps": round(comp_gbps, 3),
+        "decompress_GBps": round(decomp_gbps, 3),
+        "pipeline_roofline": {k: round(v, 4) for k, v in pipe.items()},
+        "roofline": None,
+        "cpu_baseline": cpu,
+        "end_to_end": e2e,
+        "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+    for b in (d_orig, d_work, d_out, d_pre):
+        b.free()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -74,10 +74,35 @@ _L.fz_compress.restype = C.c_int
 _L.fz_decompress.argtypes = [C.c_void_p, _u8p, C.c_size_t, C.c_int, C.c_void_p, C.c_size_t, C.c_int,
                              C.POINTER(fz_config), C.POINTER(fz_opts)]
 _L.fz_decompress.restype = C.c_int
+_L.fz_prezstd_bound.argtypes = [C.POINTER(fz_config), C.c_int]
+_L.fz_prezstd_bound.restype = C.c_size_t
+_L.fz_compress_device.argtypes = [C.c_void_p, C.POINTER(fz_config), C.c_int, C.c_void_p, C.c_void_p, C.c_size_t,
+                                  C.POINTER(C.c_size_t), C.POINTER(fz_opts)]
+_L.fz_compress_device.restype = C.c_int
+_L.fz_decompress_device.argtypes = [C.c_void_p, C.POINTER(fz_config), C.c_int, _u8p, C.c_void_p, C.c_size_t,
+                                    C.c_void_p, C.c_size_t, C.POINTER(fz_opts)]
+_L.fz_decompress_device.restype = C.c_int
+_L.fz_device_alloc.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]
+_L.fz_device_alloc.restype = C.c_int
+_L.fz_device_free.argtypes = [C.c_void_p, C.c_void_p]
+_L.fz_device_free.restype = C.c_int
+_L.fz_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+_L.fz_memcpy.restype = C.c_int
+_L.fz_synchronize.argtypes = [C.c_void_p]
+_L.fz_synchronize.restype = C.c_int
+_L.fz_zstd_version.argtypes = []
+_L.fz_zstd_version.restype = C.c_uint
 
 EXPORTED_SYMBOLS = ("fz_ctx_create", "fz_ctx_destroy", "fz_last_error", "fz_opts_init", "fz_config_init",
                     "fz_config_size_est", "fz_config_save", "fz_read_config", "fz_compress_size_bound",
-                    "fz_compress", "fz_decompress")
+                    "fz_compress", "fz_decompress", "fz_prezstd_bound", "fz_compress_device",
+                    "fz_decompress_device", "fz_device_alloc", "fz_device_free", "fz_memcpy", "fz_synchronize",
+                    "fz_zstd_version")
+
+
+def zstd_version():
+    """ZSTD_versionNumber() of the libzstd the host stage uses (0 if none loads)."""
+    return int(_L.fz_zstd_version())
 
 
 class falconsErrorBoundMode(IntEnum):
@@ -140,6 +165,12 @@ class falconsConfig:
             for k, v in keep.items():
                 setattr(self.conf, k, v)
         return self.num_elements
+
+    def copy(self):
+        other = falconsConfig.__new__(falconsConfig)
+        other.conf = fz_config()
+        C.pointer(other.conf)[0] = self.conf
+        return other
 
     @property
     def dims(self):
@@ -209,6 +240,38 @@ _DT = {np.dtype(np.float32): 0, np.dtype(np.float64): 1}
 
 def _is_device_tensor(x):
     return type(x).__module__.startswith("torch") and getattr(x, "is_cuda", False)
+
+
+class DeviceBuffer:
+    """Device memory owned by a Context (freed with it or by free())."""
+
+    def __init__(self, ctx, nbytes):
+        p = C.c_void_p()
+        rc = _L.fz_device_alloc(ctx.h, int(nbytes), C.byref(p))
+        if rc:
+            _raise(rc, ctx._err())
+        self.ctx, self.ptr, self.nbytes = ctx, p, int(nbytes)
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the device buffer")
+        rc = _L.fz_memcpy(self.ctx.h, self.ptr, C.c_void_p(a.ctypes.data), a.nbytes, 0)
+        if rc:
+            _raise(rc, self.ctx._err())
+
+    def download(self, nbytes=None, dtype=np.uint8):
+        n = self.nbytes if nbytes is None else int(nbytes)
+        out = np.empty(n // np.dtype(dtype).itemsize, dtype=dtype)
+        rc = _L.fz_memcpy(self.ctx.h, C.c_void_p(out.ctypes.data), self.ptr, out.nbytes, 1)
+        if rc:
+            _raise(rc, self.ctx._err())
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            _L.fz_device_free(self.ctx.h, self.ptr)
+        self.ptr = None
 
 
 class Context:
@@ -297,6 +360,48 @@ class Context:
         return out, cfg, opts
 
 
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def compress_device(self, config, dtype, d_data, d_pre, simd=1):
+        """fz_compress_device: field in d_data (DeviceBuffer, overwritten with the reconstruction) -> pre-zstd
+        buffer in d_pre. Returns (pre-zstd length, fz_opts). config is updated with the absolute bound."""
+        dt = _DT[np.dtype(dtype)]
+        opts = fz_opts()
+        _L.fz_opts_init(C.byref(opts))
+        opts.simd = int(simd)
+        n = C.c_size_t(0)
+        rc = _L.fz_compress_device(self.h, C.byref(config.conf), dt, d_data.ptr, d_pre.ptr, d_pre.nbytes, C.byref(n),
+                                   C.byref(opts))
+        if rc:
+            _raise(rc, self._err())
+        return n.value, opts
+
+    def decompress_device(self, config, dtype, h_pre, d_pre, d_out, simd=1):
+        """fz_decompress_device: pre-zstd buffer (host copy h_pre, device copy d_pre or None) -> field in d_out."""
+        dt = _DT[np.dtype(dtype)]
+        b = np.ascontiguousarray(np.frombuffer(bytes(h_pre), dtype=np.uint8) if not isinstance(h_pre, np.ndarray)
+                                 else h_pre, dtype=np.uint8)
+        opts = fz_opts()
+        _L.fz_opts_init(C.byref(opts))
+        opts.simd = int(simd)
+        es = np.dtype(dtype).itemsize
+        rc = _L.fz_decompress_device(self.h, C.byref(config.conf), dt, b.ctypes.data_as(_u8p),
+                                     d_pre.ptr if d_pre is not None else None, b.size, d_out.ptr, d_out.nbytes // es,
+                                     C.byref(opts))
+        if rc:
+            _raise(rc, self._err())
+        return opts
+
+    def prezstd_bound(self, config, dtype):
+        return int(_L.fz_prezstd_bound(C.byref(config.conf), _DT[np.dtype(dtype)]))
+
+    def synchronize(self):
+        rc = _L.fz_synchronize(self.h)
+        if rc:
+            _raise(rc, self._err())
+
+
 _default_ctx = None
 
 
@@ -374,4 +479,4 @@ class falcons:
 
 
 __all__ = ["falcons", "falconsConfig", "falconsErrorBoundMode", "falconsAlgorithm", "falconsInterpAlgorithm",
-           "Context", "read_config", "FalconsError", "FalconsUnsupported", "LIB_PATH", "EXPORTED_SYMBOLS"]
+           "Context", "DeviceBuffer", "zstd_version", "read_config", "FalconsError", "FalconsUnsupported", "LIB_PATH", "EXPORTED_SYMBOLS"]

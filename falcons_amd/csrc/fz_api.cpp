@@ -410,14 +410,21 @@ struct Timer {
 
 // The compression pipeline on the GPU. d_data is device memory holding the field; it is
 // overwritten with the reconstructed values.
-void gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, const void* host_data, int simd,
-                  std::vector<uint8_t>& prebuf, fz_opts* o) {
+// Byte size of decomposition.save without the unpredictable values (IDH:241-252,
+// LinearQuantizer.hpp:142-152).
+size_t decomp_head_bytes(int N) { return 8 * (size_t)N + 4 + 4 + 4 + 8 + 8 + 8 + 1 + 8 + 4 + 4; }
+
+// The compression pipeline on the GPU. d_data (device) holds the field and is overwritten with the
+// reconstructed values. The buffer handed to the zstd stage is assembled in device memory, at d_pre
+// when given (capacity pre_cap) or in the context workspace; *pre_out receives its address.
+size_t gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, const void* host_data, int simd,
+                    uint8_t* d_pre, size_t pre_cap, uint8_t** pre_out, fz_opts* o) {
     hipStream_t st = ctx->stream;
     const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
     const uint64_t n = c.num;
     DevArena& ws = ctx->arena;
     ws.reset();
-    const size_t need = n * es * 2 + n * 2 + n * 8 + (n * 3 + (1 << 20)) + select_escapes_scratch(n) +
+    const size_t need = n * es * 3 + n * 2 + n * 8 + entropy_encode_bound(n) + select_escapes_scratch(n) +
                         entropy_encode_workspace(n) + (8u << 20);
     ws.reserve(need);
     Timer tm{st, ctx->ev0, ctx->ev1};
@@ -477,18 +484,22 @@ void gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, const void
     for (const auto& p : P.passes) interp_pass(dtype, false, p, d_data, codes, escval, lv, avx2, nullptr, 0, nullptr, st);
     select_escapes(codes, P.ncodes, esc_idx, nesc_dev, sel_tmp, sel_bytes, st);
     gather_escapes(dtype, esc_idx, nesc_dev, P.ncodes, escval, static_cast<uint8_t*>(unpred) + P.nanchors * es, st);
-    uint8_t* enc = ws.take<uint8_t>(P.ncodes * 3 + (1 << 20));
-    EntropyStats es_stats;
-    const size_t enc_len = entropy_encode(codes, P.ncodes, enc, P.ncodes * 3 + (1 << 20), ws, st, &es_stats);
     uint64_t nesc = 0;
     FZ_HIP(hipMemcpyAsync(&nesc, nesc_dev, 8, hipMemcpyDeviceToHost, st));
-    const double tdev = tm.stop();
+    FZ_HIP(hipStreamSynchronize(st));
     const uint64_t nun = P.nanchors + nesc;
     // pre-zstd buffer: decomposition.save, nCodes, encoder.save + payload (FGC:61-83, IDH:241-252,
-    // LinearQuantizer.hpp:142-152)
-    const size_t head = 8 * c.N + 4 + 4 + 4 + 8 + 8 + 8 + 1 + 8 + 4 + 4;
-    prebuf.resize(head + nun * es + 8 + enc_len);
-    uint8_t* p = prebuf.data();
+    // LinearQuantizer.hpp:142-152); the encoder writes its bytes in place
+    const size_t head = decomp_head_bytes(c.N);
+    const size_t enc_cap = entropy_encode_bound(P.ncodes);
+    if (!d_pre) {
+        pre_cap = head + nun * es + 8 + enc_cap;
+        d_pre = ws.take<uint8_t>(pre_cap);
+    } else if (pre_cap < head + nun * es + 8 + enc_cap) {
+        throw ArgError("The buffer for compressed data is not large enough.");
+    }
+    std::vector<uint8_t> hbuf(head);
+    uint8_t* p = hbuf.data();
     uint64_t max_dim = 1;
     for (int i = 0; i < c.N; i++) { put<uint64_t>(p, c.dims[i]); max_dim = std::max<uint64_t>(max_dim, c.dims[i]); }
     put<uint32_t>(p, (uint32_t)max_dim);
@@ -501,18 +512,114 @@ void gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, const void
     put<double>(p, c.abs_eb);
     put<int32_t>(p, c.quantbin_cnt / 2);
     put<int32_t>(p, (int32_t)nun);
-    FZ_HIP(hipMemcpyAsync(p, unpred, nun * es, hipMemcpyDeviceToHost, st));
-    p += nun * es;
-    put<uint64_t>(p, P.ncodes);
-    FZ_HIP(hipMemcpyAsync(p, enc, enc_len, hipMemcpyDeviceToHost, st));
-    FZ_HIP(hipStreamSynchronize(st));
+    FZ_HIP(hipMemcpyAsync(d_pre, hbuf.data(), head, hipMemcpyHostToDevice, st));
+    if (nun) FZ_HIP(hipMemcpyAsync(d_pre + head, unpred, nun * es, hipMemcpyDeviceToDevice, st));
+    const uint64_t nq = P.ncodes;
+    FZ_HIP(hipMemcpyAsync(d_pre + head + nun * es, &nq, 8, hipMemcpyHostToDevice, st));
+    EntropyStats es_stats;
+    const size_t enc_len = entropy_encode(codes, P.ncodes, d_pre + head + nun * es + 8, enc_cap, ws, st, &es_stats);
+    const double tdev = tm.stop();
+    const size_t total = head + nun * es + 8 + enc_len;
+    if (pre_out) *pre_out = d_pre;
     if (o) {
         o->t_device_ms = tdev;
         o->ncodes = P.ncodes;
         o->nunpred = nun;
-        o->prezstd_bytes = prebuf.size();
+        o->prezstd_bytes = total;
         o->method = es_stats.method;
         o->use_delta = es_stats.use_delta;
+    }
+    return total;
+}
+
+// The decompression pipeline on the GPU, from the buffer the zstd stage returns: h_pre is its host
+// copy (headers and offsets are read from it), d_pre its device copy when the caller already holds
+// one (otherwise the needed parts are uploaded first, outside the timed region). *d_out_io is the
+// device field (allocated in the workspace when null).
+void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h_pre, const uint8_t* d_pre,
+                    size_t pre_len, void** d_out_io, fz_opts* o) {
+    if (c.N > 3) throw Unsupported("4-D fields are not on this path");
+    hipStream_t st = ctx->stream;
+    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    const uint64_t n = c.num;
+    // decomposition.load (IDH:256-267, LinearQuantizer.hpp:154-176)
+    const uint8_t* p = h_pre;
+    const uint8_t* bend = h_pre + pre_len;
+    auto need = [&](size_t k) {
+        if ((size_t)(bend - p) < k) throw ArgError("Truncated Falcons payload");
+    };
+    uint64_t dims[4];
+    need(decomp_head_bytes(c.N));
+    for (int i = 0; i < c.N; i++) dims[i] = get<uint64_t>(p);
+    (void)get<uint32_t>(p);
+    const int32_t algo = get<int32_t>(p);
+    const int32_t dir = get<int32_t>(p);
+    const uint64_t anchor = get<uint64_t>(p);
+    const double alpha = get<double>(p), beta = get<double>(p);
+    if (get<uint8_t>(p) != 2) throw ArgError("LinearQuantizer uid mismatch");
+    const double eb = get<double>(p);
+    (void)get<int32_t>(p);
+    const int32_t nun = get<int32_t>(p);
+    uint64_t prod = 1;
+    for (int i = 0; i < c.N; i++) prod *= dims[i];
+    if (prod != n) throw ArgError("field shape does not match the header");
+    need((size_t)nun * es + 8);
+    const size_t un_off = (size_t)(p - h_pre);
+    p += (size_t)nun * es;
+    const uint64_t ncodes = get<uint64_t>(p);
+    const size_t enc_off = (size_t)(p - h_pre);
+    const size_t enc_len = pre_len - enc_off;
+    const Plan P = make_plan(c.N, dims, (long)anchor, algo, dir);
+    if (P.ncodes != ncodes) throw ArgError("code count does not match the field shape");
+    DevArena& ws = ctx->arena;
+    ws.reset();
+    ws.reserve(n * es + ncodes * 2 + ncodes * 8 + (size_t)nun * es + pre_len + select_escapes_scratch(ncodes) +
+               entropy_decode_workspace(ncodes, enc_len) + (16u << 20));
+    void* d_out = *d_out_io ? *d_out_io : ws.take<uint8_t>(n * es);
+    *d_out_io = d_out;
+    const uint8_t* d_enc;
+    void* d_unpred = ws.take<uint8_t>((size_t)nun * es + 16);
+    if (d_pre) {
+        d_enc = d_pre + enc_off;
+    } else {
+        uint8_t* e = ws.take<uint8_t>(enc_len + 16);
+        FZ_HIP(hipMemcpyAsync(e, h_pre + enc_off, enc_len, hipMemcpyHostToDevice, st));
+        if (nun) FZ_HIP(hipMemcpyAsync(d_unpred, h_pre + un_off, (size_t)nun * es, hipMemcpyHostToDevice, st));
+        d_enc = e;
+    }
+    double* eb_dev = ws.take<double>(1);
+    FZ_HIP(hipMemcpyAsync(eb_dev, &eb, 8, hipMemcpyHostToDevice, st));
+    std::vector<double> ratio;
+    const int rmode = ratio_table(alpha, beta, P.levels, ratio);
+    double* ratio_dev = ws.take<double>(ratio.size());
+    FZ_HIP(hipMemcpyAsync(ratio_dev, ratio.data(), ratio.size() * 8, hipMemcpyHostToDevice, st));
+    LevelEb* lv = ws.take<LevelEb>(std::max(P.levels, 1));
+    int16_t* codes = ws.take<int16_t>(ncodes + 16);
+    uint64_t* esc_idx = ws.take<uint64_t>(ncodes + 1);
+    uint64_t* nesc_dev = ws.take<uint64_t>(1);
+    const size_t sel_bytes = select_escapes_scratch(ncodes);
+    void* sel_tmp = ws.take<uint8_t>(sel_bytes + 1);
+    FZ_HIP(hipStreamSynchronize(st));
+    Timer tm{st, ctx->ev0, ctx->ev1};
+    tm.start();
+    if (d_pre && nun) FZ_HIP(hipMemcpyAsync(d_unpred, d_pre + un_off, (size_t)nun * es, hipMemcpyDeviceToDevice, st));
+    entropy_decode(h_pre + enc_off, d_enc, enc_len, ncodes, codes, ws, st);
+    select_escapes(codes, ncodes, esc_idx, nesc_dev, sel_tmp, sel_bytes, st);
+    uint64_t nesc = 0;
+    FZ_HIP(hipMemcpyAsync(&nesc, nesc_dev, 8, hipMemcpyDeviceToHost, st));
+    FZ_HIP(hipStreamSynchronize(st));
+    if (P.nanchors + nesc != (uint64_t)nun) throw ArgError("unpredictable count does not match the code stream");
+    if (P.levels > 0) level_constants(eb_dev, ratio_dev, rmode, P.levels, lv, st);
+    const void* esc_vals = static_cast<const uint8_t*>(d_unpred) + P.nanchors * es;
+    if (P.anchor) interp_anchors(dtype, true, d_out, d_unpred, P.acnt, P.astr, st);
+    else interp_first(dtype, true, d_out, codes, nullptr, eb_dev, esc_idx, nesc, esc_vals, st);
+    for (const auto& ps : P.passes)
+        interp_pass(dtype, true, ps, d_out, codes, nullptr, lv, 1, esc_idx, nesc, esc_vals, st);
+    const double tdev = tm.stop();
+    if (o) {
+        o->t_device_ms = tdev;
+        o->ncodes = ncodes;
+        o->nunpred = (uint64_t)nun;
     }
 }
 
@@ -569,6 +676,9 @@ extern "C" const char* fz_last_error(const fz_ctx* c) { return c ? c->err.c_str(
     } catch (const GpuError& e) {                                         \
         set_err(ctx, e.what());                                           \
         return FZ_ERR_GPU;                                                \
+    } catch (const std::length_error& e) {                                \
+        set_err(ctx, e.what());                                           \
+        return FZ_ERR_INVALID_ARGUMENT;                                   \
     } catch (const std::exception& e) {                                   \
         set_err(ctx, e.what());                                           \
         return FZ_ERR_RUNTIME;                                            \
@@ -624,7 +734,11 @@ extern "C" int fz_compress(fz_ctx* ctx, const fz_config* conf, int dtype, void* 
             } else {
                 std::vector<uint8_t> pre;
                 try {
-                    gpu_compress(ctx, c, dtype, d_data, host_view, o->simd, pre, o);
+                    uint8_t* d_pre = nullptr;
+                    const size_t plen_pre = gpu_compress(ctx, c, dtype, d_data, host_view, o->simd, nullptr, 0, &d_pre, o);
+                    pre.resize(plen_pre);
+                    FZ_HIP(hipMemcpyAsync(pre.data(), d_pre, plen_pre, hipMemcpyDeviceToHost, ctx->stream));
+                    FZ_HIP(hipStreamSynchronize(ctx->stream));
                 } catch (const Unsupported& u) {
                     if (std::string(u.what()) != "zero bound selects the lossless mode") throw;
                     lossless = true;
@@ -707,79 +821,89 @@ extern "C" int fz_decompress(fz_ctx* ctx, const uint8_t* cmp, size_t len, int dt
             return FZ_OK;
         }
         if (c.cmpr_algo != 2) throw Unsupported("only the interpolation algorithms are on this path");
-        if (c.N > 3) throw Unsupported("4-D fields are not on this path");
-        // decomposition.load (IDH:256-267, LinearQuantizer.hpp:154-176)
-        const uint8_t* p = buf.data();
-        const uint8_t* bend = buf.data() + buf.size();
-        auto need = [&](size_t k) {
-            if ((size_t)(bend - p) < k) throw ArgError("Truncated Falcons payload");
-        };
-        uint64_t dims[4];
-        need(8 * c.N + 45);
-        for (int i = 0; i < c.N; i++) dims[i] = get<uint64_t>(p);
-        (void)get<uint32_t>(p);
-        const int32_t algo = get<int32_t>(p);
-        const int32_t dir = get<int32_t>(p);
-        const uint64_t anchor = get<uint64_t>(p);
-        const double alpha = get<double>(p), beta = get<double>(p);
-        if (get<uint8_t>(p) != 2) throw ArgError("LinearQuantizer uid mismatch");
-        const double eb = get<double>(p);
-        (void)get<int32_t>(p);
-        const int32_t nun = get<int32_t>(p);
-        need((size_t)nun * es + 8);
-        const uint8_t* unpred_h = p;
-        p += (size_t)nun * es;
-        const uint64_t ncodes = get<uint64_t>(p);
-        const uint8_t* enc_h = p;
-        const size_t enc_len = (size_t)(bend - p);
-        uint64_t pdims[4];
-        for (int i = 0; i < c.N; i++) pdims[i] = dims[i];
-        const Plan P = make_plan(c.N, pdims, (long)anchor, algo, dir);
-        if (P.ncodes != ncodes) throw ArgError("code count does not match the field shape");
-        DevArena& ws = ctx->arena;
-        ws.reset();
-        ws.reserve(n * es + ncodes * 2 + ncodes * 8 + (size_t)nun * es + enc_len * 2 + select_escapes_scratch(ncodes) +
-                   entropy_decode_workspace(ncodes, enc_len) + (16u << 20));
-        void* d_out = out_on_device ? out : ws.take<uint8_t>(n * es);
-        uint8_t* d_enc = ws.take<uint8_t>(enc_len + 16);
-        void* d_unpred = ws.take<uint8_t>((size_t)nun * es + 16);
-        FZ_HIP(hipMemcpyAsync(d_enc, enc_h, enc_len, hipMemcpyHostToDevice, st));
-        if (nun) FZ_HIP(hipMemcpyAsync(d_unpred, unpred_h, (size_t)nun * es, hipMemcpyHostToDevice, st));
-        double* eb_dev = ws.take<double>(1);
-        FZ_HIP(hipMemcpyAsync(eb_dev, &eb, 8, hipMemcpyHostToDevice, st));
-        std::vector<double> ratio;
-        const int rmode = ratio_table(alpha, beta, P.levels, ratio);
-        double* ratio_dev = ws.take<double>(ratio.size());
-        FZ_HIP(hipMemcpyAsync(ratio_dev, ratio.data(), ratio.size() * 8, hipMemcpyHostToDevice, st));
-        LevelEb* lv = ws.take<LevelEb>(std::max(P.levels, 1));
-        int16_t* codes = ws.take<int16_t>(ncodes + 16);
-        uint64_t* esc_idx = ws.take<uint64_t>(ncodes + 1);
-        uint64_t* nesc_dev = ws.take<uint64_t>(1);
-        const size_t sel_bytes = select_escapes_scratch(ncodes);
-        void* sel_tmp = ws.take<uint8_t>(sel_bytes + 1);
-        FZ_HIP(hipStreamSynchronize(st));
-        Timer tm{st, ctx->ev0, ctx->ev1};
-        tm.start();
-        entropy_decode(enc_h, d_enc, enc_len, ncodes, codes, ws, st);
-        select_escapes(codes, ncodes, esc_idx, nesc_dev, sel_tmp, sel_bytes, st);
-        uint64_t nesc = 0;
-        FZ_HIP(hipMemcpyAsync(&nesc, nesc_dev, 8, hipMemcpyDeviceToHost, st));
-        FZ_HIP(hipStreamSynchronize(st));
-        if (P.nanchors + nesc != (uint64_t)nun) throw ArgError("unpredictable count does not match the code stream");
-        if (P.levels > 0) level_constants(eb_dev, ratio_dev, rmode, P.levels, lv, st);
-        const void* esc_vals = static_cast<const uint8_t*>(d_unpred) + P.nanchors * es;
-        if (P.anchor) interp_anchors(dtype, true, d_out, d_unpred, P.acnt, P.astr, st);
-        else interp_first(dtype, true, d_out, codes, nullptr, eb_dev, esc_idx, nesc, esc_vals, st);
-        for (const auto& ps : P.passes)
-            interp_pass(dtype, true, ps, d_out, codes, nullptr, lv, 1, esc_idx, nesc, esc_vals, st);
-        o->t_device_ms = tm.stop();
+        void* d_out = out_on_device ? out : nullptr;
+        gpu_decompress(ctx, c, dtype, buf.data(), nullptr, buf.size(), &d_out, o);
         if (!out_on_device) {
             FZ_HIP(hipMemcpyAsync(out, d_out, n * es, hipMemcpyDeviceToHost, st));
             FZ_HIP(hipStreamSynchronize(st));
         }
-        o->ncodes = ncodes;
-        o->nunpred = (uint64_t)nun;
         o->t_total_ms = now_ms() - t0;
+        return FZ_OK;
+    })
+}
+
+extern "C" int fz_compress_device(fz_ctx* ctx, fz_config* conf, int dtype, void* d_data, uint8_t* d_pre,
+                                  size_t pre_cap, size_t* pre_len, fz_opts* opts) {
+    FZ_GUARD({
+        const double t0 = now_ms();
+        fz_opts local;
+        fz_opts_init(&local);
+        fz_opts* o = opts ? opts : &local;
+        FZ_HIP(hipSetDevice(ctx->device));
+        fz_config c = *conf;
+        check_supported(c, dtype);
+        if (c.cmpr_algo == 4 || (c.eb_mode == 0 && c.abs_eb == 0))
+            throw Unsupported("the lossless mode has no device-resident form");
+        if (c.cmpr_algo == 1) throw Unsupported("ALGO_INTERP_LORENZO tunes on the host; use fz_compress");
+        *pre_len = gpu_compress(ctx, c, dtype, d_data, nullptr, o->simd, d_pre, pre_cap, nullptr, o);
+        *conf = c;
+        o->t_total_ms = now_ms() - t0;
+        return FZ_OK;
+    })
+}
+
+extern "C" size_t fz_prezstd_bound(const fz_config* c, int dtype) {
+    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    return decomp_head_bytes(c->N) + c->num * es + 8 + entropy_encode_bound(c->num);
+}
+
+extern "C" int fz_decompress_device(fz_ctx* ctx, const fz_config* conf, int dtype, const uint8_t* h_pre,
+                                    const uint8_t* d_pre, size_t pre_len, void* d_out, size_t out_cap_elems,
+                                    fz_opts* opts) {
+    FZ_GUARD({
+        const double t0 = now_ms();
+        fz_opts local;
+        fz_opts_init(&local);
+        fz_opts* o = opts ? opts : &local;
+        FZ_HIP(hipSetDevice(ctx->device));
+        if (!d_out || conf->num > out_cap_elems) throw ArgError("The decompression buffer is too small");
+        void* out = d_out;
+        gpu_decompress(ctx, *conf, dtype, h_pre, d_pre, pre_len, &out, o);
+        o->t_total_ms = now_ms() - t0;
+        return FZ_OK;
+    })
+}
+
+extern "C" int fz_device_alloc(fz_ctx* ctx, size_t bytes, void** out) {
+    FZ_GUARD({
+        FZ_HIP(hipSetDevice(ctx->device));
+        FZ_HIP(hipMalloc(out, bytes ? bytes : 1));
+        return FZ_OK;
+    })
+}
+
+extern "C" int fz_device_free(fz_ctx* ctx, void* p) {
+    FZ_GUARD({
+        FZ_HIP(hipSetDevice(ctx->device));
+        FZ_HIP(hipFree(p));
+        return FZ_OK;
+    })
+}
+
+extern "C" int fz_memcpy(fz_ctx* ctx, void* dst, const void* src, size_t bytes, int kind) {
+    FZ_GUARD({
+        FZ_HIP(hipSetDevice(ctx->device));
+        const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        FZ_HIP(hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
+        FZ_HIP(hipStreamSynchronize(ctx->stream));
+        return FZ_OK;
+    })
+}
+
+extern "C" int fz_synchronize(fz_ctx* ctx) {
+    FZ_GUARD({
+        FZ_HIP(hipSetDevice(ctx->device));
+        FZ_HIP(hipStreamSynchronize(ctx->stream));
         return FZ_OK;
     })
 }

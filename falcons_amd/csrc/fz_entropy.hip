@@ -11,6 +11,7 @@
 // mantissa bit scatter, block framing, run expansion, prefix sums) run across the whole chip. The
 // tANS chains themselves are sequential state machines; each chain is run by one lane here and all
 // chains of a call run concurrently.
+#include <stdexcept>
 #include <hipcub/hipcub.hpp>
 
 #include "fz_common.h"
@@ -689,8 +690,15 @@ static void exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, De
     FZ_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int64_t)n, st));
 }
 
-// fse_enc over a class-byte array (RleFse:150-176); returns the framed length written to out.
-static uint64_t fse_enc_blocks(const uint8_t* src, uint64_t n, uint8_t* out, DevArena& ws, hipStream_t st) {
+// Every write into the caller's buffer is preceded by a host-side check of its capacity.
+static void need_room(uint64_t want, uint64_t cap) {
+    if (want > cap) throw std::length_error("The buffer for compressed data is not large enough.");
+}
+
+// fse_enc over a class-byte array (RleFse:150-176); returns the framed length written to out
+// (capacity cap).
+static uint64_t fse_enc_blocks(const uint8_t* src, uint64_t n, uint8_t* out, uint64_t cap, DevArena& ws,
+                               hipStream_t st) {
     if (n == 0) return 0;
     const uint64_t nb = (n + kFblk - 1) / kFblk;
     uint8_t* tmp = ws.take<uint8_t>(nb * (kFblk + 1024));
@@ -703,6 +711,7 @@ static uint64_t fse_enc_blocks(const uint8_t* src, uint64_t n, uint8_t* out, Dev
     d2h(hs.data(), sizes, nb, st);
     uint64_t acc = 0;
     for (uint64_t b = 0; b < nb; b++) { ho[b] = acc; acc += hs[b]; }
+    need_room(acc, cap);
     h2d(offs, ho.data(), nb, st);
     hipLaunchKernelGGL(k_pack_slots, dim3(64, (unsigned)nb), dim3(256), 0, st, slots, (uint64_t)(kFblk + 1024), sizes,
                        offs, out);
@@ -712,7 +721,9 @@ static uint64_t fse_enc_blocks(const uint8_t* src, uint64_t n, uint8_t* out, Dev
 
 static void put64(uint8_t*& p, uint64_t v) { memcpy(p, &v, 8); p += 8; }
 
-static size_t encode_method0(const int16_t* bins, uint64_t n, uint8_t* out, DevArena& ws, hipStream_t st) {
+static size_t encode_method0(const int16_t* bins, uint64_t n, uint8_t* out, uint64_t cap, DevArena& ws,
+                             hipStream_t st) {
+    need_room(48, cap);
     uint64_t* pos = ws.take<uint64_t>(n + 1);
     uint64_t* nsel = ws.take<uint64_t>(1);
     size_t bytes = 0;
@@ -751,8 +762,9 @@ static size_t encode_method0(const int16_t* bins, uint64_t n, uint8_t* out, DevA
     FZ_HIP(hipGetLastError());
     // class streams go after the 48-byte head
     uint8_t* body = out + 48;
-    const uint64_t s_rcl = fse_enc_blocks(rcl, nr, body, ws, st);
-    const uint64_t s_vcl = fse_enc_blocks(vcl, nv, body + s_rcl, ws, st);
+    const uint64_t s_rcl = fse_enc_blocks(rcl, nr, body, cap - 48, ws, st);
+    const uint64_t s_vcl = fse_enc_blocks(vcl, nv, body + s_rcl, cap - 48 - s_rcl, ws, st);
+    need_room(48 + s_rcl + s_vcl + rbp + vbp, cap);
     FZ_HIP(hipMemcpyAsync(body + s_rcl + s_vcl, rman, rbp, hipMemcpyDeviceToDevice, st));
     FZ_HIP(hipMemcpyAsync(body + s_rcl + s_vcl + rbp, vman, vbp, hipMemcpyDeviceToDevice, st));
     uint8_t head[48], *p = head;
@@ -762,7 +774,8 @@ static size_t encode_method0(const int16_t* bins, uint64_t n, uint8_t* out, DevA
     return 48 + s_rcl + s_vcl + rbp + vbp;
 }
 
-static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, DevArena& ws, hipStream_t st) {
+static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint64_t cap, DevArena& ws,
+                             hipStream_t st) {
     uint64_t S = n / kO1Seg;
     if (S < 1) S = 1;
     if (S > 16) S = 16;
@@ -850,6 +863,12 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, DevA
     d2h(hblen.data(), blen, nch, st);
     d2h(holen.data(), olen, S, st);
     // assemble: [u64 S] then per segment [n][K][blen x4][novf x4][nf][streams][ovf block]
+    uint64_t total = 8;
+    for (uint64_t g = 0; g < S; g++) {
+        total += 8 * 10 + kO1Ctx * kO1Sym * 2 + holen[g];
+        for (int k = 0; k < kO1K; k++) total += hblen[g * kO1K + k];
+    }
+    need_room(total, cap);
     uint8_t* p = out;
     {
         uint64_t s = S;
@@ -882,7 +901,7 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, DevA
 
 size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out_cap, DevArena& ws, hipStream_t st,
                       EntropyStats* stats) {
-    (void)out_cap;
+    need_room(5, out_cap);
     // preprocess_encode: 256 windows of 32 codes (RleFse:850-895)
     uint64_t NW = 256, WL = 32;
     if (n < NW * WL) { WL = n / NW; if (!WL) WL = 1; }
@@ -909,7 +928,8 @@ size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out
         FZ_HIP(hipGetLastError());
         bins = d;
     }
-    const size_t body = (method == 1) ? encode_method1(bins, n, out + 5, ws, st) : encode_method0(bins, n, out + 5, ws, st);
+    const size_t body = (method == 1) ? encode_method1(bins, n, out + 5, out_cap - 5, ws, st)
+                                      : encode_method0(bins, n, out + 5, out_cap - 5, ws, st);
     return 5 + body;
 }
 

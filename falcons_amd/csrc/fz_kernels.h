@@ -65,9 +65,14 @@ void gather_escapes(int dtype, const uint64_t* idx, const uint64_t* nsel_dev, ui
 struct EntropyStats {
     int method = 0, use_delta = 0;
 };
+// Capacity that always holds the encoder output for n codes: order-1 tANS spends at most 13 bits on
+// a symbol and 2 side-stream bytes on an escaped code; RLE+FSE at most 3 bytes per code; plus the
+// per-segment and per-block headers and tables.
+inline size_t entropy_encode_bound(uint64_t n) { return (size_t)n * 4 + (1u << 20); }
 // Device workspace the encoder needs for n codes.
 size_t entropy_encode_workspace(uint64_t n);
-// Encodes n codes (device) into out (device, capacity out_cap). Returns the byte count of
+// Encodes n codes (device) into out (device, capacity out_cap; std::length_error, before any write
+// past it, when the output would not fit). Returns the byte count of
 // [u8 method|delta<<7][u16 lo][u16 mode][payload] (RleFseEncoder::save + ::encode).
 size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out_cap, DevArena& ws, hipStream_t st,
                       EntropyStats* stats);

@@ -95,6 +95,38 @@ int fz_compress(fz_ctx *ctx, const fz_config *conf, int dtype, void *data, int d
 int fz_decompress(fz_ctx *ctx, const uint8_t *cmp, size_t len, int dtype, void *out, size_t out_cap_elems,
                   int out_on_device, fz_config *conf_out, fz_opts *opts);
 
+/* Device-resident stages of the same pipeline, for callers that keep fields in HBM and run the zstd
+ * stage themselves (or batch it).
+ *
+ * fz_compress_device = FalconsGenericCompressor::compress up to the lossless stage
+ * (include/Falcons/compressor/FalconsGenericCompressor.hpp:57-78): decomposition.save, the
+ * unpredictable list, the code count and the encoder output are written to d_pre (device memory,
+ * capacity pre_cap, at least fz_prezstd_bound bytes); *pre_len receives their length. conf is
+ * updated as Falcons_compress_Interp updates it (absolute bound, anchor stride;
+ * include/Falcons/api/impl/FalconsAlgoInterp.hpp:36-40). d_data (device) is overwritten with the
+ * reconstructed values. ALGO_INTERP only: ALGO_INTERP_LORENZO's tuner samples on the host
+ * (use fz_compress). Lossless configurations return FZ_ERR_UNSUPPORTED. */
+size_t fz_prezstd_bound(const fz_config *c, int dtype);
+int fz_compress_device(fz_ctx *ctx, fz_config *conf, int dtype, void *d_data, uint8_t *d_pre, size_t pre_cap,
+                       size_t *pre_len, fz_opts *opts);
+
+/* FalconsGenericCompressor::decompress after the lossless stage (FalconsGenericCompressor.hpp:
+ * 109-124): h_pre is the host copy of the pre-zstd buffer (headers are parsed from it), d_pre its
+ * device copy (may be NULL: then the payload is uploaded first). conf is the stream's configuration
+ * (fz_read_config). d_out (device) receives conf->num values. */
+int fz_decompress_device(fz_ctx *ctx, const fz_config *conf, int dtype, const uint8_t *h_pre, const uint8_t *d_pre,
+                         size_t pre_len, void *d_out, size_t out_cap_elems, fz_opts *opts);
+
+/* Device memory helpers on the context's GPU, so that callers need no other GPU runtime binding.
+ * fz_memcpy kind: 0 host->device, 1 device->host, 2 device->device; synchronous. */
+int fz_device_alloc(fz_ctx *ctx, size_t bytes, void **out);
+int fz_device_free(fz_ctx *ctx, void *p);
+int fz_memcpy(fz_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
+int fz_synchronize(fz_ctx *ctx);
+
+/* ZSTD_versionNumber() of the libzstd the host stage loaded (0 if none could be loaded). */
+unsigned fz_zstd_version(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,176 @@
+"""Writes the golden vectors under tests/golden/ from the reference's own sources.
+
+The reference's entropy coder (include/Falcons/encoder/RleFseEncoder.hpp with the vendored
+zfse/*.c), its scalar LinearQuantizer (include/Falcons/quantizer/LinearQuantizer.hpp) and its
+interpolators (include/Falcons/utils/Interpolators.hpp) compile from their own files; oracle/Makefile
+builds them in place into oracle/_ref/libfalcons_ref.so (`make -C oracle ref`). This script calls
+that build on seeded inputs and stores inputs and outputs:
+
+  fse.npz        FSE_compress (zfse/fse_compress.c:632-694) on byte blocks: return code + bytes
+  rlefse.npz     RleFseEncoder save+encode on int16 code arrays: encoded bytes
+  rlefse_big.json  the same for arrays too large to store: integer-only generator recipe, length,
+                 sha256 of the encoded bytes (regenerated bit-exactly by codes_from_recipe)
+  quantizer.npz  LinearQuantizer::quantize_and_overwrite and recover (f32 and f64)
+  interp.npz     interp_linear / linear1 / quad_1 / quad_2 / cubic (f32 and f64)
+
+Run from the repository root: python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+from golden_inputs import codes_from_recipe  # noqa: E402
+from oracle_lib import Ref  # noqa: E402
+
+BIG_RECIPES = [
+    # > 2^20 codes: sampled order-1 histograms; > 16 Mi codes: two tANS segments
+    {"kind": "laplace", "n": 1_300_000, "seed": 41, "scale": 3},
+    {"kind": "escapes", "n": 2_500_000, "seed": 42, "scale": 5},
+    {"kind": "laplace", "n": 17_000_000, "seed": 43, "scale": 2},
+    {"kind": "sparse", "n": 3_000_000, "seed": 44, "scale": 1},
+    {"kind": "runs", "n": 2_000_000, "seed": 45, "scale": 3},
+]
+
+SMALL_RECIPES = [
+    {"kind": "zeros", "n": 100_000, "seed": 1, "scale": 0},
+    {"kind": "sparse", "n": 100_000, "seed": 2, "scale": 1},
+    {"kind": "sparse", "n": 5_000, "seed": 3, "scale": 2},
+    {"kind": "laplace", "n": 200_000, "seed": 4, "scale": 2},
+    {"kind": "laplace", "n": 76_799, "seed": 5, "scale": 4},
+    {"kind": "laplace", "n": 76_800, "seed": 6, "scale": 4},
+    {"kind": "escapes", "n": 150_000, "seed": 7, "scale": 6},
+    {"kind": "runs", "n": 120_000, "seed": 8, "scale": 3},
+    {"kind": "runs", "n": 9_000, "seed": 9, "scale": 1},
+    {"kind": "laplace", "n": 1, "seed": 10, "scale": 2},
+    {"kind": "laplace", "n": 2, "seed": 11, "scale": 2},
+    {"kind": "laplace", "n": 31, "seed": 12, "scale": 2},
+    {"kind": "escapes", "n": 8192, "seed": 13, "scale": 9},
+    {"kind": "wide", "n": 90_000, "seed": 14, "scale": 0},
+]
+
+
+def fse_blocks():
+    rng = np.random.default_rng(7)
+    blocks = []
+    for n in (1, 2, 3, 100, 1000, 4097, 65536):
+        blocks.append(np.full(n, 5, dtype=np.uint8))                                     # one symbol
+        blocks.append(rng.integers(0, 256, n, dtype=np.uint8))                           # incompressible
+        blocks.append(np.minimum(rng.geometric(0.3, n) - 1, 255).astype(np.uint8))       # skewed
+        blocks.append(np.minimum(rng.geometric(0.05, n) - 1, 255).astype(np.uint8))      # wide
+        blocks.append(rng.choice(np.array([0, 1, 2, 17], np.uint8), n, p=[0.7, 0.2, 0.09, 0.01]))
+    for n in (65536, 30000):
+        b = np.zeros(n, np.uint8)
+        b[rng.integers(0, n, 3)] = 1                                                      # nearly RLE
+        blocks.append(b)
+        blocks.append((np.arange(n) % 3).astype(np.uint8))
+        blocks.append(np.minimum(rng.poisson(12, n), 255).astype(np.uint8))
+    return blocks
+
+
+def quantizer_points(dtype, count=1200):
+    rng = np.random.default_rng(11 if dtype == np.float32 else 12)
+    ebs = [1e-1, 1e-3, 1e-4, 6.1e-5, 1e-7] + ([1e-10, 3e-12] if dtype == np.float64 else [])
+    ori, pred, eb = [], [], []
+    for e in ebs:
+        base = rng.normal(0, 10, count).astype(dtype)
+        k = rng.integers(-40000, 40000, count)
+        frac = rng.uniform(-0.5, 0.5, count)
+        ori += list(base)
+        pred += list((base.astype(np.float64) - (k + frac) * 2 * e).astype(dtype))
+        eb += [e] * count
+        # points landing on the bound: err == +-eb after rounding
+        p = rng.normal(0, 1, 200).astype(dtype)
+        ori += list(p)
+        pred += list((p.astype(np.float64) - np.float64(e) * rng.choice([-1.0, 1.0], 200)).astype(dtype))
+        eb += [e] * 200
+    special = [np.nan, np.inf, -np.inf, 0.0, -0.0, 1e30, -1e30, np.finfo(dtype).tiny, np.finfo(dtype).max]
+    for s in special:
+        for pv in (0.0, 1.0, np.nan, np.inf):
+            ori.append(s)
+            pred.append(pv)
+            eb.append(1e-3)
+    return np.array(ori, dtype), np.array(pred, dtype), np.array(eb, np.float64)
+
+
+def interp_args(dtype, count=600):
+    rng = np.random.default_rng(21 if dtype == np.float32 else 22)
+    a = rng.normal(0, 1, (count, 4)) * rng.choice([1e-3, 1.0, 1e3, 1e7], (count, 1))
+    return a.astype(dtype)
+
+
+def main():
+    ref = Ref()
+    out = {}
+    # FSE
+    blocks = fse_blocks()
+    src = np.concatenate(blocks)
+    lens = np.array([b.size for b in blocks], np.int64)
+    rets, outs = [], []
+    for b in blocks:
+        r, o = ref.fse_compress(b)
+        rets.append(r)
+        outs.append(np.frombuffer(o, np.uint8))
+    np.savez_compressed(os.path.join(HERE, "fse.npz"), src=src, lens=lens, ret=np.array(rets, np.int64),
+                        out=np.concatenate(outs) if outs else np.zeros(0, np.uint8),
+                        out_lens=np.array([o.size for o in outs], np.int64))
+    out["fse"] = len(blocks)
+    # RLE/FSE + order-1 tANS, stored arrays
+    arrs, encs = [], []
+    for r in SMALL_RECIPES:
+        c = codes_from_recipe(r)
+        arrs.append(c)
+        encs.append(np.frombuffer(ref.rle_encode(c), np.uint8))
+        assert np.array_equal(ref.rle_decode(encs[-1].tobytes(), c.size), c)
+    np.savez_compressed(os.path.join(HERE, "rlefse.npz"),
+                        codes=np.concatenate(arrs), code_lens=np.array([a.size for a in arrs], np.int64),
+                        enc=np.concatenate(encs), enc_lens=np.array([e.size for e in encs], np.int64),
+                        recipes=np.array(json.dumps(SMALL_RECIPES)))
+    out["rlefse"] = len(arrs)
+    # large arrays: recipe + hash
+    big = []
+    for r in BIG_RECIPES:
+        c = codes_from_recipe(r)
+        e = ref.rle_encode(c)
+        big.append(dict(r, enc_len=len(e), enc_sha256=hashlib.sha256(e).hexdigest(), method=e[0] & 0x7F,
+                        delta=e[0] >> 7, codes_sha256=hashlib.sha256(c.tobytes()).hexdigest()))
+    with open(os.path.join(HERE, "rlefse_big.json"), "w") as f:
+        json.dump(big, f, indent=1)
+    out["rlefse_big"] = len(big)
+    # quantizer
+    q = {}
+    for dt, tag, fn in ((np.float32, "f32", ref.quantize_f32), (np.float64, "f64", ref.quantize_f64)):
+        o, p, e = quantizer_points(dt)
+        codes = np.zeros(o.size, np.int32)
+        dec = np.zeros(o.size, dt)
+        esc = np.zeros(o.size, np.int8)
+        for i in range(o.size):
+            codes[i], dec[i], esc[i] = fn(o[i], p[i], e[i])
+        q.update({f"{tag}_ori": o, f"{tag}_pred": p, f"{tag}_eb": e, f"{tag}_code": codes, f"{tag}_dec": dec,
+                  f"{tag}_esc": esc})
+        rec = np.array([(ref.recover_f32 if tag == "f32" else ref.recover_f64)(p[i], int(codes[i]), e[i])
+                        for i in range(o.size)], dt)
+        q[f"{tag}_recover"] = rec
+    np.savez_compressed(os.path.join(HERE, "quantizer.npz"), **q)
+    out["quantizer"] = int(q["f32_ori"].size + q["f64_ori"].size)
+    # interpolators
+    it = {}
+    for dt, tag, fn in ((np.float32, "f32", ref.interp_f32), (np.float64, "f64", ref.interp_f64)):
+        a = interp_args(dt)
+        res = np.zeros((5, a.shape[0]), dt)
+        for k in range(5):
+            for i in range(a.shape[0]):
+                res[k, i] = fn(k, *a[i])
+        it[f"{tag}_args"] = a
+        it[f"{tag}_out"] = res
+    np.savez_compressed(os.path.join(HERE, "interp.npz"), **it)
+    out["interp"] = 5 * 2 * 600
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -270,6 +270,30 @@ class Ref:
         return r, norm[: max_sym + 1]
 
 
+    def quantize_f32(self, ori, pred, eb):
+        """LinearQuantizer<float>::quantize_and_overwrite after set_eb: (code, value left, escaped)."""
+        o, u = C.c_float(0), C.c_int(0)
+        q = self.L.ref_quantize_f32(float(ori), float(pred), float(eb), 1, C.byref(o), C.byref(u))
+        return q, o.value, u.value
+
+    def quantize_f64(self, ori, pred, eb):
+        o, u = C.c_double(0), C.c_int(0)
+        q = self.L.ref_quantize_f64(float(ori), float(pred), float(eb), 1, C.byref(o), C.byref(u))
+        return q, o.value, u.value
+
+    def recover_f32(self, pred, code, eb):
+        return self.L.ref_recover_f32(float(pred), int(code), float(eb))
+
+    def recover_f64(self, pred, code, eb):
+        return self.L.ref_recover_f64(float(pred), int(code), float(eb))
+
+    def interp_f32(self, kind, a, b, c, d):
+        return self.L.ref_interp_f32(kind, float(a), float(b), float(c), float(d))
+
+    def interp_f64(self, kind, a, b, c, d):
+        return self.L.ref_interp_f64(kind, float(a), float(b), float(c), float(d))
+
+
 def oracle():
     return Oracle()
 

@@ -72,3 +72,30 @@ def test_stream_parity(ctx, oracle, case, simd):
     assert np.array_equal(dec.reshape(shape).view(np.uint8), o_dec.reshape(shape).view(np.uint8))
     bound = eb if mode == 0 else eb * float(data.max() - data.min())
     assert np.abs(dec.reshape(shape).astype(np.float64) - data.astype(np.float64)).max() <= bound
+
+
+def _window_cases():
+    from test_oracle_traversal import WINDOW
+    return [(s, d, c) for s, d in WINDOW for c in (0, 1)]
+
+
+@pytest.mark.parametrize("shape,direction,cubic", _window_cases())
+@pytest.mark.parametrize("simd", [1, 0])
+def test_compare_class_window_fields(ctx, oracle, shape, direction, cubic, simd):
+    """Fields whose first level-1 targets all sit at error exactly float(eb): the vector compare class
+    accepts them and the scalar one escapes them, so any misplaced class changes the stream."""
+    from test_oracle_traversal import _window_field
+    data = _window_field(shape, direction)
+    kw = dict(cmpr_algo=2, eb_mode=0, abs_eb=1e-3, interp_algo=cubic, interp_direction=direction,
+              interp_anchor_stride=4)
+    c, oc = _cfg_pair(oracle, shape, **kw)
+    oracle.L.fo_set_simd(simd)
+    try:
+        o_stream, o_mut, o_pre = oracle.compress(data, oc, want_pre=True)
+    finally:
+        oracle.L.fo_set_simd(1)
+    g = data.copy()
+    g_stream, _, g_pre = ctx.compress_ex(g, c, simd=simd, want_prezstd=True)
+    assert g_pre == o_pre
+    assert bytes(g_stream) == o_stream
+    assert np.array_equal(g.view(np.uint32), o_mut.view(np.uint32))

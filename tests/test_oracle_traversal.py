@@ -1,0 +1,229 @@
+"""The oracle's decomposition against what the mathematics and the specification fix, where no build of
+the reference can run (its decomposition header needs the CMake-generated version.hpp):
+
+* code-stream order: a field of distinct values with a bound far below every prediction error makes
+  every point escape, so the unpredictable list spells out the order in which points were coded; it
+  must equal the anchors in row-major order followed by traversal_spec.order() (an independent
+  closed-form enumeration), which also proves each point is coded exactly once;
+* reconstruction: |decompressed - original| <= bound, decompressing the codes reproduces exactly
+  what compression wrote back, and exact predictors leave zero codes on polynomials they reproduce;
+* the per-point compare classes of the two reference builds.
+"""
+import ctypes as C
+import itertools
+
+import numpy as np
+import pytest
+
+import traversal_spec as spec
+from falcons_amd import synth
+from oracle_lib import Oracle
+
+
+@pytest.fixture(scope="module")
+def orc():
+    o = Oracle()
+    o.L.fo_set_simd(1)
+    return o
+
+
+SHAPES = [(1,), (2,), (3,), (17,), (64,), (100,), (1, 7), (5, 5), (9, 16), (17, 33), (64, 65), (2, 3, 4),
+          (1, 1, 5), (5, 6, 7), (8, 8, 8), (9, 10, 11), (16, 17, 18), (33, 20, 9), (4, 4, 33)]
+
+
+def _dirs(n):
+    # 3-D: 0 (z, y, x) and 5 (x, y, z), the two the reference's 3-D kernels accept
+    # (interpolation_1d_simd_3d_x asserts its axis is 2); 2-D: 0 (y, x) and 1 (x, y)
+    return [0] if n == 1 else ([0, 1] if n == 2 else [0, 5])
+
+
+CASES = [(s, cub, d, a) for s in SHAPES for cub in (0, 1) for d in _dirs(len(s)) for a in (0, 4, 8)]
+
+
+@pytest.mark.parametrize("shape,cubic,direction,anchor", CASES,
+                         ids=[f"{c[0]}-c{c[1]}-d{c[2]}-a{c[3]}" for c in CASES])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_code_order_matches_spec(orc, shape, cubic, direction, anchor, dtype):
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(n * 7 + cubic * 3 + direction + anchor)
+    # distinct values in random order; the fractional noise keeps every prediction inexact
+    data = (rng.permutation(n) + rng.uniform(0.2, 0.8, n)).astype(dtype).reshape(shape)
+    assert np.unique(data).size == n
+    eb = 1e-30
+    codes, unpred, mutated, a_used = orc.interp_compress(data, cubic, direction, anchor, 1.0, 1.0, eb)
+    want, a_spec = spec.order(shape, cubic, direction, anchor)
+    assert a_used == a_spec
+    anc = spec.anchors(shape, a_spec) if a_spec else []
+    assert len(codes) == n - len(anc)
+    assert np.all(codes == -32768)
+    flat = data.reshape(-1)
+    assert np.array_equal(unpred[: len(anc)], flat[anc])
+    pos = {float(v): i for i, v in enumerate(flat)}
+    got = [pos[float(v)] for v in unpred[len(anc):]]
+    assert got == want
+    assert sorted(got + anc) == list(range(n))
+    assert np.array_equal(mutated, data)  # escapes keep the original values
+
+
+RT = [((33, 40, 65), 1, 0, 32, np.float32, 1e-3), ((33, 40, 65), 0, 5, 32, np.float32, 1e-4),
+      ((20, 31, 18), 1, 5, 0, np.float64, 1e-3), ((300, 257), 1, 0, 128, np.float32, 1e-3),
+      ((300, 257), 0, 1, 0, np.float64, 1e-3), ((5000,), 1, 0, 4096, np.float32, 1e-3),
+      ((777,), 0, 0, 0, np.float64, 1e-4), ((64, 64, 64), 1, 0, 32, np.float32, 1e-2)]
+
+
+@pytest.mark.parametrize("shape,cubic,direction,anchor,dtype,eb", RT)
+@pytest.mark.parametrize("simd", [1, 0])
+def test_round_trip_within_bound(orc, shape, cubic, direction, anchor, dtype, eb, simd):
+    orc.L.fo_set_simd(simd)
+    try:
+        data = synth.sinusoids(shape, seed=5, dtype=dtype)
+        codes, unpred, mutated, a = orc.interp_compress(data, cubic, direction, anchor, 1.0, 1.0, eb)
+        dec = orc.interp_decompress(shape, dtype, codes, unpred, cubic, direction, a, 1.0, 1.0, eb)
+        # compression writes the reconstruction back, except for escapes (which keep the original)
+        # and the last pass of level 1 (SkipOverwrite): every point holds one or the other
+        assert np.all((mutated == dec) | (mutated == data))
+        assert (mutated != data).mean() > 0.2
+        assert np.abs(dec.astype(np.float64) - data.astype(np.float64)).max() <= eb
+        assert (codes != -32768).mean() > 0.99  # predicted, not escaped
+    finally:
+        orc.L.fo_set_simd(1)
+
+
+def test_level_bounds_alpha_beta(orc):
+    # with alpha >= 1 the bound of level l is eb / min(alpha^(l-1), beta): finer levels get tighter
+    data = synth.sinusoids((33, 33, 33), seed=3)
+    eb = 1e-2
+    codes, unpred, mutated, a = orc.interp_compress(data, 1, 0, 0, 2.0, 4.0, eb)
+    dec = orc.interp_decompress(data.shape, np.float32, codes, unpred, 1, 0, a, 2.0, 4.0, eb)
+    assert np.all((mutated == dec) | (mutated == data))
+    err = np.abs(dec.astype(np.float64) - data)
+    # stride-1 targets (odd x) use eb / min(2^0, 4) = eb; the stride-32/16 level uses eb / 4
+    assert err.max() <= eb
+    coarse = err[16::32, ::32, ::32]
+    assert coarse.max() <= eb / 4
+
+
+def test_compare_classes(orc):
+    """AVX2 lanes compare float(err) with float(eb); the scalar quantizer compares in double. Points
+    whose error is exactly float(eb) > eb are accepted by the first and escaped by the second.
+
+    Construction, in units u = 2^-33 (the float spacing on [2^-10, 2^-9)), eb = 1e-3, float(eb) =
+    8589935 u: ori = k u with k odd, pred = ori - float(eb). Then diff * (1 / 2eb) = 0.50000002, so
+    q = 1; the reconstruction pred + 2eb = (k + 8589934.18) u lies on [2^-9, 2^-8), where floats are
+    2u apart, and rounds to the even k + 8589935; its error is 8589935 u = float(eb)."""
+    eb = 1e-3
+    ebf = np.float32(eb)
+    u = 2.0 ** -33
+    assert float(ebf) == 8589935 * u and float(ebf) > eb
+    out, esc = C.c_float(0), C.c_int(0)
+    for k in range(8589937, 16777216, 2 * 4999):
+        ori, pred = np.float32(k * u), np.float32((k - 8589935) * u)
+        assert float(ori) == k * u and float(pred) == (k - 8589935) * u
+        qv = orc.L.fo_quantize_point_f32(ori, pred, eb, 1, C.byref(out), C.byref(esc))
+        assert qv == 1 and esc.value == 0 and out.value == (k + 8589935) * u
+        qs = orc.L.fo_quantize_point_f32(ori, pred, eb, 0, C.byref(out), C.byref(esc))
+        assert qs == -32768 and esc.value == 1 and out.value == float(ori)
+    # in double precision both classes compare in double
+    od = C.c_double(0)
+    for k in range(8589937, 8600000, 997):
+        a, b = k * u, (k - 8589935) * u
+        assert orc.L.fo_quantize_point_f64(a, b, eb, 1, C.byref(od), C.byref(esc)) == \
+            orc.L.fo_quantize_point_f64(a, b, eb, 0, C.byref(od), C.byref(esc))
+
+
+def test_plan_levels_match_spec(orc):
+    for shape in itertools.chain([(512, 512, 512), (33, 33, 33), (32, 32, 32), (5,), (4097,), (1, 1, 1)],
+                                 [(7, 300), (1024, 1024)]):
+        for anchor in (0, 4, 16, 32, 128, 4096):
+            a, lv = orc.plan(shape, anchor)  # init()'s level, before the anchor-grid decrement
+            want_lv, want_a = spec.levels_and_anchor(shape, anchor)
+            assert (a, lv - (1 if a else 0)) == (want_a, want_lv), (shape, anchor)
+    assert spec.levels_and_anchor((512, 512, 512), 32) == (5, 32)
+
+
+MODEL_SHAPES = [(1,), (2,), (5,), (40,), (77,), (3, 9), (9, 16), (17, 12), (26, 33), (2, 3, 4), (5, 6, 7),
+                (8, 8, 8), (9, 10, 11), (12, 7, 20), (17, 9, 6), (4, 33, 5)]
+MODEL_CASES = [(s, cub, d, a, dt) for s in MODEL_SHAPES for cub in (0, 1) for d in _dirs(len(s)) for a in (0, 4)
+               for dt in (np.float32, np.float64)]
+
+
+@pytest.mark.parametrize("shape,cubic,direction,anchor,dtype", MODEL_CASES,
+                         ids=[f"{c[0]}-c{c[1]}-d{c[2]}-a{c[3]}-{np.dtype(c[4]).name}" for c in MODEL_CASES])
+@pytest.mark.parametrize("simd", [0, 1])
+def test_predictors_match_spec_model(orc, shape, cubic, direction, anchor, dtype, simd):
+    """Each target's predictor, neighbours, compare class and write-back against the spec model."""
+    import spec_model
+    data = synth.sinusoids(shape, seed=len(shape) * 100 + cubic, dtype=dtype) * 0.01
+    # a bound that leaves ordinary codes, a few escapes, and points near the compare edge
+    eb = 2e-5 if dtype == np.float32 else 1e-6
+    orc.L.fo_set_simd(simd)
+    try:
+        codes, unpred, mutated, _ = orc.interp_compress(data, cubic, direction, anchor, 1.0, 1.0, eb)
+    finally:
+        orc.L.fo_set_simd(1)
+    m_codes, m_unpred, m_mut = spec_model.model_compress(orc, data, cubic, direction, anchor, eb, simd)
+    assert np.array_equal(codes, m_codes)
+    ui = np.uint32 if dtype == np.float32 else np.uint64
+    assert np.array_equal(unpred.view(ui), m_unpred.view(ui))
+    assert np.array_equal(mutated.reshape(-1).view(ui), m_mut.reshape(-1).view(ui))
+
+
+@pytest.mark.parametrize("alpha,beta", [(2.0, 4.0), (-1.0, 1.0), (0.5, 1.0), (1.5, 10.0)])
+def test_level_bounds_match_spec_model(orc, alpha, beta):
+    import spec_model
+    data = synth.sinusoids((13, 17, 19), seed=8) * 0.01
+    codes, unpred, mutated, _ = orc.interp_compress(data, 1, 0, 4, alpha, beta, 3e-5)
+    m_codes, m_unpred, m_mut = spec_model.model_compress(orc, data, 1, 0, 4, 3e-5, 1, alpha, beta)
+    assert np.array_equal(codes, m_codes)
+    assert np.array_equal(unpred.view(np.uint32), m_unpred.view(np.uint32))
+
+
+def _window_field(shape, direction, dtype=np.float32):
+    """Constant p, except the targets of the first pass of level 1 (stride 1), which hold p + float(eb)
+    for eb = 1e-3 as in test_compare_classes (p = 192 * 2^-33, so every predictor formula reproduces p
+    exactly and every such target has error exactly float(eb) at q = 1)."""
+    u = 2.0 ** -33
+    p, k = 192 * u, 192 + 8589935
+    d = np.full(shape, p, dtype=dtype)
+    N = len(shape)
+    sl = [slice(0, None, 2)] * N
+    ax = 0 if (N > 1 and direction == 0) else N - 1
+    sl[ax] = slice(1, None, 2)
+    d[tuple(sl)] = k * u
+    return d
+
+
+WINDOW = [((9,), 0), ((30,), 0), ((41,), 0), ((6, 30), 0), ((7, 41), 1), ((10, 26), 0), ((9, 9, 30), 0),
+          ((6, 9, 41), 0), ((8, 5, 26), 0), ((5, 7, 30), 5), ((9, 6, 41), 5)]
+
+
+@pytest.mark.parametrize("shape,direction", WINDOW)
+@pytest.mark.parametrize("cubic", [0, 1])
+@pytest.mark.parametrize("simd", [0, 1])
+def test_compare_class_placement(orc, shape, direction, cubic, simd):
+    """Every target of the first level-1 pass sits at error float(eb): the vector class accepts it
+    (code 1), the scalar class escapes it. Which points fall in which class must follow the spec."""
+    import spec_model
+    data = _window_field(shape, direction)
+    orc.L.fo_set_simd(simd)
+    try:
+        codes, unpred, mutated, _ = orc.interp_compress(data, cubic, direction, 4, 1.0, 1.0, 1e-3)
+    finally:
+        orc.L.fo_set_simd(1)
+    m_codes, m_unpred, _ = spec_model.model_compress(orc, data, cubic, direction, 4, 1e-3, simd)
+    assert np.array_equal(codes, m_codes)
+    assert np.array_equal(unpred.view(np.uint32), m_unpred.view(np.uint32))
+    # codes of the planted targets, located through the code order
+    order, a = spec.order(shape, cubic, direction, 4)
+    first = 0 if a else 1
+    planted = set(np.flatnonzero(data.reshape(-1) != data.reshape(-1)[0]).tolist())
+    pc = np.array([codes[first + i] for i, pos in enumerate(order) if pos in planted])
+    assert pc.size == len(planted)
+    # targets predicted from coarse points only sit exactly at error float(eb) (a linear row tail
+    # with an even grid count predicts from the same pass and does not)
+    if simd and shape == (9,):
+        assert np.all(pc == -32768)  # 4 targets per line: no vector lanes, all scalar class
+    elif simd:
+        assert (pc == 1).any()  # vector lanes accept them
+    else:
+        assert (pc == -32768).mean() > 0.6  # the scalar build escapes them
