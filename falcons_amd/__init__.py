@@ -92,12 +92,22 @@ _L.fz_synchronize.argtypes = [C.c_void_p]
 _L.fz_synchronize.restype = C.c_int
 _L.fz_zstd_version.argtypes = []
 _L.fz_zstd_version.restype = C.c_uint
+_L.fz_profile_enable.argtypes = [C.c_void_p, C.c_int]
+_L.fz_profile_enable.restype = C.c_int
+_L.fz_profile_reset.argtypes = [C.c_void_p]
+_L.fz_profile_reset.restype = C.c_int
+_L.fz_profile_count.argtypes = [C.c_void_p]
+_L.fz_profile_count.restype = C.c_int
+_L.fz_profile_entry.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_uint64),
+                                C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+_L.fz_profile_entry.restype = C.c_int
 
 EXPORTED_SYMBOLS = ("fz_ctx_create", "fz_ctx_destroy", "fz_last_error", "fz_opts_init", "fz_config_init",
                     "fz_config_size_est", "fz_config_save", "fz_read_config", "fz_compress_size_bound",
                     "fz_compress", "fz_decompress", "fz_prezstd_bound", "fz_compress_device",
                     "fz_decompress_device", "fz_device_alloc", "fz_device_free", "fz_memcpy", "fz_synchronize",
-                    "fz_zstd_version")
+                    "fz_zstd_version", "fz_profile_enable", "fz_profile_reset", "fz_profile_count",
+                    "fz_profile_entry")
 
 
 def zstd_version():
@@ -395,6 +405,25 @@ class Context:
 
     def prezstd_bound(self, config, dtype):
         return int(_L.fz_prezstd_bound(C.byref(config.conf), _DT[np.dtype(dtype)]))
+
+    def profile(self, on=True):
+        """Switch the per-launch HIP-event profiler on or off (see kernel_times)."""
+        rc = _L.fz_profile_enable(self.h, 1 if on else 0)
+        if rc:
+            _raise(rc, self._err())
+
+    def profile_reset(self):
+        _L.fz_profile_reset(self.h)
+
+    def kernel_times(self):
+        """{kernel: (launches, total ms, algorithmic bytes)} since the last profile_reset."""
+        out = {}
+        name = C.create_string_buffer(128)
+        ln, ms, by = C.c_uint64(), C.c_double(), C.c_uint64()
+        for i in range(_L.fz_profile_count(self.h)):
+            if _L.fz_profile_entry(self.h, i, name, 128, C.byref(ln), C.byref(ms), C.byref(by)) == 0:
+                out[name.value.decode()] = (ln.value, ms.value, by.value)
+        return out
 
     def synchronize(self):
         rc = _L.fz_synchronize(self.h)

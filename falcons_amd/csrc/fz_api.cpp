@@ -11,6 +11,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <iterator>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -21,6 +23,8 @@
 #include "fz_zstd.h"
 
 using namespace fz;
+
+thread_local LaunchProfiler* fz::g_prof = nullptr;
 
 namespace {
 
@@ -48,7 +52,27 @@ struct fz_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevArena arena;
     std::string err;
+    std::unique_ptr<LaunchProfiler> prof;  // per-launch timing, when switched on
 };
+
+namespace {
+// Makes the context's launch profiler (if on) the one this thread's launches report to for the
+// duration of one API call; on the way out it waits for the stream and folds the timings in.
+struct ProfActivate {
+    fz_ctx* ctx;
+    explicit ProfActivate(fz_ctx* c) : ctx(c) { g_prof = c->prof.get(); }
+    ~ProfActivate() {
+        if (g_prof) {
+            try {
+                FZ_HIP(hipStreamSynchronize(ctx->stream));
+                g_prof->collect();
+            } catch (...) {
+            }
+        }
+        g_prof = nullptr;
+    }
+};
+}  // namespace
 
 // ------------------------------------------------------------------ Config header
 extern "C" void fz_config_init(fz_config* c, int N, const uint64_t* dims) {
@@ -679,6 +703,9 @@ extern "C" const char* fz_last_error(const fz_ctx* c) { return c ? c->err.c_str(
     } catch (const std::length_error& e) {                                \
         set_err(ctx, e.what());                                           \
         return FZ_ERR_INVALID_ARGUMENT;                                   \
+    } catch (const std::invalid_argument& e) {                            \
+        set_err(ctx, e.what());                                           \
+        return FZ_ERR_INVALID_ARGUMENT;                                   \
     } catch (const std::exception& e) {                                   \
         set_err(ctx, e.what());                                           \
         return FZ_ERR_RUNTIME;                                            \
@@ -687,6 +714,7 @@ extern "C" const char* fz_last_error(const fz_ctx* c) { return c ? c->err.c_str(
 extern "C" int fz_compress(fz_ctx* ctx, const fz_config* conf, int dtype, void* data, int data_on_device, uint8_t* out,
                            size_t cap, size_t* out_len, fz_opts* opts) {
     FZ_GUARD({
+        ProfActivate prof_scope(ctx);
         const double t0 = now_ms();
         fz_opts local;
         fz_opts_init(&local);
@@ -796,6 +824,7 @@ extern "C" int fz_compress(fz_ctx* ctx, const fz_config* conf, int dtype, void* 
 extern "C" int fz_decompress(fz_ctx* ctx, const uint8_t* cmp, size_t len, int dtype, void* out, size_t out_cap_elems,
                              int out_on_device, fz_config* conf_out, fz_opts* opts) {
     FZ_GUARD({
+        ProfActivate prof_scope(ctx);
         const double t0 = now_ms();
         fz_opts local;
         fz_opts_init(&local);
@@ -835,6 +864,7 @@ extern "C" int fz_decompress(fz_ctx* ctx, const uint8_t* cmp, size_t len, int dt
 extern "C" int fz_compress_device(fz_ctx* ctx, fz_config* conf, int dtype, void* d_data, uint8_t* d_pre,
                                   size_t pre_cap, size_t* pre_len, fz_opts* opts) {
     FZ_GUARD({
+        ProfActivate prof_scope(ctx);
         const double t0 = now_ms();
         fz_opts local;
         fz_opts_init(&local);
@@ -861,6 +891,7 @@ extern "C" int fz_decompress_device(fz_ctx* ctx, const fz_config* conf, int dtyp
                                     const uint8_t* d_pre, size_t pre_len, void* d_out, size_t out_cap_elems,
                                     fz_opts* opts) {
     FZ_GUARD({
+        ProfActivate prof_scope(ctx);
         const double t0 = now_ms();
         fz_opts local;
         fz_opts_init(&local);
@@ -906,4 +937,36 @@ extern "C" int fz_synchronize(fz_ctx* ctx) {
         FZ_HIP(hipStreamSynchronize(ctx->stream));
         return FZ_OK;
     })
+}
+
+extern "C" int fz_profile_enable(fz_ctx* ctx, int on) {
+    FZ_GUARD({
+        if (on && !ctx->prof) ctx->prof.reset(new LaunchProfiler());
+        if (!on) ctx->prof.reset();
+        return FZ_OK;
+    })
+}
+
+extern "C" int fz_profile_reset(fz_ctx* ctx) {
+    FZ_GUARD({
+        if (ctx->prof) ctx->prof->reset();
+        return FZ_OK;
+    })
+}
+
+extern "C" int fz_profile_count(const fz_ctx* ctx) { return (ctx && ctx->prof) ? (int)ctx->prof->totals().size() : 0; }
+
+extern "C" int fz_profile_entry(const fz_ctx* ctx, int i, char* name, size_t name_cap, uint64_t* launches,
+                                double* total_ms, uint64_t* bytes) {
+    if (!ctx || !ctx->prof || i < 0 || i >= (int)ctx->prof->totals().size()) return FZ_ERR_INVALID_ARGUMENT;
+    auto it = ctx->prof->totals().begin();
+    std::advance(it, i);
+    if (name && name_cap) {
+        strncpy(name, it->first.c_str(), name_cap - 1);
+        name[name_cap - 1] = 0;
+    }
+    if (launches) *launches = it->second.launches;
+    if (total_ms) *total_ms = it->second.ms;
+    if (bytes) *bytes = it->second.bytes;
+    return FZ_OK;
 }

@@ -4,8 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #define FZ_HIP(call)                                                                               \
     do {                                                                                           \
@@ -51,4 +53,89 @@ struct PassDesc {
 
 enum Dtype { kF32 = 0, kF64 = 1 };
 
+// Per-launch timing with HIP events on the launching stream, switched on per context. Every launch
+// site names its kernel and the algorithmic bytes of that launch (the bytes the kernel must read
+// and write at the least); collect() turns the event pairs of finished launches into per-kernel
+// totals.
+class LaunchProfiler {
+   public:
+    struct Total {
+        uint64_t launches = 0;
+        double ms = 0;
+        uint64_t bytes = 0;
+    };
+    ~LaunchProfiler() {
+        for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+        for (const Rec& r : pending_) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
+    void begin(const char* name, uint64_t bytes, hipStream_t st) {
+        Rec r{name, bytes, take(), take()};
+        FZ_HIP(hipEventRecord(r.a, st));
+        pending_.push_back(r);
+    }
+    void end(hipStream_t st) { FZ_HIP(hipEventRecord(pending_.back().b, st)); }
+    // Call after the stream has finished the recorded work.
+    void collect() {
+        for (const Rec& r : pending_) {
+            float ms = 0;
+            FZ_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            Total& t = totals_[r.name];
+            t.launches++;
+            t.ms += ms;
+            t.bytes += r.bytes;
+            pool_.push_back(r.a);
+            pool_.push_back(r.b);
+        }
+        pending_.clear();
+    }
+    void reset() {
+        collect();
+        totals_.clear();
+    }
+    const std::map<std::string, Total>& totals() const { return totals_; }
+
+   private:
+    struct Rec {
+        const char* name;
+        uint64_t bytes;
+        hipEvent_t a, b;
+    };
+    hipEvent_t take() {
+        if (pool_.empty()) {
+            hipEvent_t e;
+            FZ_HIP(hipEventCreate(&e));
+            return e;
+        }
+        hipEvent_t e = pool_.back();
+        pool_.pop_back();
+        return e;
+    }
+    std::vector<hipEvent_t> pool_;
+    std::vector<Rec> pending_;
+    std::map<std::string, Total> totals_;
+};
+
+// The profiler of the call running on this thread, or null.
+extern thread_local LaunchProfiler* g_prof;
+
+struct ProfScope {
+    hipStream_t st;
+    ProfScope(const char* name, uint64_t bytes, hipStream_t s) : st(s) {
+        if (g_prof) g_prof->begin(name, bytes, st);
+    }
+    ~ProfScope() {
+        if (g_prof) g_prof->end(st);
+    }
+};
+
 }  // namespace fz
+
+// Launch with a name and algorithmic bytes for the launch profiler.
+#define FZ_LAUNCH(name, bytes, kernel, grid, block, shmem, stream, ...)        \
+    do {                                                                       \
+        fz::ProfScope ps_((name), (uint64_t)(bytes), (stream));                \
+        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);   \
+    } while (0)

@@ -473,26 +473,215 @@ __global__ void k_o1_tables(const uint32_t* cnt, uint16_t* nf, CTable* ct, uint8
     build_ctable(&ct[c], norm, kO1Sym - 1, kO1Log, spread_scratch + (size_t)c * (1u << kO1Log));
 }
 
-// One tANS chain (one chunk of a segment), encoded backwards with a per-symbol context switch
-// (RleFse:681-727). chains[] = {segment lo, chunk lo, chunk hi}.
-__global__ void k_o1_encode(const int16_t* bins, const uint64_t* chains, int nchains, const CTable* ct_all,
-                            uint8_t* bufs, uint64_t buf_stride, uint64_t* blen) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nchains) return;
-    const uint64_t seg = chains[4 * q + 3];
-    const CTable* ct = ct_all + seg * kO1Ctx;
-    const uint64_t lo = chains[4 * q + 1], hi = chains[4 * q + 2];
-    if (hi <= lo) { blen[q] = 0; return; }
-    BitW w{bufs + (uint64_t)q * buf_stride, 0, 0, 0};
-    uint64_t idx = hi - 1;
-    int prev = (idx > lo) ? bins[idx - 1] : 0;
-    uint32_t st = init_state2(&ct[o1_ctx(prev)], o1_sym(bins[idx]));
-    for (uint64_t i = hi - 1; i-- > lo;) {
-        const int pc = (i > lo) ? bins[i - 1] : 0;
-        fse_encode(w, st, ct[o1_ctx(pc)], o1_sym(bins[i]));
+// ---- order-1 tANS encoding split into pieces --------------------------------------------------
+// A chain is encoded backwards: symbol hi-1 seeds the state (FSE_initCState2), then one encoder
+// step per symbol i = hi-2 .. lo. The steps are cut into pieces of kO1Piece consecutive steps,
+// piece 0 holding the first steps (i = hi-2 downwards). A piece's output bits depend on the state
+// it starts from, which the previous piece produces. Encoder steps are many-to-one (the low bits
+// of the state leave through the bitstream), so runs from different start states merge within a
+// few dozen steps; the pieces therefore run in parallel from a guessed state (k_o1_spec), then
+// each re-runs from its predecessor's end state until it meets its own logged trajectory
+// (k_o1_fix, repeated only while some piece fails to meet it), and finally every piece emits its
+// bits at its offset in the chain (k_o1_emit). The result is the sequential encoder's bitstream.
+constexpr uint32_t kO1Piece = 1024;
+
+struct O1Chains {        // device view of the chains of one call
+    const uint64_t* c;   // 4 per chain: segment lo, chunk lo, chunk hi, segment
+    const uint64_t* pfirst;  // first piece of each chain (exclusive scan), nchains + 1 entries
+    int nchains;
+};
+
+__device__ inline int o1_piece_chain(const O1Chains& C, uint64_t pid) {
+    int lo = 0, hi = C.nchains;  // last chain with pfirst <= pid
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (C.pfirst[mid] <= pid) lo = mid; else hi = mid;
     }
-    bw_add(w, st, kO1Log);
-    blen[q] = bw_close(w);
+    return lo;
+}
+
+// Steps [s_hi down to s_lo] of piece p of a chain with symbols [lo, hi): s_hi = hi-2 - p*kO1Piece.
+__device__ inline void o1_piece_range(uint64_t lo, uint64_t hi, uint64_t p, uint64_t& s_hi, uint64_t& s_lo) {
+    s_hi = hi - 2 - p * kO1Piece;
+    s_lo = (s_hi + 1 > lo + kO1Piece) ? s_hi + 1 - kO1Piece : lo;
+}
+
+__device__ inline uint32_t o1_seed_state(const int16_t* bins, uint64_t lo, uint64_t idx, const CTable* ct) {
+    const int prev = (idx > lo) ? bins[idx - 1] : 0;
+    return init_state2(&ct[o1_ctx(prev)], o1_sym(bins[idx]));
+}
+
+__device__ inline uint32_t o1_step(uint32_t& state, const CTable& ct, unsigned sym) {
+    const uint32_t nbOut = (uint32_t)(((uint64_t)state + ct.deltaNbBits[sym]) >> 16);
+    state = ct.stateTable[(state >> nbOut) + ct.deltaFindState[sym]];
+    return nbOut;
+}
+
+__global__ void __launch_bounds__(256) k_o1_spec(const int16_t* bins, O1Chains C, uint64_t npieces,
+                                                 const CTable* ct_all, uint16_t* slog, uint32_t* end_state,
+                                                 uint64_t* nbits) {
+    const uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= npieces) return;
+    const int q = o1_piece_chain(C, pid);
+    const uint64_t lo = C.c[4 * q + 1], hi = C.c[4 * q + 2];
+    const CTable* ct = ct_all + C.c[4 * q + 3] * kO1Ctx;
+    const uint64_t p = pid - C.pfirst[q];
+    uint64_t s_hi, s_lo;
+    o1_piece_range(lo, hi, p, s_hi, s_lo);
+    // piece 0 starts from the true seed; the others from the seed their own first symbol would give
+    uint32_t st = o1_seed_state(bins, lo, p == 0 ? hi - 1 : s_hi, ct);
+    uint64_t bits = 0;
+    for (uint64_t i = s_hi + 1; i-- > s_lo;) {
+        slog[i] = (uint16_t)st;
+        const int pc = (i > lo) ? bins[i - 1] : 0;
+        bits += o1_step(st, ct[o1_ctx(pc)], o1_sym(bins[i]));
+    }
+    end_state[pid] = st;
+    nbits[pid] = bits;
+}
+
+// One fix-up round over the end states of the previous round (end_prev, spec_end in the first).
+// A piece whose predecessor ended differently in the previous round (every piece in the first
+// round) re-runs from that end state until its state equals its logged one; from there on its bits
+// and end state are those of its logged run. One that never meets its log ends in the state its
+// re-run reached. A piece whose end state differs from end_prev marks itself changed, and its
+// successor runs again in the next round; after a round with no change every piece started from
+// its predecessor's true end state. Piece 0 of a chain starts from the true seed and never changes.
+__global__ void __launch_bounds__(256) k_o1_fix(const int16_t* bins, O1Chains C, uint64_t npieces,
+                                                const CTable* ct_all, const uint16_t* slog, const uint32_t* spec_end,
+                                                const uint64_t* spec_bits, const uint32_t* end_prev,
+                                                uint32_t* end_next, uint64_t* nbits, const uint8_t* changed_prev,
+                                                uint8_t* changed_next, unsigned int* any_changed) {
+    const uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= npieces) return;
+    const int q = o1_piece_chain(C, pid);
+    const uint64_t p = pid - C.pfirst[q];
+    uint32_t end = end_prev[pid];
+    uint8_t ch = 0;
+    if (p > 0 && (changed_prev == nullptr || changed_prev[pid - 1])) {
+        const uint64_t lo = C.c[4 * q + 1], hi = C.c[4 * q + 2];
+        const CTable* ct = ct_all + C.c[4 * q + 3] * kO1Ctx;
+        uint64_t s_hi, s_lo;
+        o1_piece_range(lo, hi, p, s_hi, s_lo);
+        uint32_t st = end_prev[pid - 1];
+        int64_t delta = 0;
+        bool met = false;
+        for (uint64_t i = s_hi + 1; i-- > s_lo;) {
+            if (st == slog[i]) { met = true; break; }
+            const int pc = (i > lo) ? bins[i - 1] : 0;
+            const CTable& t = ct[o1_ctx(pc)];
+            const unsigned sym = o1_sym(bins[i]);
+            uint32_t ls = slog[i];
+            delta -= (int64_t)o1_step(ls, t, sym);
+            delta += (int64_t)o1_step(st, t, sym);
+        }
+        nbits[pid] = (uint64_t)((int64_t)spec_bits[pid] + delta);
+        const uint32_t e = met ? spec_end[pid] : st;
+        if (e != end) {
+            end = e;
+            ch = 1;
+            atomicAdd(any_changed, 1u);
+        }
+    }
+    end_next[pid] = end;
+    changed_next[pid] = ch;
+}
+
+// Chain-relative bit offset of every piece and the bit count of every chain's steps, from the
+// exclusive scan of the piece bit counts over all pieces.
+__global__ void __launch_bounds__(256) k_o1_offsets(O1Chains C, uint64_t npieces, const uint64_t* scan,
+                                                    const uint64_t* nbits, uint64_t* bitoff, uint64_t* chain_bits) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < npieces) {
+        const int q = o1_piece_chain(C, t);
+        bitoff[t] = scan[t] - scan[C.pfirst[q]];
+    } else if (t < npieces + (uint64_t)C.nchains) {
+        const uint64_t q = t - npieces;
+        const uint64_t a = C.pfirst[q], b = C.pfirst[q + 1];
+        chain_bits[q] = (b > a) ? scan[b - 1] + nbits[b - 1] - scan[a] : 0;
+    }
+}
+
+// Clears the 32-bit words that several pieces share (each piece's first word, and the words of the
+// chain's closing bits); every other word of a chain is written whole by exactly one piece.
+__global__ void __launch_bounds__(256) k_o1_clear_shared(O1Chains C, uint64_t npieces, const uint64_t* bitoff,
+                                                         const uint64_t* chain_bits, uint32_t* words,
+                                                         uint64_t stride_words) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < npieces) {
+        const int q = o1_piece_chain(C, t);
+        words[q * stride_words + (bitoff[t] >> 5)] = 0;
+    } else if (t < npieces + (uint64_t)C.nchains) {
+        const uint64_t q = t - npieces;
+        const uint64_t b = chain_bits[q];  // bits before the closing state
+        for (uint64_t w = b >> 5; w <= (b + kO1Log + 1) >> 5; w++) words[q * stride_words + w] = 0;
+    }
+}
+
+struct WordWriter {  // LSB-first bits into 32-bit words; shared (first and last) words are OR-ed in
+    uint32_t* w;
+    uint64_t idx;
+    uint64_t acc;
+    uint32_t n;
+    bool first;
+};
+__device__ inline void ww_put(WordWriter& W, uint32_t v, uint32_t nb) {
+    if (!nb) return;
+    W.acc |= (uint64_t)(v & ((1u << nb) - 1)) << W.n;
+    W.n += nb;
+    if (W.n >= 32) {
+        if (W.first) { atomicOr(&W.w[W.idx], (uint32_t)W.acc); W.first = false; }
+        else W.w[W.idx] = (uint32_t)W.acc;
+        W.idx++;
+        W.acc >>= 32;
+        W.n -= 32;
+    }
+}
+__device__ inline void ww_flush(WordWriter& W) {
+    if (W.n) atomicOr(&W.w[W.idx], (uint32_t)W.acc);
+}
+
+__global__ void __launch_bounds__(256) k_o1_emit(const int16_t* bins, O1Chains C, uint64_t npieces,
+                                                 const CTable* ct_all, const uint32_t* end_state,
+                                                 const uint64_t* bitoff, uint32_t* words, uint64_t stride_words) {
+    const uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= npieces) return;
+    const int q = o1_piece_chain(C, pid);
+    const uint64_t lo = C.c[4 * q + 1], hi = C.c[4 * q + 2];
+    const CTable* ct = ct_all + C.c[4 * q + 3] * kO1Ctx;
+    const uint64_t p = pid - C.pfirst[q];
+    uint64_t s_hi, s_lo;
+    o1_piece_range(lo, hi, p, s_hi, s_lo);
+    uint32_t st = p == 0 ? o1_seed_state(bins, lo, hi - 1, ct) : end_state[pid - 1];
+    const uint64_t b0 = bitoff[pid];
+    WordWriter W{words + q * stride_words, b0 >> 5, 0, (uint32_t)(b0 & 31), true};
+    for (uint64_t i = s_hi + 1; i-- > s_lo;) {
+        const int pc = (i > lo) ? bins[i - 1] : 0;
+        const CTable& t = ct[o1_ctx(pc)];
+        const unsigned sym = o1_sym(bins[i]);
+        const uint32_t nbOut = (uint32_t)(((uint64_t)st + t.deltaNbBits[sym]) >> 16);
+        ww_put(W, st, nbOut);
+        st = t.stateTable[(st >> nbOut) + t.deltaFindState[sym]];
+    }
+    ww_flush(W);
+}
+
+// Final state (kO1Log bits) and the end mark of every chain (BIT_closeCStream); byte length out.
+__global__ void k_o1_close(const int16_t* bins, O1Chains C, const CTable* ct_all, const uint32_t* end_state,
+                           const uint64_t* chain_bits, uint32_t* words, uint64_t stride_words, uint64_t* blen) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= C.nchains) return;
+    const uint64_t lo = C.c[4 * q + 1], hi = C.c[4 * q + 2];
+    if (hi <= lo) { blen[q] = 0; return; }
+    const CTable* ct = ct_all + C.c[4 * q + 3] * kO1Ctx;
+    const uint64_t np = C.pfirst[q + 1] - C.pfirst[q];
+    const uint32_t st = np ? end_state[C.pfirst[q + 1] - 1] : o1_seed_state(bins, lo, hi - 1, ct);
+    const uint64_t b0 = chain_bits[q];
+    WordWriter W{words + q * stride_words, b0 >> 5, 0, (uint32_t)(b0 & 31), true};
+    ww_put(W, st, kO1Log);
+    ww_put(W, 1, 1);
+    if (W.n) { atomicOr(&W.w[W.idx], (uint32_t)W.acc); }
+    blen[q] = (b0 + kO1Log + 1 + 7) >> 3;
 }
 
 // Side stream of escaped codes (RleFse:483-532), one workgroup per segment.
@@ -575,28 +764,156 @@ __global__ void k_o1_dtables(const uint8_t* nf, DEntry* dt, uint8_t* spread_scra
         *err = 1;
 }
 
-// chains[] = {stream offset, stream bytes, out lo, out hi, ovf offset, dtable set}
-__global__ void k_o1_decode(const uint8_t* in, const uint64_t* chains, int nchains, const DEntry* dt_all,
-                            const int16_t* ovf, int16_t* out) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nchains) return;
-    const uint64_t* c = chains + 6 * q;
-    const uint64_t lo = c[2], hi = c[3];
-    if (hi <= lo || c[1] == 0) return;
-    const DEntry* dt = dt_all + c[5] * kO1Ctx * (1u << kO1Log);
-    BitR r;
-    br_init(r, in + c[0], c[1]);
-    uint32_t st = br_read(r, kO1Log);
-    uint64_t ovi = c[4];
-    int prev = 0;
-    for (uint64_t i = lo; i < hi; i++) {
-        const DEntry e = dt[(size_t)o1_ctx(prev) * (1u << kO1Log) + st];
-        st = e.newState + br_read(r, e.nbBits);
-        const unsigned s = e.symbol;
-        const int code = (s == 255) ? ovf[ovi++] : (int)((s >> 1) ^ (~(s & 1) + 1));
-        out[i] = (int16_t)code;
-        prev = code;
+// ---- order-1 tANS decoding split by bit position ----------------------------------------------
+// A chain's bitstream is cut into subsequences of sub_bits bits. Subsequence r owns the symbols
+// whose decoding starts at a read position B in (b_{r+1}, b_r], b_r = pos0 - r * sub_bits; the
+// last one owns everything below b_{R-1}. A decoder started at a wrong state on a wrong boundary
+// falls into step with the true one after some hundreds of symbols (both land on the same read
+// position, state and context), so all subsequences first decode from a guess (round 1); each
+// round then restarts a subsequence from its left neighbour's exit (read position, state, context)
+// and from the escape index given by the escape counts of the subsequences before it, when either
+// changed (the escape index only matters to a subsequence that read escape signs). With no change
+// left every start is the true one; k_o1d_out then writes the codes at offsets given by the
+// symbol counts. The context after an escape depends on the sign of the escaped code, taken from
+// the already decoded side stream.
+struct O1DecChain {
+    uint64_t off, len;      // stream bytes
+    uint64_t lo, hi;        // output codes
+    uint64_t ovf_base, novf;
+    uint64_t dset;          // table set (segment)
+    int64_t pos0;           // read position after the initial state
+    uint32_t x0;            // initial state
+    uint32_t nsub;          // subsequences
+    uint64_t tfirst;        // first subsequence (global index)
+    int64_t sub_bits;
+};
+
+struct O1DecState {  // where a subsequence starts, or where it left off
+    int64_t B;
+    uint32_t X;
+    uint32_t ctx;
+};
+
+__device__ inline int o1d_chain_of(const O1DecChain* ch, int nch, uint64_t t) {
+    int lo = 0, hi = nch;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (ch[mid].tfirst <= t) lo = mid; else hi = mid;
     }
+    return lo;
+}
+
+// One decoder step; returns the code. ovf_lim guards the side-stream index.
+__device__ inline int o1d_step(const DEntry* dt, const uint8_t* buf, int64_t& B, uint32_t& X, uint32_t& ctx,
+                               uint64_t& e, const int16_t* ovf, uint64_t ovf_lim, bool& bad) {
+    const DEntry d = dt[(size_t)ctx * (1u << kO1Log) + (X & ((1u << kO1Log) - 1))];
+    BitR r{buf, B};
+    X = d.newState + br_read(r, d.nbBits);
+    B = r.pos;
+    int code;
+    if (d.symbol == 255) {
+        if (e < ovf_lim) code = ovf[e];
+        else { code = 32767; bad = true; }
+        e++;
+    } else {
+        const unsigned s = d.symbol;
+        code = (int)((s >> 1) ^ (~(s & 1) + 1));
+    }
+    ctx = (uint32_t)o1_ctx(code);
+    return code;
+}
+
+// One round over the subsequences flagged in run[] (all but the last of each chain in round 1).
+__global__ void __launch_bounds__(256) k_o1d_round(const uint8_t* in, const O1DecChain* ch, int nch, uint64_t nsub,
+                                                   const DEntry* dt_all, const int16_t* ovf, const uint8_t* run,
+                                                   const O1DecState* start, const uint64_t* e_start,
+                                                   O1DecState* exit_st, uint64_t* kesc, uint64_t* cnt, uint8_t* used) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsub || !run[t]) return;
+    const O1DecChain c = ch[o1d_chain_of(ch, nch, t)];
+    const uint64_t r = t - c.tfirst;
+    if (r + 1 >= c.nsub) return;  // the last subsequence has no successor to feed
+    const DEntry* dt = dt_all + c.dset * kO1Ctx * (1u << kO1Log);
+    const int64_t stop = c.pos0 - (int64_t)(r + 1) * c.sub_bits;
+    O1DecState s = start[t];
+    uint64_t e = e_start[t];
+    const uint64_t e0 = e, m = c.hi - c.lo;
+    uint64_t n = 0;
+    bool bad = false;
+    while (s.B > stop && n < m) {
+        o1d_step(dt, in + c.off, s.B, s.X, s.ctx, e, ovf + c.ovf_base, c.novf, bad);
+        n++;
+    }
+    exit_st[t] = s;
+    kesc[t] = e - e0;
+    cnt[t] = n;
+    used[t] = (e != e0) ? 1 : 0;
+}
+
+// Starts of round 1: the true start for subsequence 0, a guess elsewhere.
+__global__ void __launch_bounds__(256) k_o1d_init(const O1DecChain* ch, int nch, uint64_t nsub, O1DecState* start,
+                                                  uint64_t* e_start, uint8_t* run) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsub) return;
+    const O1DecChain c = ch[o1d_chain_of(ch, nch, t)];
+    const uint64_t r = t - c.tfirst;
+    O1DecState s;
+    if (r == 0) {
+        s = O1DecState{c.pos0, c.x0, 0};
+        e_start[t] = 0;
+    } else {
+        s = O1DecState{c.pos0 - (int64_t)r * c.sub_bits, (uint32_t)((r * 2654435761u) & ((1u << kO1Log) - 1)), 0};
+        e_start[t] = (uint64_t)((double)c.novf * (double)(r * c.sub_bits) / (double)(c.pos0 > 0 ? c.pos0 : 1));
+    }
+    start[t] = s;
+    run[t] = 1;
+}
+
+// Links the rounds: subsequence r > 0 restarts from the exit of r-1 with the escape index of the
+// escapes before it; it runs again if its start moved, or if its escape index moved and it read
+// escape signs.
+__global__ void __launch_bounds__(256) k_o1d_link(const O1DecChain* ch, int nch, uint64_t nsub,
+                                                  const O1DecState* exit_st, const uint64_t* kscan,
+                                                  const uint8_t* used, O1DecState* start, uint64_t* e_start,
+                                                  uint8_t* run, unsigned int* nrun) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsub) return;
+    const O1DecChain c = ch[o1d_chain_of(ch, nch, t)];
+    const uint64_t r = t - c.tfirst;
+    uint8_t go = 0;
+    if (r > 0) {
+        const O1DecState ns = exit_st[t - 1];
+        const uint64_t ne = kscan[t] - kscan[c.tfirst];
+        const O1DecState os = start[t];
+        const bool moved = ns.B != os.B || ns.X != os.X || ns.ctx != os.ctx;
+        if (moved || (ne != e_start[t] && used[t])) go = 1;
+        start[t] = ns;
+        e_start[t] = ne;
+        if (go && r + 1 < c.nsub) atomicAdd(nrun, 1u);
+    }
+    run[t] = go;
+}
+
+// Final pass: every subsequence decodes from its settled start into its place in the code array.
+__global__ void __launch_bounds__(256) k_o1d_out(const uint8_t* in, const O1DecChain* ch, int nch, uint64_t nsub,
+                                                 const DEntry* dt_all, const int16_t* ovf, const O1DecState* start,
+                                                 const uint64_t* e_start, const uint64_t* cscan, int16_t* out,
+                                                 int* err) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsub) return;
+    const O1DecChain c = ch[o1d_chain_of(ch, nch, t)];
+    const uint64_t r = t - c.tfirst;
+    const DEntry* dt = dt_all + c.dset * kO1Ctx * (1u << kO1Log);
+    const bool last = r + 1 >= c.nsub;
+    const int64_t stop = last ? INT64_MIN : c.pos0 - (int64_t)(r + 1) * c.sub_bits;
+    O1DecState s = start[t];
+    uint64_t e = e_start[t];
+    uint64_t i = c.lo + (cscan[t] - cscan[c.tfirst]);
+    bool bad = false;
+    while (i < c.hi && s.B > stop) {
+        out[i++] = (int16_t)o1d_step(dt, in + c.off, s.B, s.X, s.ctx, e, ovf + c.ovf_base, c.novf, bad);
+    }
+    if (bad || (last && (i != c.hi || e != c.novf)) || (!last && s.B > stop)) *err = 1;
 }
 
 __global__ void k_ovf_dec(const uint8_t* vcl, const uint8_t* vman, uint64_t n, int16_t* ovf) {
@@ -673,6 +990,7 @@ size_t entropy_encode_workspace(uint64_t n) {
     b += (n * 3 + 4096) * 2;             // mantissas
     b += 4 * blocks * (kFblk + 1024);    // block tmp + slots (two arrays)
     b += n * 2 * 2 + (1 << 20);          // o1 chain buffers (2 bytes per symbol + slack)
+    b += n * 2 + (n / kO1Piece + 1024) * 64;  // o1 piece state log and per-piece arrays
     b += 16 * kO1Ctx * sizeof(CTable) + 16 * kO1Ctx * (1u << kO1Log);
     b += 3 * (n * 2 + 4 * (kFblk + 1024)) + (1 << 20);  // ovf
     b += 64u << 20;                      // cub scratch
@@ -705,7 +1023,7 @@ static uint64_t fse_enc_blocks(const uint8_t* src, uint64_t n, uint8_t* out, uin
     uint8_t* slots = ws.take<uint8_t>(nb * (kFblk + 1024));
     uint64_t* sizes = ws.take<uint64_t>(nb);
     uint64_t* offs = ws.take<uint64_t>(nb);
-    hipLaunchKernelGGL(k_fse_blocks, dim3((unsigned)nb), dim3(256), 0, st, src, n, tmp, slots, sizes);
+    FZ_LAUNCH("fse_blocks", n * 2, k_fse_blocks, dim3((unsigned)nb), dim3(256), 0, st, src, n, tmp, slots, sizes);
     FZ_HIP(hipGetLastError());
     std::vector<uint64_t> hs(nb), ho(nb);
     d2h(hs.data(), sizes, nb, st);
@@ -713,7 +1031,7 @@ static uint64_t fse_enc_blocks(const uint8_t* src, uint64_t n, uint8_t* out, uin
     for (uint64_t b = 0; b < nb; b++) { ho[b] = acc; acc += hs[b]; }
     need_room(acc, cap);
     h2d(offs, ho.data(), nb, st);
-    hipLaunchKernelGGL(k_pack_slots, dim3(64, (unsigned)nb), dim3(256), 0, st, slots, (uint64_t)(kFblk + 1024), sizes,
+    FZ_LAUNCH("fse_pack", 0, k_pack_slots, dim3(64, (unsigned)nb), dim3(256), 0, st, slots, (uint64_t)(kFblk + 1024), sizes,
                        offs, out);
     FZ_HIP(hipGetLastError());
     return acc;
@@ -743,7 +1061,7 @@ static size_t encode_method0(const int16_t* bins, uint64_t n, uint8_t* out, uint
     uint64_t* vbits = ws.take<uint64_t>(nv + 1);
     uint64_t* roff = ws.take<uint64_t>(nr + 1);
     uint64_t* voff = ws.take<uint64_t>(nv + 1);
-    hipLaunchKernelGGL(k_runs_values, dim3(nblk(std::max(nr, nv))), dim3(256), 0, st, bins, n, pos, nv, nr, rcl, vcl,
+    FZ_LAUNCH("rle_runs_values", n * 2, k_runs_values, dim3(nblk(std::max(nr, nv))), dim3(256), 0, st, bins, n, pos, nv, nr, rcl, vcl,
                        rbits, vbits);
     FZ_HIP(hipGetLastError());
     exclusive_scan_u64(rbits, roff, nr, ws, st);
@@ -757,7 +1075,7 @@ static size_t encode_method0(const int16_t* bins, uint64_t n, uint8_t* out, uint
     uint32_t* vman = ws.take<uint32_t>(vbp / 4 + 2);
     FZ_HIP(hipMemsetAsync(rman, 0, (rbp / 4 + 2) * 4, st));
     FZ_HIP(hipMemsetAsync(vman, 0, (vbp / 4 + 2) * 4, st));
-    hipLaunchKernelGGL(k_scatter_mantissas, dim3(nblk(std::max(nr, nv))), dim3(256), 0, st, bins, n, pos, nv, nr, roff,
+    FZ_LAUNCH("rle_mantissas", 0, k_scatter_mantissas, dim3(nblk(std::max(nr, nv))), dim3(256), 0, st, bins, n, pos, nv, nr, roff,
                        voff, rcl, vcl, rman, vman);
     FZ_HIP(hipGetLastError());
     // class streams go after the 48-byte head
@@ -796,7 +1114,7 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
     std::vector<uint64_t> hpos(nesc);
     if (nesc) d2h(hpos.data(), epos, nesc, st);
     int16_t* ovf = ws.take<int16_t>(nesc + 1);
-    if (nesc) hipLaunchKernelGGL(k_gather_i16, dim3(nblk(nesc)), dim3(256), 0, st, bins, epos, nesc, ovf);
+    if (nesc) FZ_LAUNCH("o1_gather_escapes", nesc * 10, k_gather_i16, dim3(nblk(nesc)), dim3(256), 0, st, bins, epos, nesc, ovf);
     // histograms and tables per segment
     uint32_t* cnt = ws.take<uint32_t>(S * kO1Ctx * kO1Sym);
     uint16_t* nf = ws.take<uint16_t>(S * kO1Ctx * kO1Sym);
@@ -814,16 +1132,16 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
         const int16_t* sb = bins + lo;
         uint32_t* c = cnt + g * kO1Ctx * kO1Sym;
         if (sn < (1u << 20)) {
-            hipLaunchKernelGGL(k_o1_hist_exact, dim3(std::min<uint64_t>(1024, nblk(sn))), dim3(256), 0, st, sb, sn, cs, c);
+            FZ_LAUNCH("o1_hist", sn * 2, k_o1_hist_exact, dim3(std::min<uint64_t>(1024, nblk(sn))), dim3(256), 0, st, sb, sn, cs, c);
         } else {
             uint64_t step = sn / 2048;
             if (step < 512) step = 512;
             const uint64_t nw = (sn + step - 1) / step;
-            hipLaunchKernelGGL(k_o1_hist_sample, dim3((unsigned)nw), dim3(256), 0, st, sb, sn, step, (uint64_t)512, c);
-            hipLaunchKernelGGL(k_add_one, dim3(nblk(kO1Ctx * kO1Sym)), dim3(256), 0, st, c, kO1Ctx * kO1Sym);
+            FZ_LAUNCH("o1_hist", nw * 1024, k_o1_hist_sample, dim3((unsigned)nw), dim3(256), 0, st, sb, sn, step, (uint64_t)512, c);
+            FZ_LAUNCH("o1_hist", 0, k_add_one, dim3(nblk(kO1Ctx * kO1Sym)), dim3(256), 0, st, c, kO1Ctx * kO1Sym);
         }
         FZ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_o1_tables, dim3(kO1Ctx), dim3(64), 0, st, c, nf + g * kO1Ctx * kO1Sym,
+        FZ_LAUNCH("o1_tables", 0, k_o1_tables, dim3(kO1Ctx), dim3(64), 0, st, c, nf + g * kO1Ctx * kO1Sym,
                            ct + g * kO1Ctx, spread + g * kO1Ctx * (1u << kO1Log));
         FZ_HIP(hipGetLastError());
         ovf_off[g] = e;
@@ -841,10 +1159,81 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
     h2d(dchains, chains.data(), chains.size(), st);
     uint64_t maxcs = 0;
     for (uint64_t g = 0; g < S; g++) maxcs = std::max(maxcs, (seg_n[g] + kO1K - 1) / kO1K);
-    const uint64_t bstride = maxcs * 2 + 4096;
-    uint8_t* bufs = ws.take<uint8_t>(nch * bstride);
+    // chain staging: 13 bits per symbol at most, plus the closing state
+    const uint64_t stride_words = (maxcs * 2 + 4096 + 3) / 4;
+    const uint64_t bstride = stride_words * 4;
+    uint32_t* words = ws.take<uint32_t>(nch * stride_words);
+    uint8_t* bufs = reinterpret_cast<uint8_t*>(words);
     uint64_t* blen = ws.take<uint64_t>(nch);
-    hipLaunchKernelGGL(k_o1_encode, dim3(nblk(nch, 64)), dim3(64), 0, st, bins, dchains, nch, ct, bufs, bstride, blen);
+    // pieces of every chain (steps i = hi-2 .. lo)
+    std::vector<uint64_t> pfirst(nch + 1, 0);
+    for (int q = 0; q < nch; q++) {
+        const uint64_t clo = chains[4 * q + 1], chi = chains[4 * q + 2];
+        const uint64_t steps = (chi > clo + 1) ? chi - clo - 1 : 0;
+        pfirst[q + 1] = pfirst[q] + (steps + kO1Piece - 1) / kO1Piece;
+    }
+    const uint64_t np = pfirst[nch];
+    uint64_t* dpfirst = ws.take<uint64_t>(nch + 1);
+    h2d(dpfirst, pfirst.data(), nch + 1, st);
+    const O1Chains CH{dchains, dpfirst, nch};
+    uint16_t* slog = ws.take<uint16_t>(n + 16);
+    uint32_t* spec_end = ws.take<uint32_t>(np + 1);
+    uint32_t* end_a = ws.take<uint32_t>(np + 1);
+    uint32_t* end_b = ws.take<uint32_t>(np + 1);
+    uint64_t* spec_bits = ws.take<uint64_t>(np + 1);
+    uint64_t* nbits = ws.take<uint64_t>(np + 1);
+    uint8_t* chg_a = ws.take<uint8_t>(np + 1);
+    uint8_t* chg_b = ws.take<uint8_t>(np + 1);
+    unsigned int* any = ws.take<unsigned int>(1);
+    uint64_t* scan = ws.take<uint64_t>(np + 1);
+    uint64_t* bitoff = ws.take<uint64_t>(np + 1);
+    uint64_t* chain_bits = ws.take<uint64_t>(nch);
+    if (np) {
+        FZ_LAUNCH("o1_encode_spec", n * 4, k_o1_spec, dim3(nblk(np)), dim3(256), 0, st, bins, CH, np, ct, slog, spec_end, spec_bits);
+        FZ_HIP(hipGetLastError());
+        FZ_HIP(hipMemcpyAsync(nbits, spec_bits, np * 8, hipMemcpyDeviceToDevice, st));
+        // fix-up rounds; each round finalizes at least the next piece of every chain, so the loop
+        // ends within max-pieces-per-chain rounds
+        const uint32_t* end_prev = spec_end;
+        uint32_t* end_next = end_a;
+        const uint8_t* chg_prev = nullptr;
+        uint8_t* chg_next = chg_a;
+        uint64_t max_rounds = 1;
+        for (int q = 0; q < nch; q++) max_rounds = std::max<uint64_t>(max_rounds, pfirst[q + 1] - pfirst[q] + 1);
+        for (uint64_t round = 0;; round++) {
+            if (round > max_rounds) throw std::runtime_error("tANS piece fix-up did not settle");
+            FZ_HIP(hipMemsetAsync(any, 0, sizeof(unsigned int), st));
+            FZ_LAUNCH("o1_encode_fix", 0, k_o1_fix, dim3(nblk(np)), dim3(256), 0, st, bins, CH, np, ct, slog, spec_end,
+                               spec_bits, end_prev, end_next, nbits, chg_prev, chg_next, any);
+            FZ_HIP(hipGetLastError());
+            unsigned int h_any = 0;
+            d2h(&h_any, any, 1, st);
+            end_prev = end_next;
+            end_next = (end_next == end_a) ? end_b : end_a;
+            chg_prev = chg_next;
+            chg_next = (chg_next == chg_a) ? chg_b : chg_a;
+            if (!h_any) break;
+        }
+        exclusive_scan_u64(nbits, scan, np, ws, st);
+        FZ_LAUNCH("o1_encode_offsets", 0, k_o1_offsets, dim3(nblk(np + nch)), dim3(256), 0, st, CH, np, scan, nbits, bitoff,
+                           chain_bits);
+        FZ_HIP(hipGetLastError());
+        FZ_LAUNCH("o1_encode_clear", 0, k_o1_clear_shared, dim3(nblk(np + nch)), dim3(256), 0, st, CH, np, bitoff, chain_bits,
+                           words, stride_words);
+        FZ_HIP(hipGetLastError());
+        FZ_LAUNCH("o1_encode_emit", n * 2, k_o1_emit, dim3(nblk(np)), dim3(256), 0, st, bins, CH, np, ct, end_prev, bitoff, words,
+                           stride_words);
+        FZ_HIP(hipGetLastError());
+        FZ_LAUNCH("o1_encode_close", 0, k_o1_close, dim3(nblk(nch, 64)), dim3(64), 0, st, bins, CH, ct, end_prev, chain_bits,
+                           words, stride_words, blen);
+    } else {
+        FZ_HIP(hipMemsetAsync(chain_bits, 0, nch * 8, st));
+        FZ_LAUNCH("o1_encode_clear", 0, k_o1_clear_shared, dim3(nblk(nch)), dim3(256), 0, st, CH, np, bitoff, chain_bits, words,
+                           stride_words);
+        FZ_HIP(hipGetLastError());
+        FZ_LAUNCH("o1_encode_close", 0, k_o1_close, dim3(nblk(nch, 64)), dim3(64), 0, st, bins, CH, ct, spec_end, chain_bits,
+                           words, stride_words, blen);
+    }
     FZ_HIP(hipGetLastError());
     // escape side streams
     uint64_t maxo = 0;
@@ -857,7 +1246,7 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
     uint64_t* olen = ws.take<uint64_t>(S);
     uint64_t* dovf_off = ws.take<uint64_t>(S + 1);
     h2d(dovf_off, ovf_off.data(), S + 1, st);
-    hipLaunchKernelGGL(k_o1_ovf, dim3((unsigned)S), dim3(256), 0, st, ovf, dovf_off, ovcl, ovman, otmp, oout, ostride, olen);
+    FZ_LAUNCH("o1_escape_stream", 0, k_o1_ovf, dim3((unsigned)S), dim3(256), 0, st, ovf, dovf_off, ovcl, ovman, otmp, oout, ostride, olen);
     FZ_HIP(hipGetLastError());
     std::vector<uint64_t> hblen(nch), holen(S);
     d2h(hblen.data(), blen, nch, st);
@@ -908,7 +1297,7 @@ size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out
     uint64_t step = (n > NW * WL) ? (n - WL) / NW : WL;
     if (!step) step = 1;
     PreCounts* pc = ws.take<PreCounts>(1);
-    hipLaunchKernelGGL(k_preprocess, dim3(1), dim3(256), 0, st, codes, n, NW, WL, step, pc);
+    FZ_LAUNCH("preprocess", 0, k_preprocess, dim3(1), dim3(256), 0, st, codes, n, NW, WL, step, pc);
     FZ_HIP(hipGetLastError());
     PreCounts h{};
     d2h(&h, pc, 1, st);
@@ -924,7 +1313,7 @@ size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out
     const int16_t* bins = codes;
     if (use_delta && n > 0) {
         int16_t* d = ws.take<int16_t>(n);
-        hipLaunchKernelGGL(k_delta, dim3(nblk(n)), dim3(256), 0, st, codes, d, n);
+        FZ_LAUNCH("delta", n * 4, k_delta, dim3(nblk(n)), dim3(256), 0, st, codes, d, n);
         FZ_HIP(hipGetLastError());
         bins = d;
     }
@@ -968,20 +1357,97 @@ static void fse_dec_blocks(const uint8_t* hsrc, const uint8_t* dsrc, uint64_t n,
     if (n == 0) return;
     uint64_t* doffs = ws.take<uint64_t>(offs.size());
     h2d(doffs, offs.data(), offs.size(), st);
-    hipLaunchKernelGGL(k_fse_dec_blocks, dim3((unsigned)offs.size()), dim3(256), 0, st, dsrc, doffs, n, out, err);
+    FZ_LAUNCH("fse_dec_blocks", n, k_fse_dec_blocks, dim3((unsigned)offs.size()), dim3(256), 0, st, dsrc, doffs, n, out, err);
+    FZ_HIP(hipGetLastError());
+}
+
+// Order-1 tANS chains (7 values each: stream offset, bytes, out lo, out hi, escape offset, table
+// set, escape count) decoded in parallel subsequences; see k_o1d_round.
+static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const std::vector<uint64_t>& chains,
+                               const DEntry* dt, const int16_t* ovf, int16_t* out, int* err, DevArena& ws,
+                               hipStream_t st) {
+    const int nch = (int)(chains.size() / 7);
+    std::vector<O1DecChain> hc;
+    uint64_t nsub = 0, max_sub = 1, stream_bytes = 0, ncodes = 0;
+    for (int q = 0; q < nch; q++) {
+        const uint64_t* c = &chains[7 * q];
+        const uint64_t m = c[3] - c[2];
+        if (m == 0) continue;
+        // initial state: the tableLog bits under the end mark (FSE_initDState)
+        const uint8_t* b = host + c[0];
+        const int64_t top = (int64_t)(c[1] - 1) * 8 + (int64_t)highbit32(b[c[1] - 1]);
+        uint32_t x0 = 0;
+        for (int j = 1; j <= (int)kO1Log; j++) {
+            const int64_t bit = top - j;
+            x0 = (x0 << 1) | (bit >= 0 ? ((b[bit >> 3] >> (bit & 7)) & 1u) : 0u);
+        }
+        const int64_t pos0 = top - (int64_t)kO1Log;
+        // about 4096 symbols per subsequence, enough for a guessed start to fall into step
+        const double bits_per_sym = std::max(1.0, (double)std::max<int64_t>(pos0, 1) / (double)m);
+        const int64_t sub = std::max<int64_t>(8192, (int64_t)(bits_per_sym * 4096));
+        const uint64_t ns = std::max<int64_t>(1, (pos0 + sub - 1) / sub);
+        hc.push_back(O1DecChain{c[0], c[1], c[2], c[3], c[4], c[6], c[5], pos0, x0, (uint32_t)ns, nsub, sub});
+        stream_bytes += c[1];
+        ncodes += m;
+        nsub += ns;
+        max_sub = std::max(max_sub, ns);
+    }
+    if (hc.empty()) return;
+    const int nc = (int)hc.size();
+    O1DecChain* dch = ws.take<O1DecChain>(nc);
+    O1DecState* start = ws.take<O1DecState>(nsub);
+    O1DecState* exit_st = ws.take<O1DecState>(nsub);
+    uint64_t* e_start = ws.take<uint64_t>(nsub);
+    uint64_t* kesc = ws.take<uint64_t>(nsub);
+    uint64_t* cnt = ws.take<uint64_t>(nsub);
+    uint64_t* kscan = ws.take<uint64_t>(nsub);
+    uint64_t* cscan = ws.take<uint64_t>(nsub);
+    uint8_t* used = ws.take<uint8_t>(nsub);
+    uint8_t* run = ws.take<uint8_t>(nsub);
+    unsigned int* nrun = ws.take<unsigned int>(1);
+    FZ_HIP(hipMemsetAsync(kesc, 0, nsub * 8, st));
+    FZ_HIP(hipMemsetAsync(cnt, 0, nsub * 8, st));
+    FZ_HIP(hipMemsetAsync(used, 0, nsub, st));
+    h2d(dch, hc.data(), hc.size(), st);
+    FZ_LAUNCH("o1_decode_init", 0, k_o1d_init, dim3(nblk(nsub)), dim3(256), 0, st, dch, nc, nsub, start, e_start, run);
+    FZ_HIP(hipGetLastError());
+    // every round settles at least the next subsequence of every chain
+    for (uint64_t round = 0;; round++) {
+        if (round > max_sub + 1) throw std::runtime_error("tANS subsequence decoding did not settle");
+        FZ_LAUNCH("o1_decode_round", 0, k_o1d_round, dim3(nblk(nsub)), dim3(256), 0, st, dev, dch, nc, nsub, dt, ovf, run, start,
+                           e_start, exit_st, kesc, cnt, used);
+        FZ_HIP(hipGetLastError());
+        exclusive_scan_u64(kesc, kscan, nsub, ws, st);
+        FZ_HIP(hipMemsetAsync(nrun, 0, sizeof(unsigned int), st));
+        FZ_LAUNCH("o1_decode_link", 0, k_o1d_link, dim3(nblk(nsub)), dim3(256), 0, st, dch, nc, nsub, exit_st, kscan, used, start,
+                           e_start, run, nrun);
+        FZ_HIP(hipGetLastError());
+        unsigned int h = 0;
+        d2h(&h, nrun, 1, st);
+        if (!h) break;
+    }
+    exclusive_scan_u64(cnt, cscan, nsub, ws, st);
+    FZ_LAUNCH("o1_decode_out", stream_bytes + ncodes * 2, k_o1d_out, dim3(nblk(nsub)), dim3(256), 0, st, dev, dch, nc, nsub, dt, ovf, start, e_start,
+                       cscan, out, err);
     FZ_HIP(hipGetLastError());
 }
 
 void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
                     hipStream_t st) {
-    (void)len;
+    if (len < 5) throw std::invalid_argument("Truncated entropy stream");
+    const uint8_t* const hend = host + len;
+    auto need_in = [&](const uint8_t* at, uint64_t k) {
+        if (at > hend || (uint64_t)(hend - at) < k) throw std::invalid_argument("Truncated entropy stream");
+    };
     const int method = host[0] & 0x7F, use_delta = (host[0] & 0x80) != 0;
     int* err = ws.take<int>(1);
     FZ_HIP(hipMemsetAsync(err, 0, 4, st));
     const uint8_t* hp = host + 5;
     int16_t* out = use_delta ? ws.take<int16_t>(n + 1) : codes;
     if (method == 1) {
+        need_in(hp, 8);
         uint64_t S = get64(hp);
+        if (S > 16) throw std::invalid_argument("corrupt o1 tANS stream");
         if (!S) S = 1;
         const uint64_t seg = (n + S - 1) / S;
         std::vector<uint64_t> chains;
@@ -991,33 +1457,46 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
         std::vector<OvfPiece> pieces(S);
         for (uint64_t g = 0; g < S; g++) {
             const uint64_t lo = g * seg, hi = std::min(lo + seg, n), sn = hi - lo;
-            (void)get64(hp);
+            need_in(hp, 16);
+            const uint64_t seg_n = get64(hp);
             const int K = (int)get64(hp);
-            if (K < 1 || K > 16) throw std::runtime_error("corrupt o1 tANS stream");
+            if (K < 1 || K > 16 || seg_n != sn) throw std::invalid_argument("corrupt o1 tANS stream");
             uint64_t blen[16], nvk[16];
+            need_in(hp, 16 * (uint64_t)K);
             for (int k = 0; k < K; k++) blen[k] = get64(hp);
             for (int k = 0; k < K; k++) nvk[k] = get64(hp);
             nf_off[g] = (uint64_t)(hp - host);
+            need_in(hp, kO1Ctx * kO1Sym * 2);
             hp += kO1Ctx * kO1Sym * 2;
             const uint64_t cs = sn ? (sn + K - 1) / K : 0;
             uint64_t ovfo = 0;
             for (int k = 0; k < K; k++) {
                 const uint64_t clo = lo + k * cs, chi = std::min(clo + cs, hi);
-                chains.insert(chains.end(), {(uint64_t)(hp - host), blen[k], clo, std::max(clo, chi), total_ovf + ovfo, g});
+                need_in(hp, blen[k]);
+                if (chi > clo && (blen[k] == 0 || hp[blen[k] - 1] == 0)) throw std::invalid_argument("corrupt o1 tANS chain");
+                if (nvk[k] > chi - std::min(chi, clo)) throw std::invalid_argument("corrupt o1 tANS escape count");
+                chains.insert(chains.end(),
+                              {(uint64_t)(hp - host), blen[k], clo, std::max(clo, chi), total_ovf + ovfo, g, nvk[k]});
                 hp += blen[k];
                 ovfo += nvk[k];
             }
             OvfPiece pc{ovfo, 0, 0, 0, 0};
+            need_in(hp, 1);
             pc.mode = *hp++;
             if (ovfo) {
                 if (pc.mode == 0) {
                     pc.nraw_off = (uint64_t)(hp - host);
+                    need_in(hp, ovfo * 2);
                     hp += ovfo * 2;
                 } else {
+                    need_in(hp, 16);
                     const uint64_t svcl = get64(hp), vbp = get64(hp);
                     pc.vcl_off = (uint64_t)(hp - host);
+                    need_in(hp, svcl);
                     hp += svcl;
                     pc.vman_off = (uint64_t)(hp - host);
+                    need_in(hp, vbp);
+                    if (vbp * 8 < ovfo) throw std::invalid_argument("corrupt o1 tANS escape stream");
                     hp += vbp;
                 }
             }
@@ -1034,21 +1513,22 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
             } else {
                 uint8_t* vcl = ws.take<uint8_t>(pc.n);
                 fse_dec_blocks(host + pc.vcl_off, dev + pc.vcl_off, pc.n, vcl, err, ws, st, nullptr);
-                hipLaunchKernelGGL(k_ovf_dec, dim3(1), dim3(1), 0, st, vcl, dev + pc.vman_off, pc.n, ovf + ovf_base[g]);
+                FZ_LAUNCH("o1_escape_decode", 0, k_ovf_dec, dim3(1), dim3(1), 0, st, vcl, dev + pc.vman_off, pc.n, ovf + ovf_base[g]);
                 FZ_HIP(hipGetLastError());
             }
         }
         DEntry* dt = ws.take<DEntry>(S * kO1Ctx * (1u << kO1Log));
         uint8_t* spread = ws.take<uint8_t>(S * kO1Ctx * (1u << kO1Log));
         for (uint64_t g = 0; g < S; g++) {
-            hipLaunchKernelGGL(k_o1_dtables, dim3(kO1Ctx), dim3(64), 0, st, dev + nf_off[g],
+            FZ_LAUNCH("o1_dtables", 0, k_o1_dtables, dim3(kO1Ctx), dim3(64), 0, st, dev + nf_off[g],
                                dt + g * kO1Ctx * (1u << kO1Log), spread + g * kO1Ctx * (1u << kO1Log), err);
         }
-        const int nch = (int)(chains.size() / 6);
-        uint64_t* dch = ws.take<uint64_t>(chains.size());
-        h2d(dch, chains.data(), chains.size(), st);
-        hipLaunchKernelGGL(k_o1_decode, dim3(nblk(nch, 64)), dim3(64), 0, st, dev, dch, nch, dt, ovf, out);
-        FZ_HIP(hipGetLastError());
+        {  // the tables must be whole before any decoder reads them
+            int herr = 0;
+            d2h(&herr, err, 1, st);
+            if (herr) throw std::invalid_argument("corrupt o1 tANS tables");
+        }
+        o1_decode_parallel(host, dev, chains, dt, ovf, out, err, ws, st);
     } else {
         const uint64_t nr = get64(hp), nv = get64(hp), s_rcl = get64(hp), s_vcl = get64(hp), rbp = get64(hp),
                        vbp = get64(hp);
@@ -1080,7 +1560,7 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
         (void)rk; (void)vk;
         uint64_t* runlen = ws.take<uint64_t>(nr + 1);
         uint64_t* runsum = ws.take<uint64_t>(nr + 1);
-        hipLaunchKernelGGL(k_runs_decode, dim3(nblk(nr)), dim3(256), 0, st, rcl, roff, dev + o_rman, nr, runlen);
+        FZ_LAUNCH("rle_runs_decode", 0, k_runs_decode, dim3(nblk(nr)), dim3(256), 0, st, rcl, roff, dev + o_rman, nr, runlen);
         FZ_HIP(hipGetLastError());
         {
             size_t b = 0;
@@ -1089,7 +1569,7 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
             FZ_HIP(hipcub::DeviceScan::InclusiveSum(t, b, runlen, runsum, (int64_t)nr, st));
         }
         FZ_HIP(hipMemsetAsync(out, 0, n * 2, st));
-        if (nv) hipLaunchKernelGGL(k_values_place, dim3(nblk(nv)), dim3(256), 0, st, vcl, voff, dev + o_vman, nv,
+        if (nv) FZ_LAUNCH("rle_values_place", 0, k_values_place, dim3(nblk(nv)), dim3(256), 0, st, vcl, voff, dev + o_vman, nv,
                                    runsum, n, out);
         FZ_HIP(hipGetLastError());
     }

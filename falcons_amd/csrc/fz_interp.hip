@@ -301,10 +301,12 @@ static void launch_pass(const PassDesc& p, void* data, int16_t* codes, void* esc
     T* ev = static_cast<T*>(escval);
     const T* un = static_cast<const T*>(unpred);
     if (p.n <= 1) return;
+    const uint64_t per_target = DECOMP ? sizeof(T) + 2 : 2 * sizeof(T) + 2;
     if (p.type == 1) {
         const uint64_t total = (p.n / 2) * p.n1 * p.n2;
-        if (total) hipLaunchKernelGGL((k_lines<T, DECOMP>), dim3(blocks_for(total)), dim3(256), 0, st, p, total, d,
-                                      codes, ev, lv, avx2, e, un);
+        if (total)
+            FZ_LAUNCH("interp_lines", total * per_target, (k_lines<T, DECOMP>), dim3(blocks_for(total)), dim3(256), 0,
+                      st, p, total, d, codes, ev, lv, avx2, e, un);
         return;
     }
     const uint64_t n = p.n;
@@ -313,15 +315,16 @@ static void launch_pass(const PassDesc& p, void* data, int16_t* codes, void* esc
     const bool lin_dep_tail = !p.cubic && !p.skip && (n % 2 == 0) && n >= 4;
     const uint64_t rows_main = rows - (lin_dep_tail ? 1 : 0);
     const uint64_t t0 = p.no * rows_main * p.len;
-    if (t0) hipLaunchKernelGGL((k_rows<T, DECOMP>), dim3(blocks_for(t0)), dim3(256), 0, st, p, 0, t0, d, codes, ev,
-                               lv, avx2, e, un);
+    if (t0)
+        FZ_LAUNCH("interp_rows", t0 * per_target, (k_rows<T, DECOMP>), dim3(blocks_for(t0)), dim3(256), 0, st, p, 0,
+                  t0, d, codes, ev, lv, avx2, e, un);
     const uint64_t t1 = p.no * p.len;
     if (lin_dep_tail && t1)
-        hipLaunchKernelGGL((k_rows<T, DECOMP>), dim3(blocks_for(t1)), dim3(256), 0, st, p, 1, t1, d, codes, ev, lv,
-                           avx2, e, un);
+        FZ_LAUNCH("interp_rows_tail", t1 * per_target, (k_rows<T, DECOMP>), dim3(blocks_for(t1)), dim3(256), 0, st,
+                  p, 1, t1, d, codes, ev, lv, avx2, e, un);
     if (cub3_tail && t1)
-        hipLaunchKernelGGL((k_rows<T, DECOMP>), dim3(blocks_for(t1)), dim3(256), 0, st, p, 2, t1, d, codes, ev, lv,
-                           avx2, e, un);
+        FZ_LAUNCH("interp_rows_plane", t1 * per_target, (k_rows<T, DECOMP>), dim3(blocks_for(t1)), dim3(256), 0, st,
+                  p, 2, t1, d, codes, ev, lv, avx2, e, un);
 }
 
 void interp_pass(int dtype, bool decomp, const PassDesc& p, void* data, int16_t* codes, void* escval,
@@ -341,14 +344,14 @@ void interp_first(int dtype, bool decomp, void* data, int16_t* codes, void* escv
                   const uint64_t* esc_idx, uint64_t nesc, const void* unpred, hipStream_t st) {
     Escapes e{esc_idx, nesc};
     if (dtype == kF32) {
-        if (decomp) hipLaunchKernelGGL((k_first<float, true>), dim3(1), dim3(1), 0, st, (float*)data, codes,
+        if (decomp) FZ_LAUNCH("interp_first", 0, (k_first<float, true>), dim3(1), dim3(1), 0, st, (float*)data, codes,
                                        (float*)escval, eb_dev, e, (const float*)unpred);
-        else hipLaunchKernelGGL((k_first<float, false>), dim3(1), dim3(1), 0, st, (float*)data, codes, (float*)escval,
+        else FZ_LAUNCH("interp_first", 0, (k_first<float, false>), dim3(1), dim3(1), 0, st, (float*)data, codes, (float*)escval,
                                 eb_dev, e, (const float*)unpred);
     } else {
-        if (decomp) hipLaunchKernelGGL((k_first<double, true>), dim3(1), dim3(1), 0, st, (double*)data, codes,
+        if (decomp) FZ_LAUNCH("interp_first", 0, (k_first<double, true>), dim3(1), dim3(1), 0, st, (double*)data, codes,
                                        (double*)escval, eb_dev, e, (const double*)unpred);
-        else hipLaunchKernelGGL((k_first<double, false>), dim3(1), dim3(1), 0, st, (double*)data, codes,
+        else FZ_LAUNCH("interp_first", 0, (k_first<double, false>), dim3(1), dim3(1), 0, st, (double*)data, codes,
                                 (double*)escval, eb_dev, e, (const double*)unpred);
     }
     FZ_HIP(hipGetLastError());
@@ -359,14 +362,14 @@ void interp_anchors(int dtype, bool decomp, void* data, void* unpred, const uint
     const uint64_t total = cnt[0] * cnt[1] * cnt[2];
     if (!total) return;
     if (dtype == kF32) {
-        if (decomp) hipLaunchKernelGGL((k_anchors<float, true>), dim3(blocks_for(total)), dim3(256), 0, st,
+        if (decomp) FZ_LAUNCH("interp_anchors", total * 2 * (dtype == kF32 ? 4 : 8), (k_anchors<float, true>), dim3(blocks_for(total)), dim3(256), 0, st,
                                        (float*)data, (float*)unpred, cnt[0], cnt[1], cnt[2], str[0], str[1], str[2]);
-        else hipLaunchKernelGGL((k_anchors<float, false>), dim3(blocks_for(total)), dim3(256), 0, st, (float*)data,
+        else FZ_LAUNCH("interp_anchors", total * 2 * (dtype == kF32 ? 4 : 8), (k_anchors<float, false>), dim3(blocks_for(total)), dim3(256), 0, st, (float*)data,
                                 (float*)unpred, cnt[0], cnt[1], cnt[2], str[0], str[1], str[2]);
     } else {
-        if (decomp) hipLaunchKernelGGL((k_anchors<double, true>), dim3(blocks_for(total)), dim3(256), 0, st,
+        if (decomp) FZ_LAUNCH("interp_anchors", total * 2 * (dtype == kF32 ? 4 : 8), (k_anchors<double, true>), dim3(blocks_for(total)), dim3(256), 0, st,
                                        (double*)data, (double*)unpred, cnt[0], cnt[1], cnt[2], str[0], str[1], str[2]);
-        else hipLaunchKernelGGL((k_anchors<double, false>), dim3(blocks_for(total)), dim3(256), 0, st, (double*)data,
+        else FZ_LAUNCH("interp_anchors", total * 2 * (dtype == kF32 ? 4 : 8), (k_anchors<double, false>), dim3(blocks_for(total)), dim3(256), 0, st, (double*)data,
                                 (double*)unpred, cnt[0], cnt[1], cnt[2], str[0], str[1], str[2]);
     }
     FZ_HIP(hipGetLastError());
@@ -374,7 +377,7 @@ void interp_anchors(int dtype, bool decomp, void* data, void* unpred, const uint
 
 void level_constants(const double* eb_dev, const double* ratio_dev, int mode, int levels, LevelEb* out,
                      hipStream_t st) {
-    hipLaunchKernelGGL(k_level_eb, dim3(1), dim3(64), 0, st, eb_dev, ratio_dev, mode, levels, out);
+    FZ_LAUNCH("level_eb", 0, k_level_eb, dim3(1), dim3(64), 0, st, eb_dev, ratio_dev, mode, levels, out);
     FZ_HIP(hipGetLastError());
 }
 
@@ -384,12 +387,12 @@ void abs_error_bound(int dtype, const void* data, uint64_t n, int mode, double a
                      void* scratch, double* eb_dev, hipStream_t st) {
     const int nb = (int)std::min<uint64_t>(1024, (n + 255) / 256);
     if (dtype == kF32) {
-        hipLaunchKernelGGL(k_minmax<float>, dim3(nb), dim3(256), 0, st, (const float*)data, n, (float*)scratch);
-        hipLaunchKernelGGL(k_abs_eb<float>, dim3(1), dim3(1), 0, st, (const float*)scratch, nb, mode, abs_in, rel,
+        FZ_LAUNCH("minmax", n * 4, k_minmax<float>, dim3(nb), dim3(256), 0, st, (const float*)data, n, (float*)scratch);
+        FZ_LAUNCH("abs_eb", 0, k_abs_eb<float>, dim3(1), dim3(1), 0, st, (const float*)scratch, nb, mode, abs_in, rel,
                            psnr_factor, eb_dev);
     } else {
-        hipLaunchKernelGGL(k_minmax<double>, dim3(nb), dim3(256), 0, st, (const double*)data, n, (double*)scratch);
-        hipLaunchKernelGGL(k_abs_eb<double>, dim3(1), dim3(1), 0, st, (const double*)scratch, nb, mode, abs_in, rel,
+        FZ_LAUNCH("minmax", n * 8, k_minmax<double>, dim3(nb), dim3(256), 0, st, (const double*)data, n, (double*)scratch);
+        FZ_LAUNCH("abs_eb", 0, k_abs_eb<double>, dim3(1), dim3(1), 0, st, (const double*)scratch, nb, mode, abs_in, rel,
                            psnr_factor, eb_dev);
     }
     FZ_HIP(hipGetLastError());
@@ -407,6 +410,7 @@ void select_escapes(const int16_t* codes, uint64_t n, uint64_t* out_idx, uint64_
                     size_t scratch_bytes, hipStream_t st) {
     hipcub::CountingInputIterator<uint64_t> it(0);
     IsEscape pred{codes};
+    ProfScope ps("select_escapes", n * 2, st);
     FZ_HIP(hipcub::DeviceSelect::If(scratch, scratch_bytes, it, out_idx, nsel_dev, (int64_t)n, pred, st));
 }
 
@@ -414,10 +418,10 @@ void gather_escapes(int dtype, const uint64_t* idx, const uint64_t* nsel_dev, ui
                     void* out, hipStream_t st) {
     if (!max_n) return;
     if (dtype == kF32)
-        hipLaunchKernelGGL(k_gather_escapes<float>, dim3(blocks_for(max_n)), dim3(256), 0, st, idx, nsel_dev,
+        FZ_LAUNCH("gather_escapes", 0, k_gather_escapes<float>, dim3(blocks_for(max_n)), dim3(256), 0, st, idx, nsel_dev,
                            (const float*)escval, (float*)out);
     else
-        hipLaunchKernelGGL(k_gather_escapes<double>, dim3(blocks_for(max_n)), dim3(256), 0, st, idx, nsel_dev,
+        FZ_LAUNCH("gather_escapes", 0, k_gather_escapes<double>, dim3(blocks_for(max_n)), dim3(256), 0, st, idx, nsel_dev,
                            (const double*)escval, (double*)out);
     FZ_HIP(hipGetLastError());
 }

@@ -41,7 +41,7 @@ def power_law(shape, slope=-3.0, seed=2, dtype=np.float32):
     amp[nz] = k2[nz] ** (slope / 4.0)  # |k|^(slope/2)
     spec *= amp
     del amp, k2
-    f = np.fft.irfftn(spec, s=shape).astype(np.float64)
+    f = np.fft.irfftn(spec, s=shape, axes=tuple(range(len(shape)))).astype(np.float64)
     del spec
     f -= f.mean()
     f /= f.std()

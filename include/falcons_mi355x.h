@@ -127,6 +127,16 @@ int fz_synchronize(fz_ctx *ctx);
 /* ZSTD_versionNumber() of the libzstd the host stage loaded (0 if none could be loaded). */
 unsigned fz_zstd_version(void);
 
+/* Launch profiler (no counterpart in the reference: measurement for benchmarks and tuning). When on,
+ * every kernel launch of the context's calls is bracketed by HIP events on the context's stream and
+ * accumulated per kernel name with its algorithmic bytes (the bytes the kernel has to move at the
+ * least; 0 where not stated). Entries are kept in name order until fz_profile_reset. */
+int fz_profile_enable(fz_ctx *ctx, int on);
+int fz_profile_reset(fz_ctx *ctx);
+int fz_profile_count(const fz_ctx *ctx);
+int fz_profile_entry(const fz_ctx *ctx, int i, char *name, size_t name_cap, uint64_t *launches, double *total_ms,
+                     uint64_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -167,6 +167,7 @@ def test_device_undersized_pre_buffer(ctx):
     shape = (40, 40, 40)
     data = synth.sinusoids(shape, seed=2)
     cfg = falcons_amd.falconsConfig(*shape)
+    cfg.cmprAlgo = falcons_amd.falconsAlgorithm.INTERP
     cfg.absErrorBound = 1e-3
     d_in = ctx.alloc(data.nbytes)
     d_pre = ctx.alloc(64)
