@@ -11,12 +11,15 @@
 // mantissa bit scatter, block framing, run expansion, prefix sums) run across the whole chip. The
 // tANS chains themselves are sequential state machines; each chain is run by one lane here and all
 // chains of a call run concurrently.
+#include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <hipcub/hipcub.hpp>
 
 #include "fz_common.h"
 #include "fz_fse.h"
 #include "fz_kernels.h"
+#include "fz_select.h"
 
 namespace fz {
 
@@ -102,6 +105,40 @@ __device__ inline uint32_t br_read(BitR& r, uint32_t nb) {
     }
     v = (uint32_t)((w >> sh) & ((1ull << nb) - 1));
     return v;
+}
+
+// Backward reader with a 64-bit window in registers: bits [lo, lo + 64) of the stream (bytes below the
+// start read as zero), refilled when a read reaches below lo; a refill covers at least 57 bits.
+struct BitWin {
+    const uint8_t* buf;
+    int64_t pos;  // bits below pos are unread
+    int64_t lo;
+    uint64_t w;
+};
+__device__ inline void bwin_fill(BitWin& r) {
+    const int64_t b0 = (r.pos - 57) >> 3;  // floor for negatives
+    uint64_t w = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int64_t b = b0 + j;
+        if (b >= 0) w |= (uint64_t)r.buf[b] << (8 * j);
+    }
+    r.w = w;
+    r.lo = b0 * 8;
+}
+__device__ inline BitWin bwin_at(const uint8_t* buf, int64_t pos) {
+    BitWin r{buf, pos, 0, 0};
+    bwin_fill(r);
+    return r;
+}
+__device__ inline uint32_t bwin_read(BitWin& r, uint32_t nb) {
+    const int64_t p = r.pos - nb;
+    if (p < r.lo) {
+        r.pos = p + nb;
+        bwin_fill(r);
+    }
+    r.pos = p;
+    return nb ? (uint32_t)((r.w >> (p - r.lo)) & ((1ull << nb) - 1)) : 0u;
 }
 
 // ---------------------------------------------------------------- preprocess (RleFse:850-895)
@@ -413,12 +450,10 @@ __global__ void k_add_one(uint32_t* cnt, int total) {
     if (i < total) cnt[i] += 1;
 }
 
-// o1_normalize (RleFse:410-477) and FSE_buildCTable for each context; one workgroup per context.
-__global__ void k_o1_tables(const uint32_t* cnt, uint16_t* nf, CTable* ct, uint8_t* spread_scratch) {
-    const int c = blockIdx.x;
-    if (threadIdx.x != 0) return;
-    const uint32_t* h = cnt + c * kO1Sym;
-    uint16_t* f = nf + c * kO1Sym;
+// o1_normalize (RleFse:410-477): FSE_normalizeCount at tableLog 13, with the reference's fallback
+// (proportional counts, surplus to the largest, deficit taken from the largest ones) when it fails
+// or when a context has at most one symbol. Runs in one lane; f receives the counts.
+__device__ void o1_normalize(const uint32_t* h, uint16_t* f) {
     uint64_t total = 0;
     int nonzero = 0;
     for (int s = 0; s < kO1Sym; s++) { total += h[s]; nonzero += h[s] != 0; }
@@ -469,8 +504,109 @@ __global__ void k_o1_tables(const uint32_t* cnt, uint16_t* nf, CTable* ct, uint8
             }
         }
     }
-    for (int s = 0; s < kO1Sym; s++) norm[s] = (int16_t)f[s];
-    build_ctable(&ct[c], norm, kO1Sym - 1, kO1Log, spread_scratch + (size_t)c * (1u << kO1Log));
+}
+
+// FSE symbol spread and table construction for one context in LDS, by a 256-thread workgroup.
+// Spread (zfse/fse_compress.c FSE_buildCTable_wksp, fse_decompress.c FSE_buildDTable): symbols with
+// count -1 take the top positions in symbol order; the k-th occurrence of the others (in symbol
+// order) goes to the k-th position of the walk j*step mod 2^13, j = 0, 1, ..., that lies at or
+// below the high threshold; the walk is enumerated in parallel and ranked with a block scan.
+// Within a symbol, positions in increasing order get consecutive states: one thread per symbol
+// scans the spread table.
+struct TableLds {
+    uint8_t sym[1u << kO1Log];
+    int16_t norm[kO1Sym];
+    uint32_t cum[kO1Sym + 1];   // exclusive prefix of the counts (-1 counts as 1)
+    uint32_t cump[kO1Sym + 1];  // exclusive prefix of the positive counts
+    uint32_t part[256];
+    uint32_t high;
+    int bad;
+};
+
+__device__ bool spread_parallel(TableLds& L) {
+    const unsigned t = threadIdx.x;
+    constexpr uint32_t ts = 1u << kO1Log, mask = ts - 1, step = (ts >> 1) + (ts >> 3) + 3;
+    if (t == 0) {
+        uint32_t c = 0, cp = 0, high = ts - 1;
+        int bad = 0;
+        for (int s = 0; s < kO1Sym; s++) {
+            const int v = L.norm[s];
+            if (v < -1 || v > (int)ts) bad = 1;
+            L.cum[s] = c;
+            L.cump[s] = cp;
+            if (v == -1) { c += 1; L.sym[high--] = (uint8_t)s; }
+            else if (v > 0) { c += (uint32_t)v; cp += (uint32_t)v; }
+        }
+        L.cum[kO1Sym] = c;
+        L.cump[kO1Sym] = cp;
+        if (c != ts) bad = 1;
+        L.high = high;
+        L.bad = bad;
+    }
+    __syncthreads();
+    if (L.bad) return false;
+    const uint32_t high = L.high;
+    constexpr uint32_t per = ts / 256;
+    uint32_t cnt = 0;
+    for (uint32_t k = 0; k < per; k++) cnt += (((t * per + k) * step) & mask) <= high;
+    L.part[t] = cnt;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < 256; i++) { const uint32_t v = L.part[i]; L.part[i] = acc; acc += v; }
+    }
+    __syncthreads();
+    uint32_t rank = L.part[t];
+    for (uint32_t k = 0; k < per; k++) {
+        const uint32_t pos = ((t * per + k) * step) & mask;
+        if (pos > high) continue;
+        int lo = 0, hi = kO1Sym;  // last symbol with cump <= rank: the one whose counts hold rank
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (L.cump[mid] <= rank) lo = mid; else hi = mid;
+        }
+        L.sym[pos] = (uint8_t)lo;
+        rank++;
+    }
+    __syncthreads();
+    return true;
+}
+
+// Encoding tables of every context of every segment: grid (contexts, segments).
+__global__ void __launch_bounds__(256) k_o1_tables(const uint32_t* cnt_all, uint16_t* nf_all, CTable* ct_all) {
+    __shared__ TableLds L;
+    const unsigned t = threadIdx.x;
+    const size_t c = (size_t)blockIdx.y * kO1Ctx + blockIdx.x;
+    uint16_t* nf = nf_all + c * kO1Sym;
+    CTable* ct = ct_all + c;
+    if (t == 0) o1_normalize(cnt_all + c * kO1Sym, nf);
+    __syncthreads();
+    L.norm[t] = (int16_t)nf[t];
+    __syncthreads();
+    if (!spread_parallel(L)) return;  // cannot happen for counts from o1_normalize
+    constexpr uint32_t ts = 1u << kO1Log;
+    const int nc = L.norm[t];
+    uint32_t slot = L.cum[t];
+    if (nc != 0) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(L.sym);
+        for (uint32_t u4 = 0; u4 < ts / 4; u4++) {
+            const uint32_t v = w[u4];
+            for (int b = 0; b < 4; b++)
+                if (((v >> (8 * b)) & 0xFFu) == t) ct->stateTable[slot++] = (uint16_t)(ts + 4 * u4 + b);
+        }
+    }
+    if (nc == 0) {
+        ct->deltaNbBits[t] = ((kO1Log + 1) << 16) - ts;
+        ct->deltaFindState[t] = 0;
+    } else if (nc == -1 || nc == 1) {
+        ct->deltaNbBits[t] = (kO1Log << 16) - ts;
+        ct->deltaFindState[t] = (int32_t)L.cum[t] - 1;
+    } else {
+        const uint32_t maxBitsOut = kO1Log - highbit32((uint32_t)(nc - 1));
+        ct->deltaNbBits[t] = (maxBitsOut << 16) - ((uint32_t)nc << maxBitsOut);
+        ct->deltaFindState[t] = (int32_t)L.cum[t] - nc;
+    }
+    if (t == 0) { ct->tableLog = kO1Log; ct->maxSym = kO1Sym - 1; }
 }
 
 // ---- order-1 tANS encoding split into pieces --------------------------------------------------
@@ -684,98 +820,61 @@ __global__ void k_o1_close(const int16_t* bins, O1Chains C, const CTable* ct_all
     blen[q] = (b0 + kO1Log + 1 + 7) >> 3;
 }
 
-// Side stream of escaped codes (RleFse:483-532), one workgroup per segment.
-__global__ void __launch_bounds__(256) k_o1_ovf(const int16_t* ovf_all, const uint64_t* ovf_off, uint8_t* vcl_buf,
-                                                uint8_t* vman_buf, uint8_t* tmp_buf, uint8_t* out_buf, uint64_t stride,
-                                                uint64_t* out_len) {
-    __shared__ FseLds L;
-    __shared__ uint64_t s_vbp, s_svcl;
-    const int g = blockIdx.x;
-    const uint64_t o = ovf_off[g], n = ovf_off[g + 1] - o;
-    const int16_t* ovf = ovf_all + o;
-    uint8_t* out = out_buf + g * stride;
-    uint8_t* vcl = vcl_buf + g * stride;
-    uint8_t* vman = vman_buf + g * stride;
-    uint8_t* tmp = tmp_buf + g * stride;
-    if (n == 0) {
-        if (threadIdx.x == 0) { out[0] = 0; out_len[g] = 1; }
-        return;
-    }
-    if (threadIdx.x == 0) {
-        uint64_t bp = 0, acc = 0;
-        int an = 0;
-        for (uint64_t i = 0; i < n; i++) {
-            const int32_t f = ovf[i];
-            const uint32_t zz = ((uint32_t)f << 1) ^ (uint32_t)(f >> 31);
-            const uint64_t vv = (uint64_t)zz + 1;
-            const int vk = floor_log2_64(vv);
-            vcl[i] = (uint8_t)vk;
-            acc = (acc << vk) | (vv & ((1ull << vk) - 1));
-            an += vk;
-            while (an >= 8) { an -= 8; vman[bp++] = (uint8_t)(acc >> an); }
-        }
-        if (an > 0) vman[bp++] = (uint8_t)(acc << (8 - an));
-        s_vbp = bp;
-    }
-    __syncthreads();
-    // class bytes through the framed FSE blocks, written after the 17-byte head
-    uint64_t p = 0;
-    uint8_t* body = out + 17;
-    for (uint64_t off = 0; off < n; off += kFblk) {
-        const uint32_t len = (uint32_t)min<uint64_t>(kFblk, n - off);
-        p += frame_block_wg(vcl + off, len, tmp, body + p, L);
-    }
-    if (threadIdx.x == 0) s_svcl = p;
-    __syncthreads();
-    const uint64_t vbp = s_vbp, svcl = s_svcl;
-    const uint64_t raw = n * 2, cm = svcl + vbp + 16;
-    if (cm >= raw) {
-        if (threadIdx.x == 0) out[0] = 0;
-        const uint8_t* src = reinterpret_cast<const uint8_t*>(ovf);
-        for (uint64_t i = threadIdx.x; i < raw; i += blockDim.x) out[1 + i] = src[i];
-        if (threadIdx.x == 0) out_len[g] = 1 + raw;
-    } else {
-        if (threadIdx.x == 0) {
-            out[0] = 1;
-            for (int b = 0; b < 8; b++) { out[1 + b] = (uint8_t)(svcl >> (8 * b)); out[9 + b] = (uint8_t)(vbp >> (8 * b)); }
-        }
-        for (uint64_t i = threadIdx.x; i < vbp; i += blockDim.x) body[svcl + i] = vman[i];
-        if (threadIdx.x == 0) out_len[g] = 17 + svcl + vbp;
-    }
-}
-
 __global__ void k_gather_i16(const int16_t* src, const uint64_t* idx, uint64_t n, int16_t* out) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = src[idx[i]];
 }
 
 // ---------------------------------------------------------------- method 1 decode
-// nf may sit at any byte offset of the stream, so it is read byte by byte.
-__global__ void k_o1_dtables(const uint8_t* nf, DEntry* dt, uint8_t* spread_scratch, int* err) {
-    const int c = blockIdx.x;
-    if (threadIdx.x != 0) return;
-    int16_t norm[kO1Sym];
-    for (int s = 0; s < kO1Sym; s++) {
-        const uint8_t* b = nf + 2 * (c * kO1Sym + s);
-        norm[s] = (int16_t)(uint16_t)(b[0] | (b[1] << 8));
+// Decoding tables of every context of every segment, grid (contexts, segments); the counts are read
+// from the stream byte by byte (they may sit at any offset). Invalid counts set *err.
+__global__ void __launch_bounds__(256) k_o1_dtables(const uint8_t* stream, const uint64_t* nf_off, DEntry* dt_all,
+                                                    int* err) {
+    __shared__ TableLds L;
+    const unsigned t = threadIdx.x;
+    const uint8_t* b = stream + nf_off[blockIdx.y] + 2 * ((size_t)blockIdx.x * kO1Sym + t);
+    L.norm[t] = (int16_t)(uint16_t)(b[0] | (b[1] << 8));
+    __syncthreads();
+    constexpr uint32_t ts = 1u << kO1Log;
+    DEntry* dt = dt_all + ((size_t)blockIdx.y * kO1Ctx + blockIdx.x) * ts;
+    if (!spread_parallel(L)) {
+        if (t == 0) *err = 1;
+        return;
     }
-    if (!build_dtable(dt + (size_t)c * (1u << kO1Log), norm, kO1Sym - 1, kO1Log,
-                      spread_scratch + (size_t)c * (1u << kO1Log)))
-        *err = 1;
+    const int nc = L.norm[t];
+    if (nc == 0) return;
+    uint32_t next = (nc == -1) ? 1u : (uint32_t)nc;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(L.sym);
+    for (uint32_t u4 = 0; u4 < ts / 4; u4++) {
+        const uint32_t v = w[u4];
+        for (int k = 0; k < 4; k++)
+            if (((v >> (8 * k)) & 0xFFu) == t) {
+                const uint32_t ns = next++;
+                const uint32_t nb = kO1Log - highbit32(ns);
+                dt[4 * u4 + k] = DEntry{(uint16_t)((ns << nb) - ts), (uint8_t)t, (uint8_t)nb};
+            }
+    }
 }
 
 // ---- order-1 tANS decoding split by bit position ----------------------------------------------
 // A chain's bitstream is cut into subsequences of sub_bits bits. Subsequence r owns the symbols
 // whose decoding starts at a read position B in (b_{r+1}, b_r], b_r = pos0 - r * sub_bits; the
-// last one owns everything below b_{R-1}. A decoder started at a wrong state on a wrong boundary
-// falls into step with the true one after some hundreds of symbols (both land on the same read
-// position, state and context), so all subsequences first decode from a guess (round 1); each
-// round then restarts a subsequence from its left neighbour's exit (read position, state, context)
-// and from the escape index given by the escape counts of the subsequences before it, when either
-// changed (the escape index only matters to a subsequence that read escape signs). With no change
-// left every start is the true one; k_o1d_out then writes the codes at offsets given by the
-// symbol counts. The context after an escape depends on the sign of the escaped code, taken from
-// the already decoded side stream.
+// last one owns everything below b_{R-1}.
+//
+// A decoder started at a wrong state on a wrong boundary falls into step with the true one after
+// some hundreds of symbols (both land on the same read position, state and context). What breaks
+// plain speculation is the escape symbol: the context after it depends on the sign of the escaped
+// code, which sits in the side stream at an index (the number of escapes before) that a
+// subsequence does not know. So each subsequence decodes from a *set* of possible start states and
+// follows both contexts after every escape; branches that reach the same read position, state and
+// context are merged. The result is a small graph: every escape met is a node with two out-edges
+// (escaped code >= 0 / < 0), every start state an origin edge, and every edge ends at the next node
+// on its branch or at an exit state, carrying the number of symbols decoded along it. The exit
+// states of a subsequence are the start set of the next; rounds repeat for subsequences whose
+// start set changed, and do not depend on escape indices. A walk per chain then follows the true
+// path through the graphs with the real escape signs (cost proportional to the escapes), giving
+// every subsequence its exact start, escape index and output offset; k_o1d_out decodes from those.
+// Subsequences whose graph outgrows its capacity are decoded exactly by the walk instead.
 struct O1DecChain {
     uint64_t off, len;      // stream bytes
     uint64_t lo, hi;        // output codes
@@ -803,13 +902,11 @@ __device__ inline int o1d_chain_of(const O1DecChain* ch, int nch, uint64_t t) {
     return lo;
 }
 
-// One decoder step; returns the code. ovf_lim guards the side-stream index.
-__device__ inline int o1d_step(const DEntry* dt, const uint8_t* buf, int64_t& B, uint32_t& X, uint32_t& ctx,
-                               uint64_t& e, const int16_t* ovf, uint64_t ovf_lim, bool& bad) {
+// One decoder step; returns the code. r.pos is the read position; ovf_lim guards the side-stream index.
+__device__ inline int o1d_step(const DEntry* dt, BitWin& r, uint32_t& X, uint32_t& ctx, uint64_t& e,
+                               const int16_t* ovf, uint64_t ovf_lim, bool& bad) {
     const DEntry d = dt[(size_t)ctx * (1u << kO1Log) + (X & ((1u << kO1Log) - 1))];
-    BitR r{buf, B};
-    X = d.newState + br_read(r, d.nbBits);
-    B = r.pos;
+    X = d.newState + bwin_read(r, d.nbBits);
     int code;
     if (d.symbol == 255) {
         if (e < ovf_lim) code = ovf[e];
@@ -823,81 +920,296 @@ __device__ inline int o1d_step(const DEntry* dt, const uint8_t* buf, int64_t& B,
     return code;
 }
 
-// One round over the subsequences flagged in run[] (all but the last of each chain in round 1).
-__global__ void __launch_bounds__(256) k_o1d_round(const uint8_t* in, const O1DecChain* ch, int nch, uint64_t nsub,
-                                                   const DEntry* dt_all, const int16_t* ovf, const uint8_t* run,
-                                                   const O1DecState* start, const uint64_t* e_start,
-                                                   O1DecState* exit_st, uint64_t* kesc, uint64_t* cnt, uint8_t* used) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nsub || !run[t]) return;
-    const O1DecChain c = ch[o1d_chain_of(ch, nch, t)];
-    const uint64_t r = t - c.tfirst;
-    if (r + 1 >= c.nsub) return;  // the last subsequence has no successor to feed
-    const DEntry* dt = dt_all + c.dset * kO1Ctx * (1u << kO1Log);
-    const int64_t stop = c.pos0 - (int64_t)(r + 1) * c.sub_bits;
-    O1DecState s = start[t];
-    uint64_t e = e_start[t];
-    const uint64_t e0 = e, m = c.hi - c.lo;
-    uint64_t n = 0;
-    bool bad = false;
-    while (s.B > stop && n < m) {
-        o1d_step(dt, in + c.off, s.B, s.X, s.ctx, e, ovf + c.ovf_base, c.novf, bad);
-        n++;
-    }
-    exit_st[t] = s;
-    kesc[t] = e - e0;
-    cnt[t] = n;
-    used[t] = (e != e0) ? 1 : 0;
+// Graph capacity. Where escapes are dense the branches that follow wrong signs keep meeting
+// spurious escapes and the graph grows without bound; such subsequences give up early and are
+// decoded exactly by the walk.
+constexpr int kGBr = 16;     // live branches
+constexpr int kGNode = 24;   // escapes per subsequence graph
+constexpr int kGPend = 48;   // open edges
+
+struct O1GEdge {
+    int32_t tgt;   // >= 0: node, < 0: exit -1-tgt
+    uint32_t sym;  // symbols decoded along the edge (the escape ending it included)
+};
+
+struct O1StartSet {
+    O1DecState s[kGBr];
+    int n;
+    int pad;
+};
+
+struct O1Graph {
+    O1DecState exit[kGBr];
+    int nexit, nnode, status, pad;  // status 0: complete, 1: capacity exceeded
+    O1GEdge origin[kGBr];
+    O1GEdge node[kGNode][2];
+};
+
+__device__ inline bool o1d_same(const O1DecState& a, const O1DecState& b) {
+    return a.B == b.B && a.X == b.X && a.ctx == b.ctx;
 }
 
 // Starts of round 1: the true start for subsequence 0, a guess elsewhere.
-__global__ void __launch_bounds__(256) k_o1d_init(const O1DecChain* ch, int nch, uint64_t nsub, O1DecState* start,
-                                                  uint64_t* e_start, uint8_t* run) {
+__global__ void __launch_bounds__(256) k_o1d_init(const O1DecChain* ch, int nch, uint64_t nsub, O1StartSet* start,
+                                                  uint8_t* run) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nsub) return;
     const O1DecChain c = ch[o1d_chain_of(ch, nch, t)];
     const uint64_t r = t - c.tfirst;
-    O1DecState s;
-    if (r == 0) {
-        s = O1DecState{c.pos0, c.x0, 0};
-        e_start[t] = 0;
-    } else {
-        s = O1DecState{c.pos0 - (int64_t)r * c.sub_bits, (uint32_t)((r * 2654435761u) & ((1u << kO1Log) - 1)), 0};
-        e_start[t] = (uint64_t)((double)c.novf * (double)(r * c.sub_bits) / (double)(c.pos0 > 0 ? c.pos0 : 1));
-    }
-    start[t] = s;
+    O1StartSet ss;
+    ss.n = 1;
+    ss.pad = 0;
+    if (r == 0) ss.s[0] = O1DecState{c.pos0, c.x0, 0};
+    else ss.s[0] = O1DecState{c.pos0 - (int64_t)r * c.sub_bits, (uint32_t)((r * 2654435761u) & ((1u << kO1Log) - 1)), 0};
+    start[t] = ss;
     run[t] = 1;
 }
 
-// Links the rounds: subsequence r > 0 restarts from the exit of r-1 with the escape index of the
-// escapes before it; it runs again if its start moved, or if its escape index moved and it read
-// escape signs.
-__global__ void __launch_bounds__(256) k_o1d_link(const O1DecChain* ch, int nch, uint64_t nsub,
-                                                  const O1DecState* exit_st, const uint64_t* kscan,
-                                                  const uint8_t* used, O1DecState* start, uint64_t* e_start,
-                                                  uint8_t* run, unsigned int* nrun) {
+// Graph of one subsequence from its start set (all but the last subsequence of each chain). One lane
+// per subsequence; while a single branch is live (most of the time) its state stays in registers,
+// otherwise branches and open edges live in LDS, one column per lane.
+constexpr int kGLanes = 64;
+
+__global__ void __launch_bounds__(kGLanes) k_o1d_graph(const uint8_t* in, const O1DecChain* ch, int nch, uint64_t nsub,
+                                                       const DEntry* dt_all, const uint8_t* run,
+                                                       const O1StartSet* start, O1Graph* graph) {
+    __shared__ int64_t bB[kGBr][kGLanes];
+    __shared__ uint32_t bX[kGBr][kGLanes];
+    __shared__ uint32_t bN[kGBr][kGLanes];  // symbols decoded on the branch
+    __shared__ uint8_t bC[kGBr][kGLanes];   // context
+    __shared__ int32_t pS[kGPend][kGLanes]; // open edge: >= 0 origin slot, < 0 node edge -1-(2 node + sign)
+    __shared__ uint8_t pB[kGPend][kGLanes]; // its branch
+    __shared__ uint32_t pC[kGPend][kGLanes];// branch count where it started
+    const int l = threadIdx.x;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + l;
+    if (t >= nsub || !run[t]) return;
+    const O1DecChain c = ch[o1d_chain_of(ch, nch, t)];
+    const uint64_t r = t - c.tfirst;
+    if (r + 1 >= c.nsub) return;
+    const DEntry* dt = dt_all + c.dset * kO1Ctx * (1u << kO1Log);
+    const uint8_t* buf = in + c.off;
+    const int64_t stop = c.pos0 - (int64_t)(r + 1) * c.sub_bits;
+    const uint32_t cap = (uint32_t)min<uint64_t>(c.hi - c.lo, 0xFFFFFFFFull);
+    O1Graph& G = graph[t];
+    int nb = 0, np = 0, nexit = 0, nnode = 0, status = 0;
+    {
+        const O1StartSet& ss = start[t];
+        const int n0 = min(ss.n, kGBr);
+        for (int o = 0; o < n0; o++) {
+            const O1DecState s0 = ss.s[o];
+            bB[nb][l] = s0.B; bX[nb][l] = s0.X; bC[nb][l] = (uint8_t)s0.ctx; bN[nb][l] = 0;
+            pS[np][l] = o; pB[np][l] = (uint8_t)nb; pC[np][l] = 0;
+            np++;
+            nb++;
+        }
+    }
+    auto close_edges = [&](int b, int32_t tgt) {
+        for (int k = 0; k < np;) {
+            if (pB[k][l] == b) {
+                const O1GEdge e{tgt, bN[b][l] - pC[k][l]};
+                const int32_t sl = pS[k][l];
+                if (sl >= 0) G.origin[sl] = e;
+                else { const int q = -1 - sl; G.node[q >> 1][q & 1] = e; }
+                np--;
+                pS[k][l] = pS[np][l]; pB[k][l] = pB[np][l]; pC[k][l] = pC[np][l];
+            } else {
+                k++;
+            }
+        }
+    };
+    auto drop_branch = [&](int b) {
+        nb--;
+        if (b != nb) {
+            bB[b][l] = bB[nb][l]; bX[b][l] = bX[nb][l]; bC[b][l] = bC[nb][l]; bN[b][l] = bN[nb][l];
+            for (int k = 0; k < np; k++)
+                if (pB[k][l] == nb) pB[k][l] = (uint8_t)b;
+        }
+    };
+    auto exit_branch = [&](int b) {
+        const O1DecState sb{bB[b][l], bX[b][l], bC[b][l]};
+        int x = 0;
+        while (x < nexit && !o1d_same(G.exit[x], sb)) x++;
+        if (x == nexit) {
+            if (nexit == kGBr) { status = 1; return; }
+            G.exit[nexit++] = sb;
+        }
+        close_edges(b, -1 - x);
+        drop_branch(b);
+    };
+    // an escape decoded on branch b: new node, both contexts continue
+    auto fork_branch = [&](int b) {
+        if (nnode == kGNode) { status = 2; return; }
+        if (nb == kGBr) { status = 3; return; }
+        if (np + 2 > kGPend) { status = 4; return; }
+        const int n = nnode++;
+        close_edges(b, n);
+        bB[nb][l] = bB[b][l]; bX[nb][l] = bX[b][l]; bN[nb][l] = bN[b][l];
+        bC[b][l] = 13;   // escaped code >= 128
+        bC[nb][l] = 14;  // escaped code <= -128
+        pS[np][l] = -1 - 2 * n; pB[np][l] = (uint8_t)b; pC[np][l] = bN[b][l]; np++;
+        pS[np][l] = -1 - (2 * n + 1); pB[np][l] = (uint8_t)nb; pC[np][l] = bN[b][l]; np++;
+        nb++;
+    };
+    while (nb > 0 && status == 0) {
+        if (nb == 1) {  // single branch: registers only
+            BitWin rd = bwin_at(buf, bB[0][l]);
+            uint32_t X = bX[0][l], cx = bC[0][l], cnt = bN[0][l];
+            bool esc = false;
+            while (rd.pos > stop && cnt < cap) {
+                const DEntry d = dt[(size_t)cx * (1u << kO1Log) + (X & ((1u << kO1Log) - 1))];
+                X = d.newState + bwin_read(rd, d.nbBits);
+                cnt++;
+                if (d.symbol == 255) { esc = true; break; }
+                const unsigned sy = d.symbol;
+                cx = (uint32_t)o1_ctx((int)((sy >> 1) ^ (~(sy & 1) + 1)));
+            }
+            bB[0][l] = rd.pos; bX[0][l] = X; bC[0][l] = (uint8_t)cx; bN[0][l] = cnt;
+            if (esc) fork_branch(0);
+            else exit_branch(0);
+            continue;
+        }
+        int i = 0;
+        for (int j = 1; j < nb; j++)
+            if (bB[j][l] > bB[i][l]) i = j;
+        if (bB[i][l] <= stop || bN[i][l] >= cap) {
+            exit_branch(i);
+            continue;
+        }
+        const uint32_t cx0 = bC[i][l], X0 = bX[i][l];
+        const DEntry d = dt[(size_t)cx0 * (1u << kO1Log) + (X0 & ((1u << kO1Log) - 1))];
+        BitWin rd = bwin_at(buf, bB[i][l]);
+        const uint32_t X = d.newState + bwin_read(rd, d.nbBits);
+        const int64_t B = rd.pos;
+        bB[i][l] = B; bX[i][l] = X; bN[i][l] = bN[i][l] + 1;
+        if (d.symbol == 255) {
+            fork_branch(i);
+            continue;
+        }
+        const unsigned sy = d.symbol;
+        const uint32_t cx = (uint32_t)o1_ctx((int)((sy >> 1) ^ (~(sy & 1) + 1)));
+        bC[i][l] = (uint8_t)cx;
+        for (int j = 0; j < nb; j++) {
+            if (j == i || bB[j][l] != B || bX[j][l] != X || bC[j][l] != cx) continue;
+            const uint32_t di = bN[i][l], dj = bN[j][l];
+            for (int k = 0; k < np; k++)
+                if (pB[k][l] == i) { pC[k][l] += dj - di; pB[k][l] = (uint8_t)j; }
+            drop_branch(i);
+            break;
+        }
+    }
+    G.nexit = nexit;
+    G.nnode = nnode;
+    G.status = status;
+}
+
+// Next round's start sets from this round's exits; counts the subsequences to run again.
+__global__ void __launch_bounds__(256) k_o1d_glink(const O1DecChain* ch, int nch, uint64_t nsub, const O1Graph* graph,
+                                                   O1StartSet* start, uint8_t* run, unsigned int* nrun) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nsub) return;
     const O1DecChain c = ch[o1d_chain_of(ch, nch, t)];
     const uint64_t r = t - c.tfirst;
     uint8_t go = 0;
     if (r > 0) {
-        const O1DecState ns = exit_st[t - 1];
-        const uint64_t ne = kscan[t] - kscan[c.tfirst];
-        const O1DecState os = start[t];
-        const bool moved = ns.B != os.B || ns.X != os.X || ns.ctx != os.ctx;
-        if (moved || (ne != e_start[t] && used[t])) go = 1;
-        start[t] = ns;
-        e_start[t] = ne;
+        const O1Graph& g = graph[t - 1];
+        O1StartSet& ss = start[t];
+        if (g.status == 0) {
+            bool same = ss.n == g.nexit;
+            for (int k = 0; same && k < g.nexit; k++) same = o1d_same(ss.s[k], g.exit[k]);
+            if (!same) {
+                ss.n = g.nexit;
+                for (int k = 0; k < g.nexit; k++) ss.s[k] = g.exit[k];
+                go = 1;
+            }
+        }
         if (go && r + 1 < c.nsub) atomicAdd(nrun, 1u);
     }
     run[t] = go;
 }
 
+// One workgroup per chain: the true path through the graphs. Gives every subsequence its start
+// state, escape index and output offset (relative to the chain). A subsequence without a usable
+// graph (capacity exceeded, or a start state missing from its set) is decoded exactly here, by one
+// lane, with the tables of the contexts after large codes (11..14, most of the symbols where
+// escapes are dense) staged in LDS.
+constexpr int kWalkCtx0 = 11, kWalkCtxN = 4;
+
+__global__ void __launch_bounds__(256) k_o1d_walk(const uint8_t* in, const O1DecChain* ch, int nch,
+                                                  const DEntry* dt_all, const int16_t* ovf, const O1StartSet* start,
+                                                  const O1Graph* graph, O1DecState* res_start, uint64_t* res_e,
+                                                  uint64_t* res_off, unsigned int* stats, int* err) {
+    __shared__ DEntry hot[kWalkCtxN << kO1Log];
+    const int q = blockIdx.x;
+    if (q >= nch) return;
+    const O1DecChain c = ch[q];
+    const DEntry* dt = dt_all + c.dset * kO1Ctx * (1u << kO1Log);
+    for (uint32_t i = threadIdx.x; i < (kWalkCtxN << kO1Log); i += blockDim.x)
+        hot[i] = dt[((size_t)kWalkCtx0 << kO1Log) + i];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int16_t* ov = ovf + c.ovf_base;
+    O1DecState z{c.pos0, c.x0, 0};
+    uint64_t e = 0, off = 0;
+    const uint64_t m = c.hi - c.lo;
+    for (uint64_t r = 0; r < c.nsub; r++) {
+        const uint64_t t = c.tfirst + r;
+        res_start[t] = z;
+        res_e[t] = e;
+        res_off[t] = off;
+        if (r + 1 == c.nsub) break;
+        const O1Graph& g = graph[t];
+        const O1StartSet& ss = start[t];
+        int o = -1;
+        if (g.status == 0)
+            for (int k = 0; k < ss.n; k++)
+                if (o1d_same(ss.s[k], z)) { o = k; break; }
+        if (o >= 0) {
+            O1GEdge ed = g.origin[o];
+            off += ed.sym;
+            int guard = 0;
+            while (ed.tgt >= 0 && guard++ <= kGNode) {
+                const int sign = (e < c.novf) ? (ov[e] < 0) : 0;
+                e++;
+                ed = g.node[ed.tgt][sign];
+                off += ed.sym;
+            }
+            if (ed.tgt >= 0 || -1 - ed.tgt >= g.nexit) { *err = 1; return; }
+            z = g.exit[-1 - ed.tgt];
+        } else {
+            atomicAdd(&stats[g.status ? 0 : 1], 1u);  // capacity exceeded / start state not in the set
+            if (g.status) atomicAdd(&stats[1 + g.status], 1u);
+            const int64_t stop = c.pos0 - (int64_t)(r + 1) * c.sub_bits;
+            bool bad = false;
+            uint64_t n = 0;
+            BitWin rd = bwin_at(in + c.off, z.B);
+            while (rd.pos > stop && off + n < m) {
+                const uint32_t xi = z.X & ((1u << kO1Log) - 1);
+                const uint32_t hc = z.ctx - kWalkCtx0;
+                const DEntry d = hc < (uint32_t)kWalkCtxN ? hot[(hc << kO1Log) + xi] : dt[((size_t)z.ctx << kO1Log) + xi];
+                z.X = d.newState + bwin_read(rd, d.nbBits);
+                int code;
+                if (d.symbol == 255) {
+                    if (e < c.novf) code = ov[e];
+                    else { code = 32767; bad = true; }
+                    e++;
+                } else {
+                    const unsigned sy = d.symbol;
+                    code = (int)((sy >> 1) ^ (~(sy & 1) + 1));
+                }
+                z.ctx = (uint32_t)o1_ctx(code);
+                n++;
+            }
+            z.B = rd.pos;
+            off += n;
+            if (bad) { *err = 1; return; }
+        }
+    }
+}
+
 // Final pass: every subsequence decodes from its settled start into its place in the code array.
 __global__ void __launch_bounds__(256) k_o1d_out(const uint8_t* in, const O1DecChain* ch, int nch, uint64_t nsub,
                                                  const DEntry* dt_all, const int16_t* ovf, const O1DecState* start,
-                                                 const uint64_t* e_start, const uint64_t* cscan, int16_t* out,
+                                                 const uint64_t* e_start, const uint64_t* res_off, int16_t* out,
                                                  int* err) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nsub) return;
@@ -908,27 +1220,16 @@ __global__ void __launch_bounds__(256) k_o1d_out(const uint8_t* in, const O1DecC
     const int64_t stop = last ? INT64_MIN : c.pos0 - (int64_t)(r + 1) * c.sub_bits;
     O1DecState s = start[t];
     uint64_t e = e_start[t];
-    uint64_t i = c.lo + (cscan[t] - cscan[c.tfirst]);
+    uint64_t i = c.lo + res_off[t];
     bool bad = false;
-    while (i < c.hi && s.B > stop) {
-        out[i++] = (int16_t)o1d_step(dt, in + c.off, s.B, s.X, s.ctx, e, ovf + c.ovf_base, c.novf, bad);
+    BitWin rd = bwin_at(in + c.off, s.B);
+    while (i < c.hi && rd.pos > stop) {
+        out[i++] = (int16_t)o1d_step(dt, rd, s.X, s.ctx, e, ovf + c.ovf_base, c.novf, bad);
     }
+    s.B = rd.pos;
     if (bad || (last && (i != c.hi || e != c.novf)) || (!last && s.B > stop)) *err = 1;
 }
 
-__global__ void k_ovf_dec(const uint8_t* vcl, const uint8_t* vman, uint64_t n, int16_t* ovf) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint64_t rp = 0, acc = 0;
-    int an = 0;
-    for (uint64_t i = 0; i < n; i++) {
-        const int vk = vcl[i];
-        while (an < vk) { acc = (acc << 8) | vman[rp++]; an += 8; }
-        an -= vk;
-        const uint64_t vv = (1ull << vk) | ((acc >> an) & ((1ull << vk) - 1));
-        const uint32_t zz = (uint32_t)(vv - 1);
-        ovf[i] = (int16_t)(int32_t)((zz >> 1) ^ (~(zz & 1) + 1));
-    }
-}
 
 // ---------------------------------------------------------------- method 0 decode
 __device__ inline uint64_t get_msb_field(const uint8_t* buf, uint64_t off, int k) {
@@ -938,6 +1239,22 @@ __device__ inline uint64_t get_msb_field(const uint8_t* buf, uint64_t off, int k
         v = (v << 1) | ((buf[b >> 3] >> (7 - (b & 7))) & 1u);
     }
     return v;
+}
+
+// Escaped codes from their classes and MSB-first mantissas (inverse of escape_streams), one per lane.
+__global__ void __launch_bounds__(256) k_ovf_values(const uint8_t* vcl, const uint64_t* voff, const uint8_t* vman,
+                                                    uint64_t vman_bits, uint64_t n, int16_t* ovf, int* err) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = vcl[i];
+    if (k > 16 || voff[i] + (uint64_t)k > vman_bits) {
+        *err = 1;
+        ovf[i] = 0;
+        return;
+    }
+    const uint64_t vv = (1ull << k) | get_msb_field(vman, voff[i], k);
+    const uint32_t zz = (uint32_t)(vv - 1);
+    ovf[i] = (int16_t)(int32_t)((zz >> 1) ^ (~(zz & 1) + 1));
 }
 __global__ void k_runs_decode(const uint8_t* rcl, const uint64_t* roff, const uint8_t* rman, uint64_t nr,
                               uint64_t* runlen) {
@@ -1044,12 +1361,10 @@ static size_t encode_method0(const int16_t* bins, uint64_t n, uint8_t* out, uint
     need_room(48, cap);
     uint64_t* pos = ws.take<uint64_t>(n + 1);
     uint64_t* nsel = ws.take<uint64_t>(1);
-    size_t bytes = 0;
-    hipcub::CountingInputIterator<uint64_t> it(0);
-    NonZero pred{bins};
-    FZ_HIP(hipcub::DeviceSelect::If(nullptr, bytes, it, pos, nsel, (int64_t)n, pred, st));
-    void* tmp = cub_scratch(ws, bytes);
-    FZ_HIP(hipcub::DeviceSelect::If(tmp, bytes, it, pos, nsel, (int64_t)n, pred, st));
+    {
+        const size_t bytes = select_scratch_bytes(n);
+        select_indices(NonZero{bins}, n, pos, nsel, cub_scratch(ws, bytes), bytes, "rle_select_values", st);
+    }
     uint64_t nv = 0;
     d2h(&nv, nsel, 1, st);
     uint64_t last_pos = 0;
@@ -1092,6 +1407,90 @@ static size_t encode_method0(const int16_t* bins, uint64_t n, uint8_t* out, uint
     return 48 + s_rcl + s_vcl + rbp + vbp;
 }
 
+// Escaped codes of one segment (RleFse:483-532): [u8 0][int16 x n] raw, or, when shorter,
+// [u8 1][u64 class bytes][u64 mantissa bytes][class bytes through fse_enc][mantissas], where a
+// value's class is k = floor(log2(zigzag + 1)) and its mantissa the k low bits of zigzag + 1, MSB
+// first. Classes, mantissa offsets (scan) and mantissa bits are computed one escape per lane.
+__global__ void __launch_bounds__(256) k_ovf_classes(const int16_t* ovf, uint64_t n, uint8_t* vcl, uint64_t* vbits) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t f = ovf[i];
+    const uint32_t zz = ((uint32_t)f << 1) ^ (uint32_t)(f >> 31);
+    const int k = floor_log2_64((uint64_t)zz + 1);
+    vcl[i] = (uint8_t)k;
+    vbits[i] = (uint64_t)k;
+}
+
+__global__ void __launch_bounds__(256) k_ovf_mantissas(const int16_t* ovf, uint64_t lo, uint64_t hi,
+                                                       const uint8_t* vcl, const uint64_t* voff, uint32_t* words) {
+    const uint64_t i = lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= hi) return;
+    const int32_t f = ovf[i];
+    const uint32_t zz = ((uint32_t)f << 1) ^ (uint32_t)(f >> 31);
+    put_msb_field(words, voff[i] - voff[lo], (uint64_t)zz + 1, vcl[i]);
+}
+
+static const uint8_t kZeroByte = 0;
+
+static void escape_streams(const int16_t* ovf, const std::vector<uint64_t>& off, uint64_t S,
+                           std::vector<uint64_t>& len, std::vector<uint8_t*>& block, DevArena& ws, hipStream_t st) {
+    const uint64_t n = off[S];
+    uint8_t* vcl = ws.take<uint8_t>(n + 1);
+    uint64_t* vbits = ws.take<uint64_t>(n + 1);
+    uint64_t* voff = ws.take<uint64_t>(n + 1);
+    if (n) {
+        FZ_LAUNCH("o1_escape_classes", n * 3, k_ovf_classes, dim3(nblk(n)), dim3(256), 0, st, ovf, n, vcl, vbits);
+        FZ_HIP(hipGetLastError());
+        exclusive_scan_u64(vbits, voff, n, ws, st);
+    }
+    // segment bit boundaries of the mantissa stream
+    std::vector<uint64_t> hb(S + 1, 0);
+    for (uint64_t g = 0; g <= S; g++)
+        if (off[g] < n) d2h(&hb[g], voff + off[g], 1, st);
+    if (n) {
+        uint64_t last = 0, lk = 0;
+        d2h(&last, voff + n - 1, 1, st);
+        d2h(&lk, vbits + n - 1, 1, st);
+        hb[S] = last + lk;
+        for (uint64_t g = 0; g < S; g++)
+            if (off[g] >= n) hb[g] = hb[S];
+    }
+    for (uint64_t g = 0; g < S; g++) {
+        const uint64_t lo = off[g], hi = off[g + 1], m = hi - lo;
+        const uint64_t raw = 2 * m;
+        // framed class bytes go right after the 17-byte head; the block is kept whole in the arena
+        const uint64_t fse_cap = m + 5 * ((m + kFblk - 1) / kFblk + 1) + 64;
+        const uint64_t vbp = (hb[g + 1] - hb[g] + 7) / 8;
+        uint8_t* b = ws.take<uint8_t>(std::max<uint64_t>(1 + raw, 17 + fse_cap + vbp) + 8);
+        block[g] = b;
+        if (m == 0) {
+            FZ_HIP(hipMemcpyAsync(b, &kZeroByte, 1, hipMemcpyHostToDevice, st));
+            len[g] = 1;
+            continue;
+        }
+        const uint64_t svcl = fse_enc_blocks(vcl + lo, m, b + 17, fse_cap, ws, st);
+        if (svcl + vbp + 16 >= raw) {
+            FZ_HIP(hipMemcpyAsync(b, &kZeroByte, 1, hipMemcpyHostToDevice, st));
+            FZ_HIP(hipMemcpyAsync(b + 1, ovf + lo, raw, hipMemcpyDeviceToDevice, st));
+            len[g] = 1 + raw;
+            continue;
+        }
+        uint32_t* man = ws.take<uint32_t>(vbp / 4 + 2);
+        FZ_HIP(hipMemsetAsync(man, 0, (vbp / 4 + 2) * 4, st));
+        FZ_LAUNCH("o1_escape_mantissas", m * 2 + vbp, k_ovf_mantissas, dim3(nblk(m)), dim3(256), 0, st, ovf, lo, hi,
+                  vcl, voff, man);
+        FZ_HIP(hipGetLastError());
+        uint8_t head[17];
+        head[0] = 1;
+        memcpy(head + 1, &svcl, 8);
+        memcpy(head + 9, &vbp, 8);
+        FZ_HIP(hipMemcpyAsync(b, head, 17, hipMemcpyHostToDevice, st));
+        FZ_HIP(hipMemcpyAsync(b + 17 + svcl, man, vbp, hipMemcpyDeviceToDevice, st));
+        sync(st);  // head lives on this stack frame
+        len[g] = 17 + svcl + vbp;
+    }
+}
+
 static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint64_t cap, DevArena& ws,
                              hipStream_t st) {
     uint64_t S = n / kO1Seg;
@@ -1102,12 +1501,8 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
     uint64_t* epos = ws.take<uint64_t>(n + 1);
     uint64_t* nsel = ws.take<uint64_t>(1);
     {
-        size_t bytes = 0;
-        hipcub::CountingInputIterator<uint64_t> it(0);
-        IsO1Escape pred{bins};
-        FZ_HIP(hipcub::DeviceSelect::If(nullptr, bytes, it, epos, nsel, (int64_t)n, pred, st));
-        void* tmp = cub_scratch(ws, bytes);
-        FZ_HIP(hipcub::DeviceSelect::If(tmp, bytes, it, epos, nsel, (int64_t)n, pred, st));
+        const size_t bytes = select_scratch_bytes(n);
+        select_indices(IsO1Escape{bins}, n, epos, nsel, cub_scratch(ws, bytes), bytes, "o1_select_escapes", st);
     }
     uint64_t nesc = 0;
     d2h(&nesc, nsel, 1, st);
@@ -1119,7 +1514,6 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
     uint32_t* cnt = ws.take<uint32_t>(S * kO1Ctx * kO1Sym);
     uint16_t* nf = ws.take<uint16_t>(S * kO1Ctx * kO1Sym);
     CTable* ct = ws.take<CTable>(S * kO1Ctx);
-    uint8_t* spread = ws.take<uint8_t>(S * kO1Ctx * (1u << kO1Log));
     FZ_HIP(hipMemsetAsync(cnt, 0, S * kO1Ctx * kO1Sym * 4, st));
     std::vector<uint64_t> chains;  // {seg lo, chunk lo, chunk hi, segment}
     std::vector<uint64_t> novf(S * kO1K), ovf_off(S + 1);
@@ -1141,9 +1535,6 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
             FZ_LAUNCH("o1_hist", 0, k_add_one, dim3(nblk(kO1Ctx * kO1Sym)), dim3(256), 0, st, c, kO1Ctx * kO1Sym);
         }
         FZ_HIP(hipGetLastError());
-        FZ_LAUNCH("o1_tables", 0, k_o1_tables, dim3(kO1Ctx), dim3(64), 0, st, c, nf + g * kO1Ctx * kO1Sym,
-                           ct + g * kO1Ctx, spread + g * kO1Ctx * (1u << kO1Log));
-        FZ_HIP(hipGetLastError());
         ovf_off[g] = e;
         for (int k = 0; k < kO1K; k++) {
             const uint64_t clo = lo + k * cs, chi = std::min(clo + cs, hi);
@@ -1154,6 +1545,8 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
         }
     }
     ovf_off[S] = e;
+    FZ_LAUNCH("o1_tables", 0, k_o1_tables, dim3(kO1Ctx, (unsigned)S), dim3(256), 0, st, cnt, nf, ct);
+    FZ_HIP(hipGetLastError());
     const int nch = (int)(S * kO1K);
     uint64_t* dchains = ws.take<uint64_t>(chains.size());
     h2d(dchains, chains.data(), chains.size(), st);
@@ -1235,22 +1628,12 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
                            words, stride_words, blen);
     }
     FZ_HIP(hipGetLastError());
-    // escape side streams
-    uint64_t maxo = 0;
-    for (uint64_t g = 0; g < S; g++) maxo = std::max(maxo, ovf_off[g + 1] - ovf_off[g]);
-    const uint64_t ostride = 2 * maxo + 17 + 2 * ((maxo + kFblk - 1) / kFblk + 1) * (kFblk + 1024) + 256;
-    uint8_t* ovcl = ws.take<uint8_t>(S * ostride);
-    uint8_t* ovman = ws.take<uint8_t>(S * ostride);
-    uint8_t* otmp = ws.take<uint8_t>(S * ostride);
-    uint8_t* oout = ws.take<uint8_t>(S * ostride);
-    uint64_t* olen = ws.take<uint64_t>(S);
-    uint64_t* dovf_off = ws.take<uint64_t>(S + 1);
-    h2d(dovf_off, ovf_off.data(), S + 1, st);
-    FZ_LAUNCH("o1_escape_stream", 0, k_o1_ovf, dim3((unsigned)S), dim3(256), 0, st, ovf, dovf_off, ovcl, ovman, otmp, oout, ostride, olen);
-    FZ_HIP(hipGetLastError());
-    std::vector<uint64_t> hblen(nch), holen(S);
+    // escape side streams, one block per segment
+    std::vector<uint64_t> holen(S);
+    std::vector<uint8_t*> oblock(S);
+    escape_streams(ovf, ovf_off, S, holen, oblock, ws, st);
+    std::vector<uint64_t> hblen(nch);
     d2h(hblen.data(), blen, nch, st);
-    d2h(holen.data(), olen, S, st);
     // assemble: [u64 S] then per segment [n][K][blen x4][novf x4][nf][streams][ovf block]
     uint64_t total = 8;
     for (uint64_t g = 0; g < S; g++) {
@@ -1281,7 +1664,7 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
             FZ_HIP(hipMemcpyAsync(p, bufs + q * bstride, hblen[q], hipMemcpyDeviceToDevice, st));
             p += hblen[q];
         }
-        FZ_HIP(hipMemcpyAsync(p, oout + g * ostride, holen[g], hipMemcpyDeviceToDevice, st));
+        FZ_HIP(hipMemcpyAsync(p, oblock[g], holen[g], hipMemcpyDeviceToDevice, st));
         p += holen[g];
     }
     sync(st);
@@ -1395,40 +1778,49 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
     if (hc.empty()) return;
     const int nc = (int)hc.size();
     O1DecChain* dch = ws.take<O1DecChain>(nc);
-    O1DecState* start = ws.take<O1DecState>(nsub);
-    O1DecState* exit_st = ws.take<O1DecState>(nsub);
-    uint64_t* e_start = ws.take<uint64_t>(nsub);
-    uint64_t* kesc = ws.take<uint64_t>(nsub);
-    uint64_t* cnt = ws.take<uint64_t>(nsub);
-    uint64_t* kscan = ws.take<uint64_t>(nsub);
-    uint64_t* cscan = ws.take<uint64_t>(nsub);
-    uint8_t* used = ws.take<uint8_t>(nsub);
+    O1StartSet* start = ws.take<O1StartSet>(nsub);
+    O1Graph* graph = ws.take<O1Graph>(nsub);
+    O1DecState* res_start = ws.take<O1DecState>(nsub);
+    uint64_t* res_e = ws.take<uint64_t>(nsub);
+    uint64_t* res_off = ws.take<uint64_t>(nsub);
     uint8_t* run = ws.take<uint8_t>(nsub);
     unsigned int* nrun = ws.take<unsigned int>(1);
-    FZ_HIP(hipMemsetAsync(kesc, 0, nsub * 8, st));
-    FZ_HIP(hipMemsetAsync(cnt, 0, nsub * 8, st));
-    FZ_HIP(hipMemsetAsync(used, 0, nsub, st));
+    unsigned int* stats = ws.take<unsigned int>(6);
+    FZ_HIP(hipMemsetAsync(stats, 0, 6 * sizeof(unsigned int), st));
     h2d(dch, hc.data(), hc.size(), st);
-    FZ_LAUNCH("o1_decode_init", 0, k_o1d_init, dim3(nblk(nsub)), dim3(256), 0, st, dch, nc, nsub, start, e_start, run);
+    uint64_t rounds = 0;
+    FZ_LAUNCH("o1_decode_init", 0, k_o1d_init, dim3(nblk(nsub)), dim3(256), 0, st, dch, nc, nsub, start, run);
     FZ_HIP(hipGetLastError());
-    // every round settles at least the next subsequence of every chain
+    // start sets settle like the states of a single decoder: every round settles at least the next
+    // subsequence of every chain
     for (uint64_t round = 0;; round++) {
         if (round > max_sub + 1) throw std::runtime_error("tANS subsequence decoding did not settle");
-        FZ_LAUNCH("o1_decode_round", 0, k_o1d_round, dim3(nblk(nsub)), dim3(256), 0, st, dev, dch, nc, nsub, dt, ovf, run, start,
-                           e_start, exit_st, kesc, cnt, used);
+        FZ_LAUNCH("o1_decode_graph", 0, k_o1d_graph, dim3(nblk(nsub, kGLanes)), dim3(kGLanes), 0, st, dev, dch, nc, nsub,
+                  dt, run, start, graph);
         FZ_HIP(hipGetLastError());
-        exclusive_scan_u64(kesc, kscan, nsub, ws, st);
         FZ_HIP(hipMemsetAsync(nrun, 0, sizeof(unsigned int), st));
-        FZ_LAUNCH("o1_decode_link", 0, k_o1d_link, dim3(nblk(nsub)), dim3(256), 0, st, dch, nc, nsub, exit_st, kscan, used, start,
-                           e_start, run, nrun);
+        FZ_LAUNCH("o1_decode_link", 0, k_o1d_glink, dim3(nblk(nsub)), dim3(256), 0, st, dch, nc, nsub, graph, start,
+                  run, nrun);
         FZ_HIP(hipGetLastError());
         unsigned int h = 0;
         d2h(&h, nrun, 1, st);
+        rounds = round + 1;
         if (!h) break;
     }
-    exclusive_scan_u64(cnt, cscan, nsub, ws, st);
-    FZ_LAUNCH("o1_decode_out", stream_bytes + ncodes * 2, k_o1d_out, dim3(nblk(nsub)), dim3(256), 0, st, dev, dch, nc, nsub, dt, ovf, start, e_start,
-                       cscan, out, err);
+    FZ_LAUNCH("o1_decode_walk", 0, k_o1d_walk, dim3(nc), dim3(256), 0, st, dev, dch, nc, dt, ovf, start, graph,
+              res_start, res_e, res_off, stats, err);
+    FZ_HIP(hipGetLastError());
+    FZ_LAUNCH("o1_decode_out", stream_bytes + ncodes * 2, k_o1d_out, dim3(nblk(nsub)), dim3(256), 0, st, dev, dch, nc,
+              nsub, dt, ovf, res_start, res_e, res_off, out, err);
+    if (getenv("FALCONS_DECODE_STATS")) {
+        unsigned int hs[6] = {0, 0, 0, 0, 0, 0};
+        d2h(hs, stats, 6, st);
+        fprintf(stderr, "[o1 decode] %d chains, %llu subsequences, %llu rounds, walk decoded %u subsequences "
+                        "exactly (%u graphs over capacity [exits %u, nodes %u, branches %u, open edges %u], %u start "
+                        "states outside the set)\n",
+                nc, (unsigned long long)nsub, (unsigned long long)rounds, hs[0] + hs[1], hs[0], hs[2], hs[3], hs[4],
+                hs[5], hs[1]);
+    }
     FZ_HIP(hipGetLastError());
 }
 
@@ -1453,7 +1845,7 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
         std::vector<uint64_t> chains;
         std::vector<uint64_t> nf_off(S), ovf_sizes(S), ovf_base(S);
         uint64_t total_ovf = 0;
-        struct OvfPiece { uint64_t n, vcl_off, vman_off, nraw_off, mode; };
+        struct OvfPiece { uint64_t n, vcl_off, vman_off, nraw_off, mode, vbp; };
         std::vector<OvfPiece> pieces(S);
         for (uint64_t g = 0; g < S; g++) {
             const uint64_t lo = g * seg, hi = std::min(lo + seg, n), sn = hi - lo;
@@ -1480,7 +1872,7 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
                 hp += blen[k];
                 ovfo += nvk[k];
             }
-            OvfPiece pc{ovfo, 0, 0, 0, 0};
+            OvfPiece pc{ovfo, 0, 0, 0, 0, 0};
             need_in(hp, 1);
             pc.mode = *hp++;
             if (ovfo) {
@@ -1495,6 +1887,7 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
                     need_in(hp, svcl);
                     hp += svcl;
                     pc.vman_off = (uint64_t)(hp - host);
+                    pc.vbp = vbp;
                     need_in(hp, vbp);
                     if (vbp * 8 < ovfo) throw std::invalid_argument("corrupt o1 tANS escape stream");
                     hp += vbp;
@@ -1513,16 +1906,24 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
             } else {
                 uint8_t* vcl = ws.take<uint8_t>(pc.n);
                 fse_dec_blocks(host + pc.vcl_off, dev + pc.vcl_off, pc.n, vcl, err, ws, st, nullptr);
-                FZ_LAUNCH("o1_escape_decode", 0, k_ovf_dec, dim3(1), dim3(1), 0, st, vcl, dev + pc.vman_off, pc.n, ovf + ovf_base[g]);
+                uint64_t* voff = ws.take<uint64_t>(pc.n + 1);
+                {
+                    hipcub::TransformInputIterator<uint64_t, ToU64, const uint8_t*> it(vcl, ToU64{});
+                    size_t b = 0;
+                    FZ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, it, voff, (int64_t)pc.n, st));
+                    void* t = cub_scratch(ws, b);
+                    FZ_HIP(hipcub::DeviceScan::ExclusiveSum(t, b, it, voff, (int64_t)pc.n, st));
+                }
+                FZ_LAUNCH("o1_escape_decode", pc.n * 3, k_ovf_values, dim3(nblk(pc.n)), dim3(256), 0, st, vcl, voff,
+                          dev + pc.vman_off, pc.vbp * 8, pc.n, ovf + ovf_base[g], err);
                 FZ_HIP(hipGetLastError());
             }
         }
         DEntry* dt = ws.take<DEntry>(S * kO1Ctx * (1u << kO1Log));
-        uint8_t* spread = ws.take<uint8_t>(S * kO1Ctx * (1u << kO1Log));
-        for (uint64_t g = 0; g < S; g++) {
-            FZ_LAUNCH("o1_dtables", 0, k_o1_dtables, dim3(kO1Ctx), dim3(64), 0, st, dev + nf_off[g],
-                               dt + g * kO1Ctx * (1u << kO1Log), spread + g * kO1Ctx * (1u << kO1Log), err);
-        }
+        uint64_t* dnf_off = ws.take<uint64_t>(S);
+        h2d(dnf_off, nf_off.data(), S, st);
+        FZ_LAUNCH("o1_dtables", 0, k_o1_dtables, dim3(kO1Ctx, (unsigned)S), dim3(256), 0, st, dev, dnf_off, dt, err);
+        FZ_HIP(hipGetLastError());
         {  // the tables must be whole before any decoder reads them
             int herr = 0;
             d2h(&herr, err, 1, st);

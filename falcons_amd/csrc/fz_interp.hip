@@ -12,8 +12,11 @@
 // fuses it (explicit fma).
 #include <hipcub/hipcub.hpp>
 
+#include <type_traits>
+
 #include "fz_common.h"
 #include "fz_kernels.h"
+#include "fz_select.h"
 
 namespace fz {
 
@@ -235,8 +238,24 @@ __global__ void k_level_eb(const double* eb_dev, const double* ratio, int mode, 
 template <typename T>
 __global__ void __launch_bounds__(256) k_minmax(const T* __restrict__ d, uint64_t n, T* part) {
     __shared__ T smin[256], smax[256];
+    // every partial starts from d[0] and takes an element only when it compares below / above, so the
+    // result does not depend on the order (a NaN enters only through d[0], as in the serial scan)
     T mn = d[0], mx = d[0];
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    constexpr int V = 16 / sizeof(T);
+    using Vec = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    const uint64_t nv = ((uintptr_t)d & 15) == 0 ? n / V : 0;  // vector loads only on aligned fields
+    const Vec* dv = reinterpret_cast<const Vec*>(d);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
+        const Vec w = dv[i];
+        const T* e = reinterpret_cast<const T*>(&w);
+#pragma unroll
+        for (int k = 0; k < V; k++) {
+            mn = e[k] < mn ? e[k] : mn;
+            mx = e[k] > mx ? e[k] : mx;
+        }
+    }
+    for (uint64_t i = nv * V + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const T v = d[i];
         mn = v < mn ? v : mn;
         mx = v > mx ? v : mx;
@@ -259,13 +278,27 @@ __global__ void __launch_bounds__(256) k_minmax(const T* __restrict__ d, uint64_
 
 // calAbsErrorBound (Statistic.hpp:216-240) once the partial min/max are in.
 template <typename T>
-__global__ void k_abs_eb(const T* part, int nparts, int mode, double abs_in, double rel, double psnr_factor,
-                         double* eb_out) {
+__global__ void __launch_bounds__(256) k_abs_eb(const T* part, int nparts, int mode, double abs_in, double rel,
+                                                double psnr_factor, double* eb_out) {
+    __shared__ T smin[256], smax[256];
     T mn = part[0], mx = part[1];
-    for (int b = 1; b < nparts; b++) {
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         mn = part[2 * b] < mn ? part[2 * b] : mn;
         mx = part[2 * b + 1] > mx ? part[2 * b + 1] : mx;
     }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            smin[threadIdx.x] = smin[threadIdx.x + w] < smin[threadIdx.x] ? smin[threadIdx.x + w] : smin[threadIdx.x];
+            smax[threadIdx.x] = smax[threadIdx.x + w] > smax[threadIdx.x] ? smax[threadIdx.x + w] : smax[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    mn = smin[0];
+    mx = smax[0];
     const double range = (double)(T)(mx - mn);
     double eb = abs_in;
     switch (mode) {
@@ -280,8 +313,9 @@ __global__ void k_abs_eb(const T* part, int nparts, int mode, double abs_in, dou
 
 template <typename T>
 __global__ void k_gather_escapes(const uint64_t* idx, const uint64_t* nsel, const T* escval, T* out) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < *nsel) out[i] = escval[idx[i]];
+    const uint64_t m = *nsel;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = escval[idx[i]];
 }
 
 struct IsEscape {
@@ -381,47 +415,38 @@ void level_constants(const double* eb_dev, const double* ratio_dev, int mode, in
     FZ_HIP(hipGetLastError());
 }
 
-size_t minmax_scratch_bytes() { return 2 * 1024 * sizeof(double); }
+size_t minmax_scratch_bytes() { return 2 * 4096 * sizeof(double); }
 
 void abs_error_bound(int dtype, const void* data, uint64_t n, int mode, double abs_in, double rel, double psnr_factor,
                      void* scratch, double* eb_dev, hipStream_t st) {
-    const int nb = (int)std::min<uint64_t>(1024, (n + 255) / 256);
+    const int nb = (int)std::min<uint64_t>(4096, (n + 1023) / 1024);
     if (dtype == kF32) {
         FZ_LAUNCH("minmax", n * 4, k_minmax<float>, dim3(nb), dim3(256), 0, st, (const float*)data, n, (float*)scratch);
-        FZ_LAUNCH("abs_eb", 0, k_abs_eb<float>, dim3(1), dim3(1), 0, st, (const float*)scratch, nb, mode, abs_in, rel,
+        FZ_LAUNCH("abs_eb", 0, k_abs_eb<float>, dim3(1), dim3(256), 0, st, (const float*)scratch, nb, mode, abs_in, rel,
                            psnr_factor, eb_dev);
     } else {
         FZ_LAUNCH("minmax", n * 8, k_minmax<double>, dim3(nb), dim3(256), 0, st, (const double*)data, n, (double*)scratch);
-        FZ_LAUNCH("abs_eb", 0, k_abs_eb<double>, dim3(1), dim3(1), 0, st, (const double*)scratch, nb, mode, abs_in, rel,
+        FZ_LAUNCH("abs_eb", 0, k_abs_eb<double>, dim3(1), dim3(256), 0, st, (const double*)scratch, nb, mode, abs_in, rel,
                            psnr_factor, eb_dev);
     }
     FZ_HIP(hipGetLastError());
 }
 
-size_t select_escapes_scratch(uint64_t n) {
-    size_t bytes = 0;
-    hipcub::CountingInputIterator<uint64_t> it(0);
-    IsEscape pred{nullptr};
-    (void)hipcub::DeviceSelect::If(nullptr, bytes, it, (uint64_t*)nullptr, (uint64_t*)nullptr, (int64_t)n, pred);
-    return bytes;
-}
+size_t select_escapes_scratch(uint64_t n) { return select_scratch_bytes(n); }
 
 void select_escapes(const int16_t* codes, uint64_t n, uint64_t* out_idx, uint64_t* nsel_dev, void* scratch,
                     size_t scratch_bytes, hipStream_t st) {
-    hipcub::CountingInputIterator<uint64_t> it(0);
-    IsEscape pred{codes};
-    ProfScope ps("select_escapes", n * 2, st);
-    FZ_HIP(hipcub::DeviceSelect::If(scratch, scratch_bytes, it, out_idx, nsel_dev, (int64_t)n, pred, st));
+    select_indices(IsEscape{codes}, n, out_idx, nsel_dev, scratch, scratch_bytes, "select_escapes", st);
 }
 
 void gather_escapes(int dtype, const uint64_t* idx, const uint64_t* nsel_dev, uint64_t max_n, const void* escval,
                     void* out, hipStream_t st) {
     if (!max_n) return;
     if (dtype == kF32)
-        FZ_LAUNCH("gather_escapes", 0, k_gather_escapes<float>, dim3(blocks_for(max_n)), dim3(256), 0, st, idx, nsel_dev,
+        FZ_LAUNCH("gather_escapes", 0, k_gather_escapes<float>, dim3(std::min(blocks_for(max_n), 1024u)), dim3(256), 0, st, idx, nsel_dev,
                            (const float*)escval, (float*)out);
     else
-        FZ_LAUNCH("gather_escapes", 0, k_gather_escapes<double>, dim3(blocks_for(max_n)), dim3(256), 0, st, idx, nsel_dev,
+        FZ_LAUNCH("gather_escapes", 0, k_gather_escapes<double>, dim3(std::min(blocks_for(max_n), 1024u)), dim3(256), 0, st, idx, nsel_dev,
                            (const double*)escval, (double*)out);
     FZ_HIP(hipGetLastError());
 }
