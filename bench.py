@@ -37,6 +37,14 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timing and full-size byte check")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end host API timing")
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--workload", choices=("field", "batched", "serving"), default="field",
+                    help="field: one 512^3 field (the default line); batched: the 4096-chunk mix through the "
+                         "batched entry points against plain per-call calls; serving: Poisson request mix")
+    ap.add_argument("--chunks", type=int, default=4096, help="batched: number of chunks")
+    ap.add_argument("--workers", type=int, default=8, help="batched/serving: worker sub-contexts (slots)")
+    ap.add_argument("--rate", type=float, default=200.0, help="serving: requests per second (Poisson)")
+    ap.add_argument("--requests", type=int, default=2000, help="serving: requests measured")
+    ap.add_argument("--warm-requests", type=int, default=200, help="serving: requests before the measurement")
     return ap.parse_args()
 
 
@@ -53,6 +61,10 @@ def main():
     import falcons_amd as fa
     from falcons_amd import synth
 
+    if a.workload == "batched":
+        return bench_batched(a, fa, synth)
+    if a.workload == "serving":
+        return bench_serving(a, fa, synth)
     shape = (a.side,) * 3
     t = time.time()
     field = synth.power_law(shape, slope=-3.0, seed=a.seed, dtype=np.float32)
@@ -240,6 +252,247 @@ def main():
     for b in (d_orig, d_work, d_out, d_pre):
         b.free()
     ctx.close()
+
+
+class _View:
+    """A sub-range of a device allocation, usable where the Context methods take a DeviceBuffer."""
+
+    def __init__(self, base, off, nbytes):
+        self.ptr, self.nbytes = base.ptr.value + off, nbytes
+        self.ctx = base.ctx
+
+    def download(self, nbytes=None, dtype=np.uint8):
+        n = self.nbytes if nbytes is None else nbytes
+        out = np.empty(n // np.dtype(dtype).itemsize, dtype=dtype)
+        import falcons_amd as fa
+        rc = fa._L.fz_memcpy(self.ctx.h, out.ctypes.data, self.ptr, n, 1)
+        if rc:
+            raise RuntimeError(self.ctx._err())
+        return out
+
+
+def gen_chunks(synth, count, seed=4):
+    """The 4096-chunk mix: per-axis sides floor(exp(U(ln 32, ln 128))), chunk i a k^-3 field of seed
+    seed + i, REL bound drawn uniformly from {1e-2, 1e-3, 1e-4}."""
+    from concurrent.futures import ThreadPoolExecutor
+    sides = synth.chunk_sides(count, seed=seed)
+    rels = np.random.default_rng(seed + 7919).choice([1e-2, 1e-3, 1e-4], size=count)
+    workers = min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(workers) as ex:
+        fields = list(ex.map(lambda i: synth.power_law(tuple(int(v) for v in sides[i]), -3.0, seed=seed + i),
+                             range(count)))
+    return fields, rels
+
+
+def bench_batched(a, fa, synth):
+    """The chunk mix compressed and decompressed twice on device-resident buffers: one plain per-call
+    fz_compress_device / fz_decompress_device per chunk on one context, then the batched entry points.
+    Both produce identical pre-zstd buffers and fields (checked)."""
+    t = time.time()
+    fields, rels = gen_chunks(synth, a.chunks)
+    sizes = np.array([f.nbytes for f in fields], dtype=np.int64)
+    total = int(sizes.sum())
+    log(f"{a.chunks} chunks, {total / 1e9:.2f} GB, generated in {time.time() - t:.1f} s")
+    ctx = fa.Context(0)
+    ctx.set_workers(a.workers)
+    cfgs = []
+    for f, r in zip(fields, rels):
+        c = fa.falconsConfig(*f.shape)
+        c.cmprAlgo = fa.falconsAlgorithm.INTERP
+        c.errorBoundMode = fa.falconsErrorBoundMode.REL
+        c.relErrorBound = float(r)
+        cfgs.append(c)
+    bounds = np.array([ctx.prezstd_bound(c, np.float32) for c in cfgs], dtype=np.int64)
+    align = lambda v: (v + 255) // 256 * 256
+    foff = np.concatenate([[0], np.cumsum([align(int(v)) for v in sizes])])
+    poff = np.concatenate([[0], np.cumsum([align(int(v)) for v in bounds])])
+    d_orig, d_work, d_out = ctx.alloc(int(foff[-1])), ctx.alloc(int(foff[-1])), ctx.alloc(int(foff[-1]))
+    d_pre = ctx.alloc(int(poff[-1]))
+    host = np.zeros(int(foff[-1]), dtype=np.uint8)
+    for i, f in enumerate(fields):
+        host[foff[i]:foff[i] + sizes[i]] = f.reshape(-1).view(np.uint8)
+    d_orig.upload(host)
+    del host
+    work = [_View(d_work, int(foff[i]), int(sizes[i])) for i in range(a.chunks)]
+    outs = [_View(d_out, int(foff[i]), int(sizes[i])) for i in range(a.chunks)]
+    pres = [_View(d_pre, int(poff[i]), int(bounds[i])) for i in range(a.chunks)]
+
+    def restore():
+        fa._L.fz_memcpy(ctx.h, d_work.ptr, d_orig.ptr, int(foff[-1]), 2)
+
+    # plain per-call path
+    restore()
+    ctx.synchronize()
+    run_cfgs = [c.copy() for c in cfgs]
+    t0 = time.perf_counter()
+    lens = [ctx.compress_device(run_cfgs[i], np.float32, work[i], pres[i])[0] for i in range(a.chunks)]
+    t_pc_c = time.perf_counter() - t0
+    h_pres = [pres[i].download(lens[i]) for i in range(a.chunks)]
+    t0 = time.perf_counter()
+    for i in range(a.chunks):
+        ctx.decompress_device(run_cfgs[i], np.float32, h_pres[i], pres[i], outs[i])
+    t_pc_d = time.perf_counter() - t0
+    ref_out = d_out.download(int(foff[-1]))
+    log(f"per-call: compress {t_pc_c:.2f} s, decompress {t_pc_d:.2f} s")
+
+    # batched
+    restore()
+    fa._L.fz_memcpy(ctx.h, d_out.ptr, d_orig.ptr, int(foff[-1]), 2)  # clear the outputs of the first run
+    ctx.synchronize()
+    b_cfgs = [c.copy() for c in cfgs]
+    t0 = time.perf_counter()
+    blens, status, _ = ctx.compress_batch([(b_cfgs[i], np.float32, work[i], pres[i]) for i in range(a.chunks)])
+    t_b_c = time.perf_counter() - t0
+    same_pre = blens == lens and all(np.array_equal(pres[i].download(lens[i]), h_pres[i]) for i in range(a.chunks))
+    t0 = time.perf_counter()
+    ctx.decompress_batch([(b_cfgs[i], np.float32, h_pres[i], pres[i], outs[i]) for i in range(a.chunks)])
+    t_b_d = time.perf_counter() - t0
+    same_out = bool(np.array_equal(d_out.download(int(foff[-1])), ref_out))
+    log(f"batched ({a.workers} workers): compress {t_b_c:.2f} s, decompress {t_b_d:.2f} s; "
+        f"identical pre-zstd {same_pre}, identical fields {same_out}")
+    if not (same_pre and same_out):
+        raise SystemExit("batched output differs from the per-call output")
+    value = 2.0 * total / (t_b_c + t_b_d) / 1e9
+    per_call = 2.0 * total / (t_pc_c + t_pc_d) / 1e9
+    line = {
+        "metric": "compress/decompress GB/s on 1 MI355X, byte-identical; batched p50/p99 latency",
+        "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round((t_b_c + t_b_d) * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{a.chunks} float32 chunks, sides floor(exp(U(ln32, ln128))) per axis, k^-3 fields "
+                               "(seed 4+i), REL from {1e-2, 1e-3, 1e-4}, ALGO_INTERP, device-resident compress + "
+                               "decompress through the batched entry points",
+                   "total_bytes": total, "workers": a.workers},
+        "batched": {"compress_GBps": round(total / t_b_c / 1e9, 3), "decompress_GBps": round(total / t_b_d / 1e9, 3)},
+        "per_call": {"value": round(per_call, 3), "compress_GBps": round(total / t_pc_c / 1e9, 3),
+                     "decompress_GBps": round(total / t_pc_d / 1e9, 3)},
+        "speedup_vs_per_call": round(value / per_call, 3),
+        "parity": {"prezstd_identical": same_pre, "fields_identical": same_out},
+    }
+    print(json.dumps(line), flush=True)
+    for b in (d_orig, d_work, d_out, d_pre):
+        b.free()
+    ctx.close()
+
+
+def serving_mix(synth, count, seed=5):
+    """Request mix: 60% compress / 40% decompress; 70% 3D float32 side in [64, 512], 20% 2D float32 side
+    in [1024, 8192], 10% 3D float64 side in [128, 256] (sides log-uniform); ABS or REL bound
+    log-uniform in [1e-5, 1e-2]. Fields are crops of one k^-3 field per class at random offsets."""
+    rng = np.random.default_rng(seed)
+    return [{
+        "op": 0 if rng.random() < 0.6 else 1,
+        "cls": int(rng.choice(3, p=[0.7, 0.2, 0.1])),
+        "u": rng.random(3),
+        "mode": int(rng.integers(0, 2)),
+        "eb": float(10 ** rng.uniform(-5, -2)),
+    } for _ in range(count)]
+
+
+def bench_serving(a, fa, synth):
+    """Poisson arrivals at --rate through fz_server; latency = completion - arrival per request."""
+    t = time.time()
+    base = {0: synth.power_law((512, 512, 512), -3.0, seed=5, dtype=np.float32),
+            1: synth.power_law((8192, 8192), -3.0, seed=6, dtype=np.float32),
+            2: synth.power_law((256, 256, 256), -3.0, seed=7, dtype=np.float64)}
+    log(f"base fields generated in {time.time() - t:.1f} s")
+    lims = {0: (64, 512, 3), 1: (1024, 8192, 2), 2: (128, 256, 3)}
+    reqs = serving_mix(synth, a.warm_requests + a.requests)
+
+    def field_of(r):
+        lo, hi, nd = lims[r["cls"]]
+        side = int(np.floor(np.exp(np.log(lo) + r["u"][0] * (np.log(hi) - np.log(lo)))))
+        b = base[r["cls"]]
+        off = [int(r["u"][1 + (k % 2)] * (b.shape[k] - side)) for k in range(nd)]
+        return np.ascontiguousarray(b[tuple(slice(o, o + side) for o in off)])
+
+    def config_of(r, f):
+        c = fa.falconsConfig(*f.shape)
+        c.cmprAlgo = fa.falconsAlgorithm.INTERP
+        c.errorBoundMode = r["mode"]
+        if r["mode"] == 1:
+            c.relErrorBound = r["eb"]
+        else:
+            c.absErrorBound = r["eb"]
+        return c
+
+    # decompress requests need streams: compress a pool of inputs of the same mix up front
+    pool_ctx = fa.Context(0)
+    t = time.time()
+    streams = {}
+    for k, r in enumerate(reqs):
+        if r["op"] == 1 and len(streams) < 48:
+            f = field_of(r)
+            s, _, _ = pool_ctx.compress_ex(f.copy(), config_of(r, f), copy_back=False)
+            streams[k] = (np.asarray(s).copy(), f.dtype, f.size)
+    pool = list(streams.values())
+    pool_ctx.close()
+    log(f"{len(pool)} streams for decompress requests prepared in {time.time() - t:.1f} s")
+
+    srv = fa.Server(0, slots=a.workers)
+    rng = np.random.default_rng(11)
+    gaps = rng.exponential(1.0 / a.rate, size=len(reqs))
+    done = []  # (tag, submit, finish, field bytes) of the measured requests
+
+    def take(res):
+        if res["status"] != 0:
+            raise SystemExit(f"request {res['tag']} failed: {res['msg']}")
+        if res["tag"] >= a.warm_requests:
+            done.append((res["tag"], res["t_submit_ms"], res["t_done_ms"], res["field_bytes"]))
+
+    t_start = time.perf_counter()
+    due = t_start
+    pending = 0
+    for k, r in enumerate(reqs):
+        due += gaps[k]
+        if r["op"] == 0:
+            f = field_of(r)
+            payload = ("c", f, config_of(r, f))
+        else:
+            payload = ("d",) + pool[k % len(pool)]
+        while True:  # collect what finished while waiting for the arrival time
+            now = time.perf_counter()
+            res = srv.poll(timeout_ms=0) if now < due else None
+            if res is None:
+                if now >= due:
+                    break
+                time.sleep(min(due - now, 0.0005))
+                continue
+            pending -= 1
+            take(res)
+        if payload[0] == "c":
+            srv.submit_compress(k, payload[1], payload[2])
+        else:
+            srv.submit_decompress(k, payload[1], payload[2], payload[3])
+        pending += 1
+    while pending:
+        res = srv.poll(timeout_ms=600000)
+        if res is None:
+            raise SystemExit("server stalled")
+        pending -= 1
+        take(res)
+    st = srv.stats()
+    srv.close()
+    lat = np.array([d[2] - d[1] for d in done])
+    span_s = (max(d[2] for d in done) - min(d[1] for d in done)) * 1e-3
+    agg = sum(d[3] for d in done) / span_s / 1e9
+    offered = len(done) / (span_s or 1.0)
+    log(f"served {len(reqs)} requests; measured {len(done)}: p50 {np.percentile(lat, 50):.1f} ms, "
+        f"p99 {np.percentile(lat, 99):.1f} ms, {agg:.2f} GB/s; stats {st}")
+    line = {
+        "metric": "compress/decompress GB/s on 1 MI355X, byte-identical; batched p50/p99 latency",
+        "value": round(agg, 3), "unit": "GB/s", "n_gpus": 1, "steps": a.requests, "warmup": a.warm_requests,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"serving: Poisson arrivals at {a.rate:g} req/s, 60% compress / 40% decompress; 70% "
+                               "3D float32 side 64-512, 20% 2D float32 side 1024-8192, 10% 3D float64 side 128-256 "
+                               "(log-uniform); ABS or REL 1e-5..1e-2; full host API calls (zstd included)",
+                   "slots": a.workers},
+        "completed_per_s": round(offered, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                       "mean": round(float(lat.mean()), 3), "max": round(float(lat.max()), 3)},
+        "server": st,
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

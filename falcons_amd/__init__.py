@@ -102,12 +102,60 @@ _L.fz_profile_entry.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_size_t, C.P
                                 C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
 _L.fz_profile_entry.restype = C.c_int
 
+
+class fz_batch_item(C.Structure):
+    _fields_ = [("conf", fz_config), ("dtype", C.c_int32), ("status", C.c_int32), ("d_data", C.c_void_p),
+                ("d_pre", C.c_void_p), ("pre_cap", C.c_size_t), ("pre_len", C.c_size_t)]
+
+
+class fz_batch_dec_item(C.Structure):
+    _fields_ = [("conf", fz_config), ("dtype", C.c_int32), ("status", C.c_int32), ("h_pre", C.c_void_p),
+                ("d_pre", C.c_void_p), ("pre_len", C.c_size_t), ("d_out", C.c_void_p),
+                ("out_cap_elems", C.c_size_t)]
+
+
+class fz_request(C.Structure):
+    _fields_ = [("op", C.c_int32), ("dtype", C.c_int32), ("conf", fz_config), ("src", C.c_void_p),
+                ("src_len", C.c_size_t), ("dst", C.c_void_p), ("dst_cap", C.c_size_t), ("tag", C.c_uint64)]
+
+
+class fz_result(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("op", C.c_int32), ("status", C.c_int32), ("out_len", C.c_size_t),
+                ("field_bytes", C.c_size_t), ("t_submit_ms", C.c_double), ("t_start_ms", C.c_double),
+                ("t_done_ms", C.c_double), ("t_device_ms", C.c_double), ("t_zstd_ms", C.c_double),
+                ("msg", C.c_char * 160)]
+
+
+class fz_server_stats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("submitted", "completed", "failed", "in_flight", "max_in_flight",
+                                          "pending", "max_pending", "held_bytes", "max_held_bytes",
+                                          "budget_bytes")]
+
+
+_L.fz_set_workers.argtypes = [C.c_void_p, C.c_int]
+_L.fz_set_workers.restype = C.c_int
+_L.fz_compress_batch.argtypes = [C.c_void_p, C.POINTER(fz_batch_item), C.c_int, C.POINTER(fz_opts)]
+_L.fz_compress_batch.restype = C.c_int
+_L.fz_decompress_batch.argtypes = [C.c_void_p, C.POINTER(fz_batch_dec_item), C.c_int, C.POINTER(fz_opts)]
+_L.fz_decompress_batch.restype = C.c_int
+_L.fz_server_create.argtypes = [C.c_int, C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]
+_L.fz_server_create.restype = C.c_int
+_L.fz_server_submit.argtypes = [C.c_void_p, C.POINTER(fz_request)]
+_L.fz_server_submit.restype = C.c_int
+_L.fz_server_poll.argtypes = [C.c_void_p, C.POINTER(fz_result), C.c_int]
+_L.fz_server_poll.restype = C.c_int
+_L.fz_server_get_stats.argtypes = [C.c_void_p, C.POINTER(fz_server_stats)]
+_L.fz_server_get_stats.restype = C.c_int
+_L.fz_server_destroy.argtypes = [C.c_void_p]
+
 EXPORTED_SYMBOLS = ("fz_ctx_create", "fz_ctx_destroy", "fz_last_error", "fz_opts_init", "fz_config_init",
                     "fz_config_size_est", "fz_config_save", "fz_read_config", "fz_compress_size_bound",
                     "fz_compress", "fz_decompress", "fz_prezstd_bound", "fz_compress_device",
                     "fz_decompress_device", "fz_device_alloc", "fz_device_free", "fz_memcpy", "fz_synchronize",
                     "fz_zstd_version", "fz_profile_enable", "fz_profile_reset", "fz_profile_count",
-                    "fz_profile_entry")
+                    "fz_profile_entry", "fz_set_workers", "fz_compress_batch", "fz_decompress_batch",
+                    "fz_server_create", "fz_server_submit", "fz_server_poll", "fz_server_get_stats",
+                    "fz_server_destroy")
 
 
 def zstd_version():
@@ -429,6 +477,123 @@ class Context:
         rc = _L.fz_synchronize(self.h)
         if rc:
             _raise(rc, self._err())
+
+    def set_workers(self, n):
+        rc = _L.fz_set_workers(self.h, int(n))
+        if rc:
+            _raise(rc, self._err())
+
+    def compress_batch(self, items, simd=1):
+        """fz_compress_batch. items: sequence of (config, dtype, d_data, d_pre) with DeviceBuffers; every
+        config is updated as compress_device updates it. Returns (pre-zstd lengths, statuses, wall ms);
+        raises on the first failure after the whole batch ran."""
+        arr = (fz_batch_item * len(items))()
+        for it, (cfg, dtype, d_data, d_pre) in zip(arr, items):
+            it.conf = cfg.conf
+            it.dtype = _DT[np.dtype(dtype)]
+            it.d_data, it.d_pre, it.pre_cap = d_data.ptr, d_pre.ptr, d_pre.nbytes
+        opts = fz_opts()
+        _L.fz_opts_init(C.byref(opts))
+        opts.simd = int(simd)
+        rc = _L.fz_compress_batch(self.h, arr, len(items), C.byref(opts))
+        for it, (cfg, _, _, _) in zip(arr, items):
+            cfg.conf = it.conf
+        if rc:
+            _raise(rc, self._err())
+        return [int(it.pre_len) for it in arr], [int(it.status) for it in arr], opts.t_total_ms
+
+    def decompress_batch(self, items):
+        """fz_decompress_batch. items: sequence of (config, dtype, h_pre ndarray, d_pre DeviceBuffer or None,
+        d_out DeviceBuffer). Returns (statuses, wall ms)."""
+        arr = (fz_batch_dec_item * len(items))()
+        keep = []
+        for it, (cfg, dtype, h_pre, d_pre, d_out) in zip(arr, items):
+            b = np.ascontiguousarray(h_pre, dtype=np.uint8)
+            keep.append(b)
+            it.conf = cfg.conf
+            it.dtype = _DT[np.dtype(dtype)]
+            it.h_pre, it.pre_len = b.ctypes.data, b.size
+            it.d_pre = d_pre.ptr if d_pre is not None else None
+            it.d_out, it.out_cap_elems = d_out.ptr, d_out.nbytes // np.dtype(dtype).itemsize
+        opts = fz_opts()
+        _L.fz_opts_init(C.byref(opts))
+        rc = _L.fz_decompress_batch(self.h, arr, len(items), C.byref(opts))
+        if rc:
+            _raise(rc, self._err())
+        return [int(it.status) for it in arr], opts.t_total_ms
+
+
+class Server:
+    """fz_server: a request loop on one GPU with `slots` workers and an HBM budget in bytes (0 = 80% of the
+    free device memory). submit() queues a compress (numpy field -> stream) or decompress (stream -> numpy
+    field) request and returns at once; poll() hands back finished requests in completion order."""
+
+    def __init__(self, device=0, slots=8, hbm_budget=0):
+        h = C.c_void_p()
+        rc = _L.fz_server_create(int(device), int(slots), int(hbm_budget), C.byref(h))
+        if rc:
+            raise FalconsError(f"cannot create a server on device {device}")
+        self.h = h
+        self._live = {}  # tag -> (request, buffers) kept alive until polled
+
+    def submit_compress(self, tag, data, config):
+        a = np.ascontiguousarray(data)
+        dt = _DT[a.dtype]
+        cfg = config.copy()
+        if cfg.num_elements != a.size:
+            cfg.setDims(*a.shape)
+        out = np.empty(int(_L.fz_compress_size_bound(C.byref(cfg.conf), dt)), dtype=np.uint8)
+        r = fz_request(op=0, dtype=dt, conf=cfg.conf, src=a.ctypes.data, src_len=a.nbytes,
+                       dst=out.ctypes.data, dst_cap=out.size, tag=int(tag))
+        self._submit(tag, r, (a, out))
+
+    def submit_decompress(self, tag, stream, dtype, num):
+        b = np.ascontiguousarray(np.frombuffer(bytes(stream), dtype=np.uint8)
+                                 if not isinstance(stream, np.ndarray) else stream, dtype=np.uint8)
+        out = np.empty(int(num), dtype=dtype)
+        r = fz_request(op=1, dtype=_DT[np.dtype(dtype)], src=b.ctypes.data, src_len=b.size, dst=out.ctypes.data,
+                       dst_cap=out.nbytes, tag=int(tag))
+        self._submit(tag, r, (b, out))
+
+    def _submit(self, tag, r, bufs):
+        if int(tag) in self._live:
+            raise ValueError(f"tag {tag} is still in flight")
+        self._live[int(tag)] = (r, bufs)
+        rc = _L.fz_server_submit(self.h, C.byref(r))
+        if rc:
+            del self._live[int(tag)]
+            _raise(rc, "request rejected (malformed, or larger than the HBM budget)")
+
+    def poll(self, timeout_ms=-1):
+        """Next finished request as a dict (tag, op, status, output array, times in ms), or None."""
+        res = fz_result()
+        if _L.fz_server_poll(self.h, C.byref(res), int(timeout_ms)) == 0:
+            return None
+        _, bufs = self._live.pop(int(res.tag))
+        out = bufs[1]
+        if res.op == 0 and res.status == FZ_OK:
+            out = out[:res.out_len]
+        return {"tag": int(res.tag), "op": int(res.op), "status": int(res.status), "msg": res.msg.decode(),
+                "out": out, "field_bytes": int(res.field_bytes), "t_submit_ms": res.t_submit_ms,
+                "t_start_ms": res.t_start_ms, "t_done_ms": res.t_done_ms, "t_device_ms": res.t_device_ms,
+                "t_zstd_ms": res.t_zstd_ms}
+
+    def stats(self):
+        st = fz_server_stats()
+        _L.fz_server_get_stats(self.h, C.byref(st))
+        return {k: int(getattr(st, k)) for k, _ in fz_server_stats._fields_}
+
+    def close(self):
+        if self.h:
+            _L.fz_server_destroy(self.h)
+            self.h = None
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default_ctx = None

@@ -18,6 +18,7 @@
 
 #include "../../include/falcons_mi355x.h"
 #include "fz_common.h"
+#include "fz_ctx.h"
 #include "fz_kernels.h"
 #include "fz_tuner.h"
 #include "fz_zstd.h"
@@ -33,27 +34,11 @@ constexpr uint32_t kDataVer = 0x01000000u;  // versionInt("1.0.0"), CMakeLists.t
 constexpr unsigned long long kZstdContentSizeUnknown = 0ULL - 1;  // zstd.h ZSTD_CONTENTSIZE_UNKNOWN
 constexpr unsigned long long kZstdContentSizeError = 0ULL - 2;    // zstd.h ZSTD_CONTENTSIZE_ERROR
 
-struct ArgError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-struct Unsupported : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-double now_ms() {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
 }  // namespace
 
-struct fz_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevArena arena;
-    std::string err;
-    std::unique_ptr<LaunchProfiler> prof;  // per-launch timing, when switched on
-};
+double fz::now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 namespace {
 // Makes the context's launch profiler (if on) the one this thread's launches report to for the
@@ -432,6 +417,22 @@ struct Timer {
     }
 };
 
+}  // namespace
+
+size_t fz::compress_workspace_bytes(uint64_t n, int dtype) {
+    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    return n * es * 3 + n * 2 + n * 8 + entropy_encode_bound(n) + select_escapes_scratch(n) +
+           entropy_encode_workspace(n) + (8u << 20);
+}
+
+size_t fz::decompress_workspace_bytes(uint64_t n, int dtype, uint64_t nunpred, size_t pre_len) {
+    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    return n * es + n * 2 + n * 8 + nunpred * es + pre_len + select_escapes_scratch(n) +
+           entropy_decode_workspace(n, pre_len) + (16u << 20);
+}
+
+namespace {
+
 // The compression pipeline on the GPU. d_data is device memory holding the field; it is
 // overwritten with the reconstructed values.
 // Byte size of decomposition.save without the unpredictable values (IDH:241-252,
@@ -448,9 +449,7 @@ size_t gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, const vo
     const uint64_t n = c.num;
     DevArena& ws = ctx->arena;
     ws.reset();
-    const size_t need = n * es * 3 + n * 2 + n * 8 + entropy_encode_bound(n) + select_escapes_scratch(n) +
-                        entropy_encode_workspace(n) + (8u << 20);
-    ws.reserve(need);
+    ws.reserve(compress_workspace_bytes(n, dtype));
     Timer tm{st, ctx->ev0, ctx->ev1};
     double* eb_dev = ws.take<double>(1);
     void* mm_scratch = ws.take<uint8_t>(minmax_scratch_bytes());
@@ -597,8 +596,7 @@ void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h
     if (P.ncodes != ncodes) throw ArgError("code count does not match the field shape");
     DevArena& ws = ctx->arena;
     ws.reset();
-    ws.reserve(n * es + ncodes * 2 + ncodes * 8 + (size_t)nun * es + pre_len + select_escapes_scratch(ncodes) +
-               entropy_decode_workspace(ncodes, enc_len) + (16u << 20));
+    ws.reserve(decompress_workspace_bytes(n, dtype, (uint64_t)nun, pre_len));
     void* d_out = *d_out_io ? *d_out_io : ws.take<uint8_t>(n * es);
     *d_out_io = d_out;
     const uint8_t* d_enc;
@@ -647,10 +645,6 @@ void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h
     }
 }
 
-void set_err(fz_ctx* ctx, const char* m) {
-    if (ctx) ctx->err = m;
-}
-
 }  // namespace
 
 extern "C" void fz_opts_init(fz_opts* o) {
@@ -678,6 +672,8 @@ extern "C" int fz_ctx_create(int device, fz_ctx** out) {
 
 extern "C" void fz_ctx_destroy(fz_ctx* c) {
     if (!c) return;
+    fz::destroy_workers(c->workers);
+    c->workers = nullptr;
     (void)hipSetDevice(c->device);
     c->arena.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -688,28 +684,128 @@ extern "C" void fz_ctx_destroy(fz_ctx* c) {
 
 extern "C" const char* fz_last_error(const fz_ctx* c) { return c ? c->err.c_str() : ""; }
 
-#define FZ_GUARD(...)                                                    \
-    try {                                                                 \
-        __VA_ARGS__                                                       \
-    } catch (const ArgError& e) {                                         \
-        set_err(ctx, e.what());                                           \
-        return FZ_ERR_INVALID_ARGUMENT;                                   \
-    } catch (const Unsupported& e) {                                      \
-        set_err(ctx, e.what());                                           \
-        return FZ_ERR_UNSUPPORTED;                                        \
-    } catch (const GpuError& e) {                                         \
-        set_err(ctx, e.what());                                           \
-        return FZ_ERR_GPU;                                                \
-    } catch (const std::length_error& e) {                                \
-        set_err(ctx, e.what());                                           \
-        return FZ_ERR_INVALID_ARGUMENT;                                   \
-    } catch (const std::invalid_argument& e) {                            \
-        set_err(ctx, e.what());                                           \
-        return FZ_ERR_INVALID_ARGUMENT;                                   \
-    } catch (const std::exception& e) {                                   \
-        set_err(ctx, e.what());                                           \
-        return FZ_ERR_RUNTIME;                                            \
+namespace fz {
+
+// Falcons_compress (falcons.hpp:54-97, FalconsDispatcher.hpp:25-133) on a field that is already in
+// device memory. d_data is overwritten with the reconstructed values; host_view (may be null) is
+// a host copy of the original values for the ALGO_INTERP_LORENZO tuner. *mutated tells whether
+// the lossy path ran (the reference's input buffer then holds the reconstruction).
+size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_data, const void* host_view,
+                         uint8_t* out, size_t cap, fz_opts* o, bool* mutated) {
+    fz_config c = *conf;
+    if (cap < fz_compress_size_bound(&c, dtype)) throw ArgError("The buffer for compressed data is not large enough.");
+    check_supported(c, dtype);
+    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    const uint64_t n = c.num;
+    // the payload starts after the header size estimated from the caller's configuration; the
+    // final header is written at the front afterwards (falcons.hpp:59-96)
+    const size_t est = fz_config_size_est(&c);
+    memset(out, 0, est);
+    uint8_t* payload = out + est;
+    std::vector<uint8_t> host_copy;
+    if (c.cmpr_algo == 1 && !host_view) {
+        host_copy.resize(n * es);
+        FZ_HIP(hipMemcpyAsync(host_copy.data(), d_data, n * es, hipMemcpyDeviceToHost, ctx->stream));
+        FZ_HIP(hipStreamSynchronize(ctx->stream));
+        host_view = host_copy.data();
     }
+    size_t plen = 0;
+    bool lossless = c.cmpr_algo == 4;
+    std::vector<uint8_t> framed;
+    double tz = 0;
+    if (!lossless) {
+        // a zero absolute bound (after REL conversion) selects the lossless mode
+        // (FalconsDispatcher.hpp:29-31); an ABS bound of zero is known up front
+        if (c.eb_mode == 0 && c.abs_eb == 0) {
+            lossless = true;
+        } else {
+            std::vector<uint8_t> pre;
+            try {
+                uint8_t* d_pre = nullptr;
+                const size_t plen_pre = gpu_compress(ctx, c, dtype, d_data, host_view, o->simd, nullptr, 0, &d_pre, o);
+                pre.resize(plen_pre);
+                FZ_HIP(hipMemcpyAsync(pre.data(), d_pre, plen_pre, hipMemcpyDeviceToHost, ctx->stream));
+                FZ_HIP(hipStreamSynchronize(ctx->stream));
+            } catch (const Unsupported& u) {
+                if (std::string(u.what()) != "zero bound selects the lossless mode") throw;
+                lossless = true;
+            }
+            if (!lossless) {
+                if (o->prezstd) {
+                    o->prezstd_len = pre.size();
+                    if (pre.size() <= o->prezstd_cap) memcpy(o->prezstd, pre.data(), pre.size());
+                }
+                const double tz0 = now_ms();
+                zstd_frame(pre.data(), pre.size(), framed);
+                tz += now_ms() - tz0;
+                plen = framed.size();
+            }
+        }
+    }
+    *mutated = !lossless;
+    if (lossless) {
+        c.cmpr_algo = 4;
+        std::vector<uint8_t> src(n * es);
+        FZ_HIP(hipMemcpyAsync(src.data(), d_data, n * es, hipMemcpyDeviceToHost, ctx->stream));
+        FZ_HIP(hipStreamSynchronize(ctx->stream));
+        const double tz0 = now_ms();
+        zstd_frame(src.data(), src.size(), framed);
+        tz += now_ms() - tz0;
+        plen = framed.size();
+        if (c.eb_mode != 0) c.eb_mode = 0;
+    } else if ((double)(n * es) / 1.0 / (double)plen < 3) {
+        // ratio below 3: zstd of the reconstructed field may win (FalconsDispatcher.hpp:121-133)
+        std::vector<uint8_t> src(n * es), z;
+        FZ_HIP(hipMemcpyAsync(src.data(), d_data, n * es, hipMemcpyDeviceToHost, ctx->stream));
+        FZ_HIP(hipStreamSynchronize(ctx->stream));
+        const double tz0 = now_ms();
+        zstd_frame(src.data(), src.size(), z);
+        tz += now_ms() - tz0;
+        if (z.size() < plen && z.size() <= cap - est) {
+            c.cmpr_algo = 4;
+            framed.swap(z);
+            plen = framed.size();
+        }
+    }
+    if (plen > cap - est) throw ArgError("The buffer for compressed data is not large enough.");
+    memcpy(payload, framed.data(), plen);
+    const size_t hs = fz_config_save(&c, out);
+    o->t_zstd_ms = tz;
+    return hs + plen;
+}
+
+// Falcons_decompress (falcons.hpp:184-191) from a host stream into device memory: *d_out_io is
+// the destination (allocated in the context workspace when null).
+void decompress_to_device(fz_ctx* ctx, const uint8_t* cmp, size_t len, int dtype, void** d_out_io,
+                          size_t out_cap_elems, fz_config* conf_out, fz_opts* o) {
+    fz_config c;
+    const size_t hs = load_config(cmp, len, &c);
+    if (conf_out) *conf_out = c;
+    if (c.num > out_cap_elems) throw ArgError("The decompression buffer is too small");
+    if (c.openmp) throw Unsupported("the OpenMP stream format is not produced by this path");
+    if (dtype != FZ_FLOAT && dtype != FZ_DOUBLE) throw Unsupported("only float and double fields are on this path");
+    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    const uint64_t n = c.num;
+    hipStream_t st = ctx->stream;
+    const double tz0 = now_ms();
+    std::vector<uint8_t> buf = zstd_unframe(cmp + hs, len - hs);
+    o->t_zstd_ms = now_ms() - tz0;
+    if (c.cmpr_algo == 4) {
+        if (buf.size() != n * es) throw std::runtime_error("Decompressed data size does not match the original data size");
+        if (!*d_out_io) {
+            ctx->arena.reset();
+            ctx->arena.reserve(n * es + 256);
+            *d_out_io = ctx->arena.take<uint8_t>(n * es);
+        }
+        FZ_HIP(hipMemcpyAsync(*d_out_io, buf.data(), n * es, hipMemcpyHostToDevice, st));
+        FZ_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    if (c.cmpr_algo != 2) throw Unsupported("only the interpolation algorithms are on this path");
+    gpu_decompress(ctx, c, dtype, buf.data(), nullptr, buf.size(), d_out_io, o);
+}
+
+}  // namespace fz
 
 extern "C" int fz_compress(fz_ctx* ctx, const fz_config* conf, int dtype, void* data, int data_on_device, uint8_t* out,
                            size_t cap, size_t* out_len, fz_opts* opts) {
@@ -720,16 +816,10 @@ extern "C" int fz_compress(fz_ctx* ctx, const fz_config* conf, int dtype, void* 
         fz_opts_init(&local);
         fz_opts* o = opts ? opts : &local;
         FZ_HIP(hipSetDevice(ctx->device));
-        fz_config c = *conf;
-        if (cap < fz_compress_size_bound(&c, dtype)) throw ArgError("The buffer for compressed data is not large enough.");
-        check_supported(c, dtype);
+        if (cap < fz_compress_size_bound(conf, dtype)) throw ArgError("The buffer for compressed data is not large enough.");
+        check_supported(*conf, dtype);
         const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
-        const uint64_t n = c.num;
-        // Falcons_compress: the payload starts after the header size estimated from the caller's
-        // configuration; the final header is written at the front afterwards (falcons.hpp:59-96).
-        const size_t est = fz_config_size_est(&c);
-        memset(out, 0, est);
-        uint8_t* payload = out + est;
+        const uint64_t n = conf->num;
         // device copy of the field
         void* d_data = data;
         void* d_owned = nullptr;
@@ -742,80 +832,12 @@ extern "C" int fz_compress(fz_ctx* ctx, const fz_config* conf, int dtype, void* 
             void* p;
             ~Free() { if (p) (void)hipFree(p); }
         } guard{d_owned};
-        std::vector<uint8_t> host_copy;
-        const void* host_view = data_on_device ? nullptr : data;
-        if (c.cmpr_algo == 1 && data_on_device) {
-            host_copy.resize(n * es);
-            FZ_HIP(hipMemcpyAsync(host_copy.data(), data, n * es, hipMemcpyDeviceToHost, ctx->stream));
-            FZ_HIP(hipStreamSynchronize(ctx->stream));
-            host_view = host_copy.data();
-        }
-        size_t plen = 0;
-        bool lossless = c.cmpr_algo == 4;
-        std::vector<uint8_t> framed;
-        double tz = 0;
-        if (!lossless) {
-            // a zero absolute bound (after REL conversion) selects the lossless mode
-            // (FalconsDispatcher.hpp:29-31); an ABS bound of zero is known up front
-            if (c.eb_mode == 0 && c.abs_eb == 0) {
-                lossless = true;
-            } else {
-                std::vector<uint8_t> pre;
-                try {
-                    uint8_t* d_pre = nullptr;
-                    const size_t plen_pre = gpu_compress(ctx, c, dtype, d_data, host_view, o->simd, nullptr, 0, &d_pre, o);
-                    pre.resize(plen_pre);
-                    FZ_HIP(hipMemcpyAsync(pre.data(), d_pre, plen_pre, hipMemcpyDeviceToHost, ctx->stream));
-                    FZ_HIP(hipStreamSynchronize(ctx->stream));
-                } catch (const Unsupported& u) {
-                    if (std::string(u.what()) != "zero bound selects the lossless mode") throw;
-                    lossless = true;
-                }
-                if (!lossless) {
-                    if (o->prezstd) {
-                        o->prezstd_len = pre.size();
-                        if (pre.size() <= o->prezstd_cap) memcpy(o->prezstd, pre.data(), pre.size());
-                    }
-                    const double tz0 = now_ms();
-                    zstd_frame(pre.data(), pre.size(), framed);
-                    tz += now_ms() - tz0;
-                    plen = framed.size();
-                }
-            }
-        }
-        if (!lossless && !data_on_device && o->copy_back) {
+        bool mutated = false;
+        *out_len = compress_resident(ctx, conf, dtype, d_data, data_on_device ? nullptr : data, out, cap, o, &mutated);
+        if (mutated && !data_on_device && o->copy_back) {
             FZ_HIP(hipMemcpyAsync(data, d_data, n * es, hipMemcpyDeviceToHost, ctx->stream));
             FZ_HIP(hipStreamSynchronize(ctx->stream));
         }
-        if (lossless) {
-            c.cmpr_algo = 4;
-            std::vector<uint8_t> src(n * es);
-            FZ_HIP(hipMemcpyAsync(src.data(), d_data, n * es, hipMemcpyDeviceToHost, ctx->stream));
-            FZ_HIP(hipStreamSynchronize(ctx->stream));
-            const double tz0 = now_ms();
-            zstd_frame(src.data(), src.size(), framed);
-            tz += now_ms() - tz0;
-            plen = framed.size();
-            if (c.eb_mode != 0) c.eb_mode = 0;
-        } else if ((double)(n * es) / 1.0 / (double)plen < 3) {
-            // ratio below 3: zstd of the reconstructed field may win (FalconsDispatcher.hpp:121-133)
-            std::vector<uint8_t> src(n * es), z;
-            FZ_HIP(hipMemcpyAsync(src.data(), d_data, n * es, hipMemcpyDeviceToHost, ctx->stream));
-            FZ_HIP(hipStreamSynchronize(ctx->stream));
-            const double tz0 = now_ms();
-            zstd_frame(src.data(), src.size(), z);
-            tz += now_ms() - tz0;
-            if (z.size() < plen && z.size() <= cap - est) {
-                c.cmpr_algo = 4;
-                framed.swap(z);
-                plen = framed.size();
-            }
-        }
-        if (plen > cap - est) throw ArgError("The buffer for compressed data is not large enough.");
-        memcpy(payload, framed.data(), plen);
-        const size_t hs = fz_config_save(&c, out);
-        *out_len = hs + plen;
-        o->t_zstd_ms = tz;
         o->t_total_ms = now_ms() - t0;
         return FZ_OK;
     })
@@ -830,31 +852,16 @@ extern "C" int fz_decompress(fz_ctx* ctx, const uint8_t* cmp, size_t len, int dt
         fz_opts_init(&local);
         fz_opts* o = opts ? opts : &local;
         FZ_HIP(hipSetDevice(ctx->device));
-        fz_config c;
-        const size_t hs = load_config(cmp, len, &c);
-        if (conf_out) *conf_out = c;
-        if (!out || c.num > out_cap_elems) throw ArgError("The decompression buffer is too small");
-        if (c.openmp) throw Unsupported("the OpenMP stream format is not produced by this path");
-        if (dtype != FZ_FLOAT && dtype != FZ_DOUBLE) throw Unsupported("only float and double fields are on this path");
-        const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
-        const uint64_t n = c.num;
-        hipStream_t st = ctx->stream;
-        const double tz0 = now_ms();
-        std::vector<uint8_t> buf = zstd_unframe(cmp + hs, len - hs);
-        o->t_zstd_ms = now_ms() - tz0;
-        if (c.cmpr_algo == 4) {
-            if (buf.size() != n * es) throw std::runtime_error("Decompressed data size does not match the original data size");
-            FZ_HIP(hipMemcpyAsync(out, buf.data(), n * es, out_on_device ? hipMemcpyHostToDevice : hipMemcpyHostToHost, st));
-            FZ_HIP(hipStreamSynchronize(st));
-            o->t_total_ms = now_ms() - t0;
-            return FZ_OK;
-        }
-        if (c.cmpr_algo != 2) throw Unsupported("only the interpolation algorithms are on this path");
+        fz_config tmp;
+        fz_config* c = conf_out ? conf_out : &tmp;
+        load_config(cmp, len, c);
+        if (!out) throw ArgError("The decompression buffer is too small");
         void* d_out = out_on_device ? out : nullptr;
-        gpu_decompress(ctx, c, dtype, buf.data(), nullptr, buf.size(), &d_out, o);
+        decompress_to_device(ctx, cmp, len, dtype, &d_out, out_cap_elems, c, o);
         if (!out_on_device) {
-            FZ_HIP(hipMemcpyAsync(out, d_out, n * es, hipMemcpyDeviceToHost, st));
-            FZ_HIP(hipStreamSynchronize(st));
+            const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+            FZ_HIP(hipMemcpyAsync(out, d_out, c->num * es, hipMemcpyDeviceToHost, ctx->stream));
+            FZ_HIP(hipStreamSynchronize(ctx->stream));
         }
         o->t_total_ms = now_ms() - t0;
         return FZ_OK;

@@ -1,0 +1,82 @@
+// Context of the C-ABI and the host-side pieces shared by the per-call entry points (fz_api.cpp)
+// and the batched and serving forms (fz_serve.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/falcons_mi355x.h"
+#include "fz_common.h"
+#include "fz_kernels.h"
+
+namespace fz {
+
+struct ArgError : std::runtime_error {  // the reference's std::invalid_argument
+    using std::runtime_error::runtime_error;
+};
+struct Unsupported : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+class WorkerSet;  // sub-contexts used by the batched entry points
+void destroy_workers(WorkerSet* w);
+
+double now_ms();
+
+// Device bytes one call reserves in its context workspace.
+size_t compress_workspace_bytes(uint64_t n, int dtype);
+size_t decompress_workspace_bytes(uint64_t n, int dtype, uint64_t nunpred, size_t pre_len);
+
+}  // namespace fz
+
+struct fz_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    fz::DevArena arena;
+    std::string err;
+    std::unique_ptr<fz::LaunchProfiler> prof;  // per-launch timing, when switched on
+    fz::WorkerSet* workers = nullptr;          // created by fz_set_workers
+};
+
+namespace fz {
+
+// Falcons_compress on a field already in device memory (overwritten with the reconstruction);
+// host_view, when given, is a host copy of the original values. *mutated: the lossy path ran.
+size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_data, const void* host_view,
+                         uint8_t* out, size_t cap, fz_opts* o, bool* mutated);
+// Falcons_decompress from a host stream into device memory (*d_out_io; workspace when null).
+void decompress_to_device(fz_ctx* ctx, const uint8_t* cmp, size_t len, int dtype, void** d_out_io,
+                          size_t out_cap_elems, fz_config* conf_out, fz_opts* o);
+
+inline void set_err(fz_ctx* ctx, const char* m) {
+    if (ctx) ctx->err = m;
+}
+
+}  // namespace fz
+
+// Maps the exceptions of a call to status codes, keeping the message in the context.
+#define FZ_GUARD(...)                                                    \
+    try {                                                                 \
+        __VA_ARGS__                                                       \
+    } catch (const fz::ArgError& e) {                                     \
+        fz::set_err(ctx, e.what());                                       \
+        return FZ_ERR_INVALID_ARGUMENT;                                   \
+    } catch (const fz::Unsupported& e) {                                  \
+        fz::set_err(ctx, e.what());                                       \
+        return FZ_ERR_UNSUPPORTED;                                        \
+    } catch (const fz::GpuError& e) {                                     \
+        fz::set_err(ctx, e.what());                                       \
+        return FZ_ERR_GPU;                                                \
+    } catch (const std::length_error& e) {                                \
+        fz::set_err(ctx, e.what());                                       \
+        return FZ_ERR_INVALID_ARGUMENT;                                   \
+    } catch (const std::invalid_argument& e) {                            \
+        fz::set_err(ctx, e.what());                                       \
+        return FZ_ERR_INVALID_ARGUMENT;                                   \
+    } catch (const std::exception& e) {                                   \
+        fz::set_err(ctx, e.what());                                       \
+        return FZ_ERR_RUNTIME;                                            \
+    }

@@ -184,24 +184,147 @@ __global__ void k_delta(const int16_t* __restrict__ in, int16_t* __restrict__ ou
 
 static inline unsigned nblk(uint64_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
+// ---------------------------------------------------------------- FSE tables (workgroup-wide)
+// FSE symbol spread and table construction in LDS by a 256-thread workgroup, for any table log up
+// to 13. Spread (zfse/fse_compress.c FSE_buildCTable_wksp, fse_decompress.c FSE_buildDTable):
+// symbols with count -1 take the top positions in symbol order; the k-th occurrence of the others
+// (in symbol order) goes to the k-th position of the walk j*step mod 2^tableLog, j = 0, 1, ...,
+// that lies at or below the high threshold. The walk is a permutation (step is odd), so it is
+// enumerated in parallel and ranked with a block scan. Within a symbol, positions in increasing
+// order get consecutive states: one thread per symbol scans the spread table.
+struct TableLds {
+    uint8_t sym[1u << kFseMaxTableLog];
+    int16_t norm[256];          // symbols above maxSym hold 0
+    uint32_t cum[257];          // exclusive prefix of the counts (-1 counts as 1)
+    uint32_t cump[257];         // exclusive prefix of the positive counts
+    uint32_t part[256];
+    int32_t high;               // last position of the walk part; -1 when every count is -1
+    int bad;
+};
+
+__device__ bool spread_parallel(TableLds& L, unsigned tl) {
+    const unsigned t = threadIdx.x;
+    const uint32_t ts = 1u << tl, mask = ts - 1, step = (ts >> 1) + (ts >> 3) + 3;
+    if (t == 0) {
+        uint32_t c = 0, cp = 0;
+        int32_t high = (int32_t)ts - 1;
+        int bad = 0;
+        for (int s = 0; s < 256; s++) {
+            const int v = L.norm[s];
+            if (v < -1 || v > (int)ts) bad = 1;
+            L.cum[s] = c;
+            L.cump[s] = cp;
+            if (v == -1) {
+                c += 1;
+                if (c <= ts) L.sym[high--] = (uint8_t)s;
+            } else if (v > 0) {
+                c += (uint32_t)v;
+                cp += (uint32_t)v;
+            }
+        }
+        L.cum[256] = c;
+        L.cump[256] = cp;
+        if (c != ts) bad = 1;
+        L.high = high;
+        L.bad = bad;
+    }
+    __syncthreads();
+    if (L.bad) return false;
+    const int32_t high = L.high;
+    const uint32_t per = (ts + 255) / 256;  // walk steps per thread
+    const uint32_t k0 = min(ts, t * per), k1 = min(ts, k0 + per);
+    uint32_t cnt = 0;
+    for (uint32_t k = k0; k < k1; k++) cnt += (int32_t)((k * step) & mask) <= high;
+    L.part[t] = cnt;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < 256; i++) { const uint32_t v = L.part[i]; L.part[i] = acc; acc += v; }
+    }
+    __syncthreads();
+    uint32_t rank = L.part[t];
+    for (uint32_t k = k0; k < k1; k++) {
+        const uint32_t pos = (k * step) & mask;
+        if ((int32_t)pos > high) continue;
+        int lo = 0, hi = 256;  // last symbol with cump <= rank: the one whose counts hold rank
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (L.cump[mid] <= rank) lo = mid; else hi = mid;
+        }
+        L.sym[pos] = (uint8_t)lo;
+        rank++;
+    }
+    __syncthreads();
+    return true;
+}
+
+// FSE_buildCTable_wksp (fse_compress.c:65-155) after spread_parallel; thread t handles symbol t.
+__device__ void ctable_parallel(const TableLds& L, unsigned tl, unsigned maxSym, CTable* ct) {
+    const unsigned t = threadIdx.x;
+    const uint32_t ts = 1u << tl;
+    const int nc = L.norm[t];
+    uint32_t slot = L.cum[t];
+    if (nc != 0) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(L.sym);
+        for (uint32_t u4 = 0; u4 < ts / 4; u4++) {
+            const uint32_t v = w[u4];
+            for (int b = 0; b < 4; b++)
+                if (((v >> (8 * b)) & 0xFFu) == t) ct->stateTable[slot++] = (uint16_t)(ts + 4 * u4 + b);
+        }
+    }
+    if (nc == 0) {
+        ct->deltaNbBits[t] = ((tl + 1) << 16) - ts;
+        ct->deltaFindState[t] = 0;
+    } else if (nc == -1 || nc == 1) {
+        ct->deltaNbBits[t] = (tl << 16) - ts;
+        ct->deltaFindState[t] = (int32_t)L.cum[t] - 1;
+    } else {
+        const uint32_t maxBitsOut = tl - highbit32((uint32_t)(nc - 1));
+        ct->deltaNbBits[t] = (maxBitsOut << 16) - ((uint32_t)nc << maxBitsOut);
+        ct->deltaFindState[t] = (int32_t)L.cum[t] - nc;
+    }
+    if (t == 0) { ct->tableLog = tl; ct->maxSym = maxSym; }
+}
+
+// FSE_buildDTable (fse_decompress.c:70-127) after spread_parallel; thread t handles symbol t.
+__device__ void dtable_parallel(const TableLds& L, unsigned tl, DEntry* dt) {
+    const unsigned t = threadIdx.x;
+    const uint32_t ts = 1u << tl;
+    const int nc = L.norm[t];
+    if (nc == 0) return;
+    uint32_t next = (nc == -1) ? 1u : (uint32_t)nc;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(L.sym);
+    for (uint32_t u4 = 0; u4 < ts / 4; u4++) {
+        const uint32_t v = w[u4];
+        for (int k = 0; k < 4; k++)
+            if (((v >> (8 * k)) & 0xFFu) == t) {
+                const uint32_t ns = next++;
+                const uint32_t nb = tl - highbit32(ns);
+                dt[4 * u4 + k] = DEntry{(uint16_t)((ns << nb) - ts), (uint8_t)t, (uint8_t)nb};
+            }
+    }
+}
+
 // ---------------------------------------------------------------- FSE block (workgroup-wide)
 // FSE_compress of one block of at most 64 K bytes (fse_compress.c:632-694), run by one workgroup:
-// the histogram is shared, the table build and the two-state encode run on lane 0.
+// histogram, spread and table build are shared by the workgroup; normalization, the table header
+// and the two-state encode run on lane 0.
 struct FseLds {
     uint32_t count[256];
-    int16_t norm[256];
-    uint8_t spread[1u << kFseMaxTableLog];
+    TableLds tab;
     CTable ct;
-    uint64_t result;
+    uint64_t result, nc;
+    uint32_t tl, maxSym;
 };
 
 __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_t* dst, FseLds& L) {
+    constexpr uint64_t kPending = ~1ull;
     for (int s = threadIdx.x; s < 256; s += blockDim.x) L.count[s] = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&L.count[src[i]], 1u);
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t r;
+        uint64_t r = kPending;
         if (n <= 1) {
             r = 0;
         } else {
@@ -213,40 +336,55 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_
             else if (maxCount == 1 || maxCount < (n >> 7)) r = 0;
             else {
                 const unsigned tl = optimal_table_log(kFseDefaultTableLog, n, maxSym);
-                if (normalize_count(L.norm, tl, L.count, n, maxSym) == kFseErr) {
+                for (unsigned s = maxSym + 1; s < 256; s++) L.tab.norm[s] = 0;
+                if (normalize_count(L.tab.norm, tl, L.count, n, maxSym) == kFseErr) {
                     r = kFseErr;
                 } else {
-                    const uint64_t nc = write_ncount(dst, L.norm, maxSym, tl);
-                    build_ctable(&L.ct, L.norm, maxSym, tl, L.spread);
-                    BitW w{dst + nc, 0, 0, 0};
-                    uint32_t ip = n, s1, s2;
-                    if (n & 1) {
-                        s1 = init_state2(&L.ct, src[--ip]);
-                        s2 = init_state2(&L.ct, src[--ip]);
-                        fse_encode(w, s1, L.ct, src[--ip]);
-                    } else {
-                        s2 = init_state2(&L.ct, src[--ip]);
-                        s1 = init_state2(&L.ct, src[--ip]);
-                    }
-                    if ((n - 2) & 2) {
-                        fse_encode(w, s2, L.ct, src[--ip]);
-                        fse_encode(w, s1, L.ct, src[--ip]);
-                    }
-                    while (ip > 0) {
-                        fse_encode(w, s2, L.ct, src[--ip]);
-                        fse_encode(w, s1, L.ct, src[--ip]);
-                        fse_encode(w, s2, L.ct, src[--ip]);
-                        fse_encode(w, s1, L.ct, src[--ip]);
-                    }
-                    bw_add(w, s2, tl);
-                    bw_add(w, s1, tl);
-                    const uint64_t bs = bw_close(w);
-                    const uint64_t total = nc + bs;
-                    r = (total >= n - 1) ? 0 : total;
+                    L.nc = write_ncount(dst, L.tab.norm, maxSym, tl);
+                    L.tl = tl;
+                    L.maxSym = maxSym;
                 }
             }
         }
         L.result = r;
+    }
+    __syncthreads();
+    if (L.result != kPending) return L.result;
+    const unsigned tl = L.tl;
+    if (!spread_parallel(L.tab, tl)) {  // cannot happen for counts from normalize_count
+        if (threadIdx.x == 0) L.result = kFseErr;
+        __syncthreads();
+        return L.result;
+    }
+    ctable_parallel(L.tab, tl, L.maxSym, &L.ct);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint64_t nc = L.nc;
+        BitW w{dst + nc, 0, 0, 0};
+        uint32_t ip = n, s1, s2;
+        if (n & 1) {
+            s1 = init_state2(&L.ct, src[--ip]);
+            s2 = init_state2(&L.ct, src[--ip]);
+            fse_encode(w, s1, L.ct, src[--ip]);
+        } else {
+            s2 = init_state2(&L.ct, src[--ip]);
+            s1 = init_state2(&L.ct, src[--ip]);
+        }
+        if ((n - 2) & 2) {
+            fse_encode(w, s2, L.ct, src[--ip]);
+            fse_encode(w, s1, L.ct, src[--ip]);
+        }
+        while (ip > 0) {
+            fse_encode(w, s2, L.ct, src[--ip]);
+            fse_encode(w, s1, L.ct, src[--ip]);
+            fse_encode(w, s2, L.ct, src[--ip]);
+            fse_encode(w, s1, L.ct, src[--ip]);
+        }
+        bw_add(w, s2, tl);
+        bw_add(w, s1, tl);
+        const uint64_t bs = bw_close(w);
+        const uint64_t total = nc + bs;
+        L.result = (total >= n - 1) ? 0 : total;
     }
     __syncthreads();
     return L.result;
@@ -297,33 +435,18 @@ __global__ void k_pack_slots(const uint8_t* slots, uint64_t slot_stride, const u
         out[offs[b] + i] = slots[b * slot_stride + i];
 }
 
-// FSE_decompress of one block (fse_decompress.c:254-282) by lane 0 of a workgroup.
+// FSE_decompress of one block (fse_decompress.c:254-282): the workgroup reads the header and builds
+// the decoding table, lane 0 decodes.
 struct FseDecLds {
-    int16_t norm[256];
-    uint8_t spread[1u << kFseMaxTableLog];
+    TableLds tab;
     DEntry dt[1u << kFseMaxTableLog];
+    uint32_t tl;
+    uint64_t hl;
 };
-__device__ bool fse_block_decompress(const uint8_t* src, uint64_t srcSize, uint8_t* dst, uint32_t n, FseDecLds& L) {
-    unsigned maxSym = 255, tl = 0;
-    const uint64_t hl = read_ncount(L.norm, &maxSym, &tl, src, srcSize);
-    if (hl == kFseErr) return false;
-    if (!build_dtable(L.dt, L.norm, maxSym, tl, L.spread)) return false;
-    BitR r;
-    if (!br_init(r, src + hl, srcSize - hl)) return false;
-    uint32_t st[2];
-    st[0] = br_read(r, tl);
-    st[1] = br_read(r, tl);
-    for (uint32_t i = 0; i < n; i++) {
-        uint32_t& s = st[i & 1];
-        const DEntry e = L.dt[s];
-        dst[i] = e.symbol;
-        s = e.newState + br_read(r, e.nbBits);
-    }
-    return true;
-}
 
 // Decodes framed class-byte blocks; block offsets were found by the host.
-__global__ void k_fse_dec_blocks(const uint8_t* in, const uint64_t* offs, uint64_t n, uint8_t* out, int* err) {
+__global__ void __launch_bounds__(256) k_fse_dec_blocks(const uint8_t* in, const uint64_t* offs, uint64_t n,
+                                                        uint8_t* out, int* err) {
     __shared__ FseDecLds L;
     const uint64_t b = blockIdx.x;
     const uint64_t o = offs[b];
@@ -332,12 +455,50 @@ __global__ void k_fse_dec_blocks(const uint8_t* in, const uint64_t* offs, uint64
     uint8_t* dst = out + b * kFblk;
     if (flag == 0) {
         for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = in[o + 1 + i];
-    } else if (flag == 2) {
+        return;
+    }
+    if (flag == 2) {
         for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = in[o + 1];
-    } else if (threadIdx.x == 0) {
-        const uint32_t rr = (uint32_t)in[o + 1] | ((uint32_t)in[o + 2] << 8) | ((uint32_t)in[o + 3] << 16) |
-                            ((uint32_t)in[o + 4] << 24);
-        if (!fse_block_decompress(in + o + 5, rr, dst, len, L)) *err = 1;
+        return;
+    }
+    const uint8_t* src = in + o + 5;
+    const uint32_t srcSize = (uint32_t)in[o + 1] | ((uint32_t)in[o + 2] << 8) | ((uint32_t)in[o + 3] << 16) |
+                             ((uint32_t)in[o + 4] << 24);
+    if (threadIdx.x == 0) {
+        unsigned maxSym = 255, tl = 0;
+        L.hl = read_ncount(L.tab.norm, &maxSym, &tl, src, srcSize);
+        for (unsigned s = maxSym + 1; s < 256; s++) L.tab.norm[s] = 0;
+        L.tl = tl;
+    }
+    __syncthreads();
+    const uint64_t hl = L.hl;
+    if (hl == kFseErr || hl >= srcSize) {
+        if (threadIdx.x == 0) *err = 1;
+        return;
+    }
+    const unsigned tl = L.tl;
+    if (!spread_parallel(L.tab, tl)) {
+        if (threadIdx.x == 0) *err = 1;
+        return;
+    }
+    dtable_parallel(L.tab, tl, L.dt);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const uint8_t* bs = src + hl;
+    const uint64_t bsz = srcSize - hl;
+    if (bs[bsz - 1] == 0) {
+        *err = 1;
+        return;
+    }
+    BitWin r = bwin_at(bs, (int64_t)(bsz - 1) * 8 + (int64_t)highbit32(bs[bsz - 1]));
+    uint32_t st[2];
+    st[0] = bwin_read(r, tl);
+    st[1] = bwin_read(r, tl);
+    for (uint32_t i = 0; i < len; i++) {
+        uint32_t& s = st[i & 1];
+        const DEntry e = L.dt[s];
+        dst[i] = e.symbol;
+        s = e.newState + bwin_read(r, e.nbBits);
     }
 }
 
@@ -506,108 +667,20 @@ __device__ void o1_normalize(const uint32_t* h, uint16_t* f) {
     }
 }
 
-// FSE symbol spread and table construction for one context in LDS, by a 256-thread workgroup.
-// Spread (zfse/fse_compress.c FSE_buildCTable_wksp, fse_decompress.c FSE_buildDTable): symbols with
-// count -1 take the top positions in symbol order; the k-th occurrence of the others (in symbol
-// order) goes to the k-th position of the walk j*step mod 2^13, j = 0, 1, ..., that lies at or
-// below the high threshold; the walk is enumerated in parallel and ranked with a block scan.
-// Within a symbol, positions in increasing order get consecutive states: one thread per symbol
-// scans the spread table.
-struct TableLds {
-    uint8_t sym[1u << kO1Log];
-    int16_t norm[kO1Sym];
-    uint32_t cum[kO1Sym + 1];   // exclusive prefix of the counts (-1 counts as 1)
-    uint32_t cump[kO1Sym + 1];  // exclusive prefix of the positive counts
-    uint32_t part[256];
-    uint32_t high;
-    int bad;
-};
-
-__device__ bool spread_parallel(TableLds& L) {
-    const unsigned t = threadIdx.x;
-    constexpr uint32_t ts = 1u << kO1Log, mask = ts - 1, step = (ts >> 1) + (ts >> 3) + 3;
-    if (t == 0) {
-        uint32_t c = 0, cp = 0, high = ts - 1;
-        int bad = 0;
-        for (int s = 0; s < kO1Sym; s++) {
-            const int v = L.norm[s];
-            if (v < -1 || v > (int)ts) bad = 1;
-            L.cum[s] = c;
-            L.cump[s] = cp;
-            if (v == -1) { c += 1; L.sym[high--] = (uint8_t)s; }
-            else if (v > 0) { c += (uint32_t)v; cp += (uint32_t)v; }
-        }
-        L.cum[kO1Sym] = c;
-        L.cump[kO1Sym] = cp;
-        if (c != ts) bad = 1;
-        L.high = high;
-        L.bad = bad;
-    }
-    __syncthreads();
-    if (L.bad) return false;
-    const uint32_t high = L.high;
-    constexpr uint32_t per = ts / 256;
-    uint32_t cnt = 0;
-    for (uint32_t k = 0; k < per; k++) cnt += (((t * per + k) * step) & mask) <= high;
-    L.part[t] = cnt;
-    __syncthreads();
-    if (t == 0) {
-        uint32_t acc = 0;
-        for (int i = 0; i < 256; i++) { const uint32_t v = L.part[i]; L.part[i] = acc; acc += v; }
-    }
-    __syncthreads();
-    uint32_t rank = L.part[t];
-    for (uint32_t k = 0; k < per; k++) {
-        const uint32_t pos = ((t * per + k) * step) & mask;
-        if (pos > high) continue;
-        int lo = 0, hi = kO1Sym;  // last symbol with cump <= rank: the one whose counts hold rank
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (L.cump[mid] <= rank) lo = mid; else hi = mid;
-        }
-        L.sym[pos] = (uint8_t)lo;
-        rank++;
-    }
-    __syncthreads();
-    return true;
-}
-
 // Encoding tables of every context of every segment: grid (contexts, segments).
 __global__ void __launch_bounds__(256) k_o1_tables(const uint32_t* cnt_all, uint16_t* nf_all, CTable* ct_all) {
     __shared__ TableLds L;
     const unsigned t = threadIdx.x;
     const size_t c = (size_t)blockIdx.y * kO1Ctx + blockIdx.x;
     uint16_t* nf = nf_all + c * kO1Sym;
-    CTable* ct = ct_all + c;
     if (t == 0) o1_normalize(cnt_all + c * kO1Sym, nf);
     __syncthreads();
     L.norm[t] = (int16_t)nf[t];
     __syncthreads();
-    if (!spread_parallel(L)) return;  // cannot happen for counts from o1_normalize
-    constexpr uint32_t ts = 1u << kO1Log;
-    const int nc = L.norm[t];
-    uint32_t slot = L.cum[t];
-    if (nc != 0) {
-        const uint32_t* w = reinterpret_cast<const uint32_t*>(L.sym);
-        for (uint32_t u4 = 0; u4 < ts / 4; u4++) {
-            const uint32_t v = w[u4];
-            for (int b = 0; b < 4; b++)
-                if (((v >> (8 * b)) & 0xFFu) == t) ct->stateTable[slot++] = (uint16_t)(ts + 4 * u4 + b);
-        }
-    }
-    if (nc == 0) {
-        ct->deltaNbBits[t] = ((kO1Log + 1) << 16) - ts;
-        ct->deltaFindState[t] = 0;
-    } else if (nc == -1 || nc == 1) {
-        ct->deltaNbBits[t] = (kO1Log << 16) - ts;
-        ct->deltaFindState[t] = (int32_t)L.cum[t] - 1;
-    } else {
-        const uint32_t maxBitsOut = kO1Log - highbit32((uint32_t)(nc - 1));
-        ct->deltaNbBits[t] = (maxBitsOut << 16) - ((uint32_t)nc << maxBitsOut);
-        ct->deltaFindState[t] = (int32_t)L.cum[t] - nc;
-    }
-    if (t == 0) { ct->tableLog = kO1Log; ct->maxSym = kO1Sym - 1; }
+    if (!spread_parallel(L, kO1Log)) return;  // cannot happen for counts from o1_normalize
+    ctable_parallel(L, kO1Log, kO1Sym - 1, ct_all + c);
 }
+
 
 // ---- order-1 tANS encoding split into pieces --------------------------------------------------
 // A chain is encoded backwards: symbol hi-1 seeds the state (FSE_initCState2), then one encoder
@@ -835,26 +908,13 @@ __global__ void __launch_bounds__(256) k_o1_dtables(const uint8_t* stream, const
     const uint8_t* b = stream + nf_off[blockIdx.y] + 2 * ((size_t)blockIdx.x * kO1Sym + t);
     L.norm[t] = (int16_t)(uint16_t)(b[0] | (b[1] << 8));
     __syncthreads();
-    constexpr uint32_t ts = 1u << kO1Log;
-    DEntry* dt = dt_all + ((size_t)blockIdx.y * kO1Ctx + blockIdx.x) * ts;
-    if (!spread_parallel(L)) {
+    if (!spread_parallel(L, kO1Log)) {
         if (t == 0) *err = 1;
         return;
     }
-    const int nc = L.norm[t];
-    if (nc == 0) return;
-    uint32_t next = (nc == -1) ? 1u : (uint32_t)nc;
-    const uint32_t* w = reinterpret_cast<const uint32_t*>(L.sym);
-    for (uint32_t u4 = 0; u4 < ts / 4; u4++) {
-        const uint32_t v = w[u4];
-        for (int k = 0; k < 4; k++)
-            if (((v >> (8 * k)) & 0xFFu) == t) {
-                const uint32_t ns = next++;
-                const uint32_t nb = kO1Log - highbit32(ns);
-                dt[4 * u4 + k] = DEntry{(uint16_t)((ns << nb) - ts), (uint8_t)t, (uint8_t)nb};
-            }
-    }
+    dtable_parallel(L, kO1Log, dt_all + ((size_t)blockIdx.y * kO1Ctx + blockIdx.x) * (1u << kO1Log));
 }
+
 
 // ---- order-1 tANS decoding split by bit position ----------------------------------------------
 // A chain's bitstream is cut into subsequences of sub_bits bits. Subsequence r owns the symbols

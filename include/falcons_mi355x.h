@@ -124,6 +124,86 @@ int fz_device_free(fz_ctx *ctx, void *p);
 int fz_memcpy(fz_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
 int fz_synchronize(fz_ctx *ctx);
 
+/* Batched form (no counterpart in the reference, which compresses one field per call): many
+ * independent device-resident fields in one call. Each item is exactly one fz_compress_device (or
+ * fz_decompress_device) call and produces the same bytes; the calls run side by side on worker
+ * sub-contexts (own stream and workspace each), largest field first. Returns FZ_OK when every item
+ * succeeded, otherwise the first failing item's status (every item's own status is in the item).
+ * opts (optional): simd in, t_total_ms out (wall time of the batch). */
+typedef struct {
+    fz_config conf;        /* in: configuration; out: as fz_compress_device leaves it */
+    int32_t dtype;         /* FZ_FLOAT / FZ_DOUBLE */
+    int32_t status;        /* out */
+    void *d_data;          /* device field, overwritten with the reconstructed values */
+    uint8_t *d_pre;        /* device output, capacity pre_cap (at least fz_prezstd_bound) */
+    size_t pre_cap;
+    size_t pre_len;        /* out: pre-zstd buffer length */
+} fz_batch_item;
+
+typedef struct {
+    fz_config conf;        /* in: the stream's configuration (fz_read_config) */
+    int32_t dtype;
+    int32_t status;        /* out */
+    const uint8_t *h_pre;  /* host copy of the pre-zstd buffer */
+    const uint8_t *d_pre;  /* device copy, or NULL */
+    size_t pre_len;
+    void *d_out;           /* device field, room for out_cap_elems values */
+    size_t out_cap_elems;
+} fz_batch_dec_item;
+
+/* Number of worker sub-contexts of ctx for the batched calls (default 8, or FALCONS_WORKERS). */
+int fz_set_workers(fz_ctx *ctx, int workers);
+int fz_compress_batch(fz_ctx *ctx, fz_batch_item *items, int n, fz_opts *opts);
+int fz_decompress_batch(fz_ctx *ctx, fz_batch_dec_item *items, int n, fz_opts *opts);
+
+/* Request loop for serving (no counterpart in the reference). Requests are full Falcons_compress /
+ * Falcons_decompress calls on host buffers; they queue in arrival order and each of `slots` workers
+ * takes the oldest request that fits the HBM budget (hbm_budget bytes, 0 = 80% of the free device
+ * memory at creation), counting the workspace and field buffer every slot keeps resident between
+ * requests. Requests overlap on the device; results come back in completion order with their
+ * submit/start/done times (steady clock, ms). The request's host input is not modified (the server
+ * works on a device copy). Buffers stay owned by the caller and must live until the result is
+ * polled. fz_server_destroy finishes the running requests and drops the queued ones. */
+typedef struct {
+    int32_t op;            /* 0 compress (field -> stream), 1 decompress (stream -> field) */
+    int32_t dtype;
+    fz_config conf;        /* compress: the configuration */
+    const void *src;       /* compress: host field (conf.num values); decompress: host stream */
+    size_t src_len;        /* decompress: stream bytes */
+    void *dst;             /* compress: stream buffer (fz_compress_size_bound); decompress: field */
+    size_t dst_cap;        /* bytes */
+    uint64_t tag;          /* returned in the result */
+} fz_request;
+
+typedef struct {
+    uint64_t tag;
+    int32_t op;
+    int32_t status;
+    size_t out_len;        /* compress: stream bytes; decompress: field bytes */
+    size_t field_bytes;
+    double t_submit_ms, t_start_ms, t_done_ms;
+    double t_device_ms;    /* GPU time of the call (as fz_opts) */
+    double t_zstd_ms;
+    char msg[160];         /* error message when status != FZ_OK */
+} fz_result;
+
+typedef struct {
+    uint64_t submitted, completed, failed;
+    uint64_t in_flight, max_in_flight;
+    uint64_t pending, max_pending;
+    uint64_t held_bytes, max_held_bytes, budget_bytes;
+} fz_server_stats;
+
+typedef struct fz_server fz_server;
+int fz_server_create(int device, int slots, size_t hbm_budget, fz_server **out);
+/* FZ_ERR_INVALID_ARGUMENT for a malformed request or one larger than the whole budget. */
+int fz_server_submit(fz_server *s, const fz_request *r);
+/* Takes one finished request: returns 1 and fills *res, or 0 when none finished within timeout_ms
+ * (0 = do not wait, negative = wait until one finishes). */
+int fz_server_poll(fz_server *s, fz_result *res, int timeout_ms);
+int fz_server_get_stats(fz_server *s, fz_server_stats *out);
+void fz_server_destroy(fz_server *s);
+
 /* ZSTD_versionNumber() of the libzstd the host stage loaded (0 if none could be loaded). */
 unsigned fz_zstd_version(void);
 
