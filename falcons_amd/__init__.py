@@ -132,6 +132,8 @@ class fz_server_stats(C.Structure):
                                           "budget_bytes")]
 
 
+_L.fz_tune_report.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int]
+_L.fz_tune_report.restype = C.c_int
 _L.fz_set_workers.argtypes = [C.c_void_p, C.c_int]
 _L.fz_set_workers.restype = C.c_int
 _L.fz_compress_batch.argtypes = [C.c_void_p, C.POINTER(fz_batch_item), C.c_int, C.POINTER(fz_opts)]
@@ -155,7 +157,7 @@ EXPORTED_SYMBOLS = ("fz_ctx_create", "fz_ctx_destroy", "fz_last_error", "fz_opts
                     "fz_zstd_version", "fz_profile_enable", "fz_profile_reset", "fz_profile_count",
                     "fz_profile_entry", "fz_set_workers", "fz_compress_batch", "fz_decompress_batch",
                     "fz_server_create", "fz_server_submit", "fz_server_poll", "fz_server_get_stats",
-                    "fz_server_destroy")
+                    "fz_server_destroy", "fz_tune_report")
 
 
 def zstd_version():
@@ -477,6 +479,12 @@ class Context:
         rc = _L.fz_synchronize(self.h)
         if rc:
             _raise(rc, self._err())
+
+    def tune_report(self):
+        """Ratios of the candidates of the last ALGO_INTERP_LORENZO parameter search on this context."""
+        buf = (C.c_double * 16)()
+        n = _L.fz_tune_report(self.h, buf, 16)
+        return [float(buf[i]) for i in range(min(n, 16))]
 
     def set_workers(self, n):
         rc = _L.fz_set_workers(self.h, int(n))

@@ -422,7 +422,7 @@ struct Timer {
 size_t fz::compress_workspace_bytes(uint64_t n, int dtype) {
     const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
     return n * es * 3 + n * 2 + n * 8 + entropy_encode_bound(n) + select_escapes_scratch(n) +
-           entropy_encode_workspace(n) + (8u << 20);
+           entropy_encode_workspace(n) + tune_workspace_bytes(n, dtype) + (8u << 20);
 }
 
 size_t fz::decompress_workspace_bytes(uint64_t n, int dtype, uint64_t nunpred, size_t pre_len) {
@@ -442,7 +442,7 @@ size_t decomp_head_bytes(int N) { return 8 * (size_t)N + 4 + 4 + 4 + 8 + 8 + 8 +
 // The compression pipeline on the GPU. d_data (device) holds the field and is overwritten with the
 // reconstructed values. The buffer handed to the zstd stage is assembled in device memory, at d_pre
 // when given (capacity pre_cap) or in the context workspace; *pre_out receives its address.
-size_t gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, const void* host_data, int simd,
+size_t gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, int simd,
                     uint8_t* d_pre, size_t pre_cap, uint8_t** pre_out, fz_opts* o) {
     hipStream_t st = ctx->stream;
     const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
@@ -479,10 +479,10 @@ size_t gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, const vo
         static const int def[4] = {4096, 128, 32, 16};
         c.interp_anchor_stride = def[c.N - 1];
     }
+    ctx->tune.n = 0;
     if (c.cmpr_algo == 1) {
         // ALGO_INTERP_LORENZO: parameter search on sampled blocks (FalconsAlgoInterp.hpp:273-507)
-        if (!host_data) throw Unsupported("ALGO_INTERP_LORENZO needs the field on the host");
-        const int r = tune_interp(c, dtype, host_data);
+        const int r = tune_interp(c, dtype, d_data, ws, st, &ctx->tune);
         if (r == 2) throw Unsupported("ALGO_INTERP_LORENZO on 1-D data needs the Lorenzo/regression path");
         c.cmpr_algo = 2;
     }
@@ -687,11 +687,10 @@ extern "C" const char* fz_last_error(const fz_ctx* c) { return c ? c->err.c_str(
 namespace fz {
 
 // Falcons_compress (falcons.hpp:54-97, FalconsDispatcher.hpp:25-133) on a field that is already in
-// device memory. d_data is overwritten with the reconstructed values; host_view (may be null) is
-// a host copy of the original values for the ALGO_INTERP_LORENZO tuner. *mutated tells whether
-// the lossy path ran (the reference's input buffer then holds the reconstruction).
-size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_data, const void* host_view,
-                         uint8_t* out, size_t cap, fz_opts* o, bool* mutated) {
+// device memory. d_data is overwritten with the reconstructed values. *mutated tells whether the
+// lossy path ran (the reference's input buffer then holds the reconstruction).
+size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_data, uint8_t* out, size_t cap,
+                         fz_opts* o, bool* mutated) {
     fz_config c = *conf;
     if (cap < fz_compress_size_bound(&c, dtype)) throw ArgError("The buffer for compressed data is not large enough.");
     check_supported(c, dtype);
@@ -702,13 +701,6 @@ size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_
     const size_t est = fz_config_size_est(&c);
     memset(out, 0, est);
     uint8_t* payload = out + est;
-    std::vector<uint8_t> host_copy;
-    if (c.cmpr_algo == 1 && !host_view) {
-        host_copy.resize(n * es);
-        FZ_HIP(hipMemcpyAsync(host_copy.data(), d_data, n * es, hipMemcpyDeviceToHost, ctx->stream));
-        FZ_HIP(hipStreamSynchronize(ctx->stream));
-        host_view = host_copy.data();
-    }
     size_t plen = 0;
     bool lossless = c.cmpr_algo == 4;
     std::vector<uint8_t> framed;
@@ -722,7 +714,7 @@ size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_
             std::vector<uint8_t> pre;
             try {
                 uint8_t* d_pre = nullptr;
-                const size_t plen_pre = gpu_compress(ctx, c, dtype, d_data, host_view, o->simd, nullptr, 0, &d_pre, o);
+                const size_t plen_pre = gpu_compress(ctx, c, dtype, d_data, o->simd, nullptr, 0, &d_pre, o);
                 pre.resize(plen_pre);
                 FZ_HIP(hipMemcpyAsync(pre.data(), d_pre, plen_pre, hipMemcpyDeviceToHost, ctx->stream));
                 FZ_HIP(hipStreamSynchronize(ctx->stream));
@@ -833,7 +825,7 @@ extern "C" int fz_compress(fz_ctx* ctx, const fz_config* conf, int dtype, void* 
             ~Free() { if (p) (void)hipFree(p); }
         } guard{d_owned};
         bool mutated = false;
-        *out_len = compress_resident(ctx, conf, dtype, d_data, data_on_device ? nullptr : data, out, cap, o, &mutated);
+        *out_len = compress_resident(ctx, conf, dtype, d_data, out, cap, o, &mutated);
         if (mutated && !data_on_device && o->copy_back) {
             FZ_HIP(hipMemcpyAsync(data, d_data, n * es, hipMemcpyDeviceToHost, ctx->stream));
             FZ_HIP(hipStreamSynchronize(ctx->stream));
@@ -881,8 +873,7 @@ extern "C" int fz_compress_device(fz_ctx* ctx, fz_config* conf, int dtype, void*
         check_supported(c, dtype);
         if (c.cmpr_algo == 4 || (c.eb_mode == 0 && c.abs_eb == 0))
             throw Unsupported("the lossless mode has no device-resident form");
-        if (c.cmpr_algo == 1) throw Unsupported("ALGO_INTERP_LORENZO tunes on the host; use fz_compress");
-        *pre_len = gpu_compress(ctx, c, dtype, d_data, nullptr, o->simd, d_pre, pre_cap, nullptr, o);
+        *pre_len = gpu_compress(ctx, c, dtype, d_data, o->simd, d_pre, pre_cap, nullptr, o);
         *conf = c;
         o->t_total_ms = now_ms() - t0;
         return FZ_OK;
@@ -976,4 +967,11 @@ extern "C" int fz_profile_entry(const fz_ctx* ctx, int i, char* name, size_t nam
     if (total_ms) *total_ms = it->second.ms;
     if (bytes) *bytes = it->second.bytes;
     return FZ_OK;
+}
+
+extern "C" int fz_tune_report(const fz_ctx* ctx, double* ratios, int cap) {
+    if (!ctx) return 0;
+    const int n = ctx->tune.n;
+    for (int i = 0; i < n && i < cap; i++) ratios[i] = ctx->tune.ratio[i];
+    return n;
 }

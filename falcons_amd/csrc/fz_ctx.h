@@ -10,6 +10,7 @@
 #include "../../include/falcons_mi355x.h"
 #include "fz_common.h"
 #include "fz_kernels.h"
+#include "fz_tuner.h"
 
 namespace fz {
 
@@ -39,14 +40,15 @@ struct fz_ctx {
     std::string err;
     std::unique_ptr<fz::LaunchProfiler> prof;  // per-launch timing, when switched on
     fz::WorkerSet* workers = nullptr;          // created by fz_set_workers
+    fz::TuneReport tune;                       // the last ALGO_INTERP_LORENZO search on this context
 };
 
 namespace fz {
 
-// Falcons_compress on a field already in device memory (overwritten with the reconstruction);
-// host_view, when given, is a host copy of the original values. *mutated: the lossy path ran.
-size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_data, const void* host_view,
-                         uint8_t* out, size_t cap, fz_opts* o, bool* mutated);
+// Falcons_compress on a field already in device memory (overwritten with the reconstruction).
+// *mutated: the lossy path ran.
+size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_data, uint8_t* out, size_t cap,
+                         fz_opts* o, bool* mutated);
 // Falcons_decompress from a host stream into device memory (*d_out_io; workspace when null).
 void decompress_to_device(fz_ctx* ctx, const uint8_t* cmp, size_t len, int dtype, void** d_out_io,
                           size_t out_cap_elems, fz_config* conf_out, fz_opts* o);

@@ -30,6 +30,11 @@ class DevArena {
         return p;
     }
     size_t used() const { return used_; }
+    // Gives back everything taken after `mark` (a value of used()); the caller has finished with it
+    // in stream order.
+    void rewind(size_t mark) {
+        if (mark <= used_) used_ = mark;
+    }
     size_t capacity() const { return cap_; }
     void release() {
         if (base_) (void)hipFree(base_);

@@ -278,8 +278,8 @@ void run_job(fz_server* S, Slot& s, const Job& j, fz_result& r) {
     if (q.op == 0) {
         FZ_HIP(hipMemcpyAsync(s.field, q.src, j.field_bytes, hipMemcpyHostToDevice, ctx->stream));
         bool mutated = false;
-        r.out_len = compress_resident(ctx, &q.conf, q.dtype, s.field, q.src, static_cast<uint8_t*>(q.dst), q.dst_cap,
-                                      &o, &mutated);
+        r.out_len = compress_resident(ctx, &q.conf, q.dtype, s.field, static_cast<uint8_t*>(q.dst), q.dst_cap, &o,
+                                      &mutated);
     } else {
         fz_config c;
         void* d = s.field;

@@ -207,6 +207,12 @@ void fz_server_destroy(fz_server *s);
 /* ZSTD_versionNumber() of the libzstd the host stage loaded (0 if none could be loaded). */
 unsigned fz_zstd_version(void);
 
+/* Ratios of the candidates evaluated by the last ALGO_INTERP_LORENZO parameter search on this context
+ * (FalconsAlgoInterp.hpp:273-507: interp_compress_test_batch, in evaluation order), up to cap of them;
+ * returns how many there were (0 when the search was skipped). No counterpart call in the reference,
+ * which keeps these values internal. */
+int fz_tune_report(const fz_ctx *ctx, double *ratios, int cap);
+
 /* Launch profiler (no counterpart in the reference: measurement for benchmarks and tuning). When on,
  * every kernel launch of the context's calls is bracketed by HIP events on the context's stream and
  * accumulated per kernel name with its algorithmic bytes (the bytes the kernel has to move at the

@@ -85,6 +85,8 @@ const char *fo_last_error(void);
  * interp_algo / interp_direction / alpha / beta into conf and returns 1 when interpolation with
  * tuned parameters is used, 0 when tuning is skipped. */
 int fo_tune_interp(fo_config *conf, int dtype, const void *data, int omp_threads);
+/* Ratios of the candidates the last fo_tune_interp evaluated, in evaluation order; returns their number. */
+int fo_tune_report(double *out, int cap);
 
 #ifdef __cplusplus
 }

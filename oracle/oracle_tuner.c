@@ -217,10 +217,25 @@ static double tn_ratio(int dtype, void **blocks, size_t nb, size_t side, int N, 
     return (double)(per * nb * es) * 1.0 / (double)cs;
 }
 
+/* ratios of the last search, in evaluation order (for tests) */
+static double g_tune_ratio[16];
+static int g_tune_n;
+
+static double tn_rec(double r) {
+    if (g_tune_n < 16) g_tune_ratio[g_tune_n++] = r;
+    return r;
+}
+
+int fo_tune_report(double *out, int cap) {
+    for (int i = 0; i < g_tune_n && i < cap; i++) out[i] = g_tune_ratio[i];
+    return g_tune_n;
+}
+
 #define VAL(dtype, data, idx) ((dtype) ? ((const double *)(data))[idx] : (double)((const float *)(data))[idx])
 
 int fo_tune_interp(fo_config *c, int dtype, const void *data, int omp_threads) {
     int N = c->N;
+    g_tune_n = 0;
     if (c->interp_anchor_stride < 0) {
         static const int def[4] = {4096, 128, 32, 16};
         c->interp_anchor_stride = def[N - 1];
@@ -339,7 +354,7 @@ int fo_tune_interp(fo_config *c, int dtype, const void *data, int omp_threads) {
     if (N >= 3) {
         for (int i = 0; i < 4; i++) {
             int cub = (i / 2 == 0) ? 0 : 1, dir = (i % 2 == 0) ? 0 : reverse;
-            double r = tn_ratio(dtype, blocks, nsel, side, N, cub, dir, anchor, c->interp_alpha, c->interp_beta, c->abs_eb);
+            double r = tn_rec(tn_ratio(dtype, blocks, nsel, side, N, cub, dir, anchor, c->interp_alpha, c->interp_beta, c->abs_eb));
             if (r > best) { best = r; c->interp_algo = cub; c->interp_direction = dir; }
         }
     } else if (omp_threads >= 6) {
@@ -347,19 +362,19 @@ int fo_tune_interp(fo_config *c, int dtype, const void *data, int omp_threads) {
         for (int i = 0; i < 8; i++) {
             int ad = i % 4, ab = i / 4;
             int cub = (ad / 2 == 0) ? 0 : 1, dir = (ad % 2 == 0) ? 0 : reverse;
-            double r = tn_ratio(dtype, blocks, nsel, side, N, cub, dir, anchor, al[ab], be[ab], c->abs_eb);
+            double r = tn_rec(tn_ratio(dtype, blocks, nsel, side, N, cub, dir, anchor, al[ab], be[ab], c->abs_eb));
             if (r > best) { best = r; c->interp_algo = cub; c->interp_direction = dir; c->interp_alpha = al[ab]; c->interp_beta = be[ab]; }
         }
     } else {
         for (int i = 0; i < 4; i++) {
             int cub = (i / 2 == 0) ? 0 : 1, dir = (i % 2 == 0) ? 0 : reverse;
-            double r = tn_ratio(dtype, blocks, nsel, side, N, cub, dir, anchor, c->interp_alpha, c->interp_beta, c->abs_eb);
+            double r = tn_rec(tn_ratio(dtype, blocks, nsel, side, N, cub, dir, anchor, c->interp_alpha, c->interp_beta, c->abs_eb));
             if (r > best) { best = r; c->interp_algo = cub; c->interp_direction = dir; }
         }
         static const double al[3] = {1.0, 1.25, 1.5}, be[3] = {1.0, 2.0, 2.5};
         double abr[3];
         for (int i = 1; i < 3; i++)
-            abr[i] = tn_ratio(dtype, blocks, nsel, side, N, c->interp_algo, c->interp_direction, anchor, al[i], be[i], c->abs_eb);
+            abr[i] = tn_rec(tn_ratio(dtype, blocks, nsel, side, N, c->interp_algo, c->interp_direction, anchor, al[i], be[i], c->abs_eb));
         for (int i = 1; i < 3; i++)
             if (abr[i] > best * 1.02) { best = abr[i]; c->interp_alpha = al[i]; c->interp_beta = be[i]; }
     }

@@ -85,6 +85,8 @@ class Oracle:
         L.fo_set_simd.argtypes = [C.c_int]
         L.fo_tune_interp.restype = C.c_int
         L.fo_tune_interp.argtypes = [C.POINTER(FoConfig), C.c_int, C.c_void_p, C.c_int]
+        L.fo_tune_report.restype = C.c_int
+        L.fo_tune_report.argtypes = [C.POINTER(C.c_double), C.c_int]
         L.fo_o1_histogram.argtypes = [_i16p, C.c_size_t, C.POINTER(C.c_uint32), _i16p, _szp]
         L.fo_o1_ctx.restype = C.c_int
         L.fo_o1_sym.restype = C.c_int
