@@ -625,7 +625,7 @@ void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h
     Timer tm{st, ctx->ev0, ctx->ev1};
     tm.start();
     if (d_pre && nun) FZ_HIP(hipMemcpyAsync(d_unpred, d_pre + un_off, (size_t)nun * es, hipMemcpyDeviceToDevice, st));
-    entropy_decode(h_pre + enc_off, d_enc, enc_len, ncodes, codes, ws, st);
+    entropy_decode(h_pre + enc_off, d_enc, enc_len, ncodes, codes, ws, st, ctx->aux);
     select_escapes(codes, ncodes, esc_idx, nesc_dev, sel_tmp, sel_bytes, st);
     uint64_t nesc = 0;
     FZ_HIP(hipMemcpyAsync(&nesc, nesc_dev, 8, hipMemcpyDeviceToHost, st));
@@ -661,6 +661,9 @@ extern "C" int fz_ctx_create(int device, fz_ctx** out) {
         FZ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         FZ_HIP(hipEventCreate(&c->ev0));
         FZ_HIP(hipEventCreate(&c->ev1));
+        FZ_HIP(hipStreamCreateWithFlags(&c->aux.stream, hipStreamNonBlocking));
+        FZ_HIP(hipEventCreateWithFlags(&c->aux.fork, hipEventDisableTiming));
+        FZ_HIP(hipEventCreateWithFlags(&c->aux.join, hipEventDisableTiming));
         *out = c;
         return FZ_OK;
     } catch (const std::exception& e) {
@@ -678,6 +681,9 @@ extern "C" void fz_ctx_destroy(fz_ctx* c) {
     c->arena.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->aux.fork) (void)hipEventDestroy(c->aux.fork);
+    if (c->aux.join) (void)hipEventDestroy(c->aux.join);
+    if (c->aux.stream) (void)hipStreamDestroy(c->aux.stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

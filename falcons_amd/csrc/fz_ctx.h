@@ -36,6 +36,7 @@ struct fz_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    fz::AuxStream aux;                         // second stream for work beside the main one
     fz::DevArena arena;
     std::string err;
     std::unique_ptr<fz::LaunchProfiler> prof;  // per-launch timing, when switched on

@@ -16,6 +16,8 @@
 #include <stdexcept>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "fz_common.h"
 #include "fz_fse.h"
 #include "fz_kernels.h"
@@ -75,7 +77,8 @@ __device__ inline uint64_t bw_close(BitW& w) {
     }
     return w.pos;
 }
-__device__ inline void fse_encode(BitW& w, uint32_t& state, const CTable& ct, unsigned sym) {
+template <class CT>
+__device__ inline void fse_encode(BitW& w, uint32_t& state, const CT& ct, unsigned sym) {
     const uint32_t nbOut = (uint32_t)(((uint64_t)state + ct.deltaNbBits[sym]) >> 16);
     bw_add(w, state, nbOut);
     state = ct.stateTable[(state >> nbOut) + ct.deltaFindState[sym]];
@@ -115,23 +118,25 @@ struct BitWin {
     int64_t lo;
     uint64_t w;
 };
-__device__ inline void bwin_fill(BitWin& r) {
+__device__ __forceinline__ void bwin_fill(BitWin& r) {
     const int64_t b0 = (r.pos - 57) >> 3;  // floor for negatives
+    // eight independent loads (addresses clamped, bytes below the start masked to zero) issued
+    // together, so a refill costs one memory latency
+    uint32_t v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = r.buf[b0 + j < 0 ? 0 : b0 + j];
     uint64_t w = 0;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const int64_t b = b0 + j;
-        if (b >= 0) w |= (uint64_t)r.buf[b] << (8 * j);
-    }
+    for (int j = 0; j < 8; j++) w |= (uint64_t)(b0 + j < 0 ? 0u : v[j]) << (8 * j);
     r.w = w;
     r.lo = b0 * 8;
 }
-__device__ inline BitWin bwin_at(const uint8_t* buf, int64_t pos) {
+__device__ __forceinline__ BitWin bwin_at(const uint8_t* buf, int64_t pos) {
     BitWin r{buf, pos, 0, 0};
     bwin_fill(r);
     return r;
 }
-__device__ inline uint32_t bwin_read(BitWin& r, uint32_t nb) {
+__device__ __forceinline__ uint32_t bwin_read(BitWin& r, uint32_t nb) {
     const int64_t p = r.pos - nb;
     if (p < r.lo) {
         r.pos = p + nb;
@@ -192,17 +197,20 @@ static inline unsigned nblk(uint64_t n, unsigned bs = 256) { return (unsigned)((
 // that lies at or below the high threshold. The walk is a permutation (step is odd), so it is
 // enumerated in parallel and ranked with a block scan. Within a symbol, positions in increasing
 // order get consecutive states: one thread per symbol scans the spread table.
-struct TableLds {
-    uint8_t sym[1u << kFseMaxTableLog];
+template <unsigned MaxLog>
+struct TableLdsT {
     int16_t norm[256];          // symbols above maxSym hold 0
     uint32_t cum[257];          // exclusive prefix of the counts (-1 counts as 1)
     uint32_t cump[257];         // exclusive prefix of the positive counts
     uint32_t part[256];
     int32_t high;               // last position of the walk part; -1 when every count is -1
     int bad;
+    uint8_t sym[1u << MaxLog];
 };
+using TableLds = TableLdsT<kFseMaxTableLog>;
 
-__device__ bool spread_parallel(TableLds& L, unsigned tl) {
+template <class TL>
+__device__ bool spread_parallel(TL& L, unsigned tl) {
     const unsigned t = threadIdx.x;
     const uint32_t ts = 1u << tl, mask = ts - 1, step = (ts >> 1) + (ts >> 3) + 3;
     if (t == 0) {
@@ -259,7 +267,8 @@ __device__ bool spread_parallel(TableLds& L, unsigned tl) {
 }
 
 // FSE_buildCTable_wksp (fse_compress.c:65-155) after spread_parallel; thread t handles symbol t.
-__device__ void ctable_parallel(const TableLds& L, unsigned tl, unsigned maxSym, CTable* ct) {
+template <class TL, class CT>
+__device__ void ctable_parallel(const TL& L, unsigned tl, unsigned maxSym, CT* ct) {
     const unsigned t = threadIdx.x;
     const uint32_t ts = 1u << tl;
     const int nc = L.norm[t];
@@ -287,7 +296,8 @@ __device__ void ctable_parallel(const TableLds& L, unsigned tl, unsigned maxSym,
 }
 
 // FSE_buildDTable (fse_decompress.c:70-127) after spread_parallel; thread t handles symbol t.
-__device__ void dtable_parallel(const TableLds& L, unsigned tl, DEntry* dt) {
+template <class TL>
+__device__ void dtable_parallel(const TL& L, unsigned tl, DEntry* dt) {
     const unsigned t = threadIdx.x;
     const uint32_t ts = 1u << tl;
     const int nc = L.norm[t];
@@ -306,22 +316,31 @@ __device__ void dtable_parallel(const TableLds& L, unsigned tl, DEntry* dt) {
 }
 
 // ---------------------------------------------------------------- FSE block (workgroup-wide)
-// FSE_compress of one block of at most 64 K bytes (fse_compress.c:632-694), run by one workgroup:
-// histogram, spread and table build are shared by the workgroup; normalization, the table header
-// and the two-state encode run on lane 0.
+// FSE_compress of one block of at most 64 K bytes (fse_compress.c:632-694), run by one workgroup.
+// The block is staged in LDS, histogram, spread and table build are shared by the workgroup, and
+// normalization, the table header and the two-state encode run on lane 0 against LDS only (its
+// input and its output bytes); the framed result is copied out by the whole workgroup.
+// FSE_compress bounds the table log by FSE_DEFAULT_TABLELOG (11).
+constexpr unsigned kFblkLog = kFseDefaultTableLog;
+
 struct FseLds {
+    uint8_t in[kFblk];
+    uint8_t out[kFblk + 1024];
     uint32_t count[256];
-    TableLds tab;
-    CTable ct;
+    TableLdsT<kFblkLog> tab;
+    CTableT<kFblkLog> ct;
     uint64_t result, nc;
     uint32_t tl, maxSym;
 };
 
-__device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_t* dst, FseLds& L) {
+// Leaves the FSE bytes (when the result is > 1) in L.out. Returns 0 (not compressible), 1 (one
+// symbol), kFseErr, or the compressed size.
+__device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, FseLds& L) {
     constexpr uint64_t kPending = ~1ull;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) L.in[i] = src[i];
     for (int s = threadIdx.x; s < 256; s += blockDim.x) L.count[s] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&L.count[src[i]], 1u);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&L.count[L.in[i]], 1u);
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t r = kPending;
@@ -337,10 +356,10 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_
             else {
                 const unsigned tl = optimal_table_log(kFseDefaultTableLog, n, maxSym);
                 for (unsigned s = maxSym + 1; s < 256; s++) L.tab.norm[s] = 0;
-                if (normalize_count(L.tab.norm, tl, L.count, n, maxSym) == kFseErr) {
+                if (tl > kFblkLog || normalize_count(L.tab.norm, tl, L.count, n, maxSym) == kFseErr) {
                     r = kFseErr;
                 } else {
-                    L.nc = write_ncount(dst, L.tab.norm, maxSym, tl);
+                    L.nc = write_ncount(L.out, L.tab.norm, maxSym, tl);
                     L.tl = tl;
                     L.maxSym = maxSym;
                 }
@@ -360,47 +379,68 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint64_t nc = L.nc;
-        BitW w{dst + nc, 0, 0, 0};
+        const uint8_t* in = L.in;
+        BitW w{L.out + nc, 0, 0, 0};
         uint32_t ip = n, s1, s2;
         if (n & 1) {
-            s1 = init_state2(&L.ct, src[--ip]);
-            s2 = init_state2(&L.ct, src[--ip]);
-            fse_encode(w, s1, L.ct, src[--ip]);
+            s1 = init_state2(&L.ct, in[--ip]);
+            s2 = init_state2(&L.ct, in[--ip]);
+            fse_encode(w, s1, L.ct, in[--ip]);
         } else {
-            s2 = init_state2(&L.ct, src[--ip]);
-            s1 = init_state2(&L.ct, src[--ip]);
+            s2 = init_state2(&L.ct, in[--ip]);
+            s1 = init_state2(&L.ct, in[--ip]);
         }
         if ((n - 2) & 2) {
-            fse_encode(w, s2, L.ct, src[--ip]);
-            fse_encode(w, s1, L.ct, src[--ip]);
+            fse_encode(w, s2, L.ct, in[--ip]);
+            fse_encode(w, s1, L.ct, in[--ip]);
         }
+        // a block whose output reaches n - 1 bytes is stored raw: stop there (the LDS buffer holds
+        // n + 1024 bytes; one round writes at most 8)
+        // the two states are independent chains: each round forms both output widths, writes the
+        // bits in the reference's order (s2 then s1), then looks both next states up together
+        bool full = false;
+        const CTableT<kFblkLog>& ct = L.ct;
         while (ip > 0) {
-            fse_encode(w, s2, L.ct, src[--ip]);
-            fse_encode(w, s1, L.ct, src[--ip]);
-            fse_encode(w, s2, L.ct, src[--ip]);
-            fse_encode(w, s1, L.ct, src[--ip]);
+            if (nc + w.pos >= n) { full = true; break; }
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const unsigned a = in[ip - 1], b = in[ip - 2];
+                ip -= 2;
+                const uint32_t da = ct.deltaNbBits[a], db = ct.deltaNbBits[b];
+                const int32_t fa = ct.deltaFindState[a], fb = ct.deltaFindState[b];
+                const uint32_t na = (uint32_t)(((uint64_t)s2 + da) >> 16), nbb = (uint32_t)(((uint64_t)s1 + db) >> 16);
+                bw_add(w, s2, na);
+                bw_add(w, s1, nbb);
+                const uint32_t t2 = ct.stateTable[(s2 >> na) + fa], t1 = ct.stateTable[(s1 >> nbb) + fb];
+                s2 = t2;
+                s1 = t1;
+            }
         }
-        bw_add(w, s2, tl);
-        bw_add(w, s1, tl);
-        const uint64_t bs = bw_close(w);
-        const uint64_t total = nc + bs;
-        L.result = (total >= n - 1) ? 0 : total;
+        if (full) {
+            L.result = 0;
+        } else {
+            bw_add(w, s2, tl);
+            bw_add(w, s1, tl);
+            const uint64_t bs = bw_close(w);
+            const uint64_t total = nc + bs;
+            L.result = (total >= n - 1) ? 0 : total;
+        }
     }
     __syncthreads();
     return L.result;
 }
 
 // Frames one class-byte block as in RleFse:150-176: [0][raw] | [2][sym] | [1][u32 len][FSE bytes].
-// tmp holds the FSE attempt; returns the framed length written to out.
-__device__ uint64_t frame_block_wg(const uint8_t* src, uint32_t len, uint8_t* tmp, uint8_t* out, FseLds& L) {
-    const uint64_t r = fse_block_compress_wg(src, len, tmp, L);
+// Returns the framed length written to out.
+__device__ uint64_t frame_block_wg(const uint8_t* src, uint32_t len, uint8_t* out, FseLds& L) {
+    const uint64_t r = fse_block_compress_wg(src, len, L);
     uint64_t sz;
     if (r == kFseErr || r == 0 || r >= len) {
         if (threadIdx.x == 0) out[0] = 0;
-        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) out[1 + i] = src[i];
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) out[1 + i] = L.in[i];
         sz = 1 + len;
     } else if (r == 1) {
-        if (threadIdx.x == 0) { out[0] = 2; out[1] = src[0]; }
+        if (threadIdx.x == 0) { out[0] = 2; out[1] = L.in[0]; }
         sz = 2;
     } else {
         if (threadIdx.x == 0) {
@@ -408,7 +448,7 @@ __device__ uint64_t frame_block_wg(const uint8_t* src, uint32_t len, uint8_t* tm
             const uint32_t rr = (uint32_t)r;
             out[1] = (uint8_t)rr; out[2] = (uint8_t)(rr >> 8); out[3] = (uint8_t)(rr >> 16); out[4] = (uint8_t)(rr >> 24);
         }
-        for (uint32_t i = threadIdx.x; i < r; i += blockDim.x) out[5 + i] = tmp[i];
+        for (uint32_t i = threadIdx.x; i < r; i += blockDim.x) out[5 + i] = L.out[i];
         sz = 5 + r;
     }
     __syncthreads();
@@ -416,13 +456,12 @@ __device__ uint64_t frame_block_wg(const uint8_t* src, uint32_t len, uint8_t* tm
 }
 
 // One workgroup per 64 K block of a class-byte array; framed blocks land in fixed slots.
-__global__ void __launch_bounds__(256) k_fse_blocks(const uint8_t* src, uint64_t n, uint8_t* tmp, uint8_t* slots,
-                                                    uint64_t* sizes) {
+__global__ void __launch_bounds__(256) k_fse_blocks(const uint8_t* src, uint64_t n, uint8_t* slots, uint64_t* sizes) {
     __shared__ FseLds L;
     const uint64_t b = blockIdx.x;
     const uint64_t off = b * kFblk;
     const uint32_t len = (uint32_t)min<uint64_t>(kFblk, n - off);
-    const uint64_t sz = frame_block_wg(src + off, len, tmp + b * (kFblk + 1024), slots + b * (kFblk + 1024), L);
+    const uint64_t sz = frame_block_wg(src + off, len, slots + b * (kFblk + 1024), L);
     if (threadIdx.x == 0) sizes[b] = sz;
 }
 
@@ -436,13 +475,46 @@ __global__ void k_pack_slots(const uint8_t* slots, uint64_t slot_stride, const u
 }
 
 // FSE_decompress of one block (fse_decompress.c:254-282): the workgroup reads the header and builds
-// the decoding table, lane 0 decodes.
-struct FseDecLds {
-    TableLds tab;
+// the decoding table; lane 0 decodes. Tables up to log 11 (all that FSE_compress writes) run with
+// the compressed bytes and the decoded bytes in LDS, copied in and out by the whole workgroup;
+// logs 12 and 13 read and write device memory directly.
+struct FseDecSmall {
+    TableLdsT<kFblkLog> tab;
+    DEntry dt[1u << kFblkLog];
+    uint8_t in[kFblk + 64];
+    uint8_t out[kFblk];
+};
+struct FseDecBig {
+    TableLdsT<kFseMaxTableLog> tab;
     DEntry dt[1u << kFseMaxTableLog];
+};
+struct FseDecLds {
+    union {
+        FseDecSmall s;
+        FseDecBig g;
+    };
+    int16_t norm[256];
     uint32_t tl;
     uint64_t hl;
 };
+
+// The two-state FSE decode loop (fse_decompress.c:254-282 via FSE_decompress_usingDTable).
+// The two states decode alternate symbols; their table entries are read together.
+__device__ __forceinline__ void fse_decode_run(const DEntry* dt, const uint8_t* bs, uint64_t bsz, unsigned tl,
+                                               uint8_t* dst, uint32_t len) {
+    BitWin r = bwin_at(bs, (int64_t)(bsz - 1) * 8 + (int64_t)highbit32(bs[bsz - 1]));
+    uint32_t s0 = bwin_read(r, tl);
+    uint32_t s1 = bwin_read(r, tl);
+    uint32_t i = 0;
+    for (; i + 2 <= len; i += 2) {
+        const DEntry e0 = dt[s0], e1 = dt[s1];
+        dst[i] = e0.symbol;
+        dst[i + 1] = e1.symbol;
+        s0 = e0.newState + bwin_read(r, e0.nbBits);
+        s1 = e1.newState + bwin_read(r, e1.nbBits);
+    }
+    if (i < len) dst[i] = dt[s0].symbol;
+}
 
 // Decodes framed class-byte blocks; block offsets were found by the host.
 __global__ void __launch_bounds__(256) k_fse_dec_blocks(const uint8_t* in, const uint64_t* offs, uint64_t n,
@@ -466,8 +538,8 @@ __global__ void __launch_bounds__(256) k_fse_dec_blocks(const uint8_t* in, const
                              ((uint32_t)in[o + 4] << 24);
     if (threadIdx.x == 0) {
         unsigned maxSym = 255, tl = 0;
-        L.hl = read_ncount(L.tab.norm, &maxSym, &tl, src, srcSize);
-        for (unsigned s = maxSym + 1; s < 256; s++) L.tab.norm[s] = 0;
+        L.hl = read_ncount(L.norm, &maxSym, &tl, src, srcSize);
+        for (unsigned s = maxSym + 1; s < 256; s++) L.norm[s] = 0;
         L.tl = tl;
     }
     __syncthreads();
@@ -477,29 +549,41 @@ __global__ void __launch_bounds__(256) k_fse_dec_blocks(const uint8_t* in, const
         return;
     }
     const unsigned tl = L.tl;
-    if (!spread_parallel(L.tab, tl)) {
+    const uint64_t bsz = srcSize - hl;
+    if (tl <= kFblkLog && bsz <= kFblk) {
+        FseDecSmall& S = L.s;
+        S.tab.norm[threadIdx.x] = L.norm[threadIdx.x];
+        for (uint32_t i = threadIdx.x; i < bsz; i += blockDim.x) S.in[i] = src[hl + i];
+        __syncthreads();
+        if (!spread_parallel(S.tab, tl)) {
+            if (threadIdx.x == 0) *err = 1;
+            return;
+        }
+        dtable_parallel(S.tab, tl, S.dt);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (S.in[bsz - 1] == 0) *err = 1;
+            else fse_decode_run(S.dt, S.in, bsz, tl, S.out, len);
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = S.out[i];
+        return;
+    }
+    FseDecBig& G = L.g;
+    G.tab.norm[threadIdx.x] = L.norm[threadIdx.x];
+    __syncthreads();
+    if (!spread_parallel(G.tab, tl)) {
         if (threadIdx.x == 0) *err = 1;
         return;
     }
-    dtable_parallel(L.tab, tl, L.dt);
+    dtable_parallel(G.tab, tl, G.dt);
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const uint8_t* bs = src + hl;
-    const uint64_t bsz = srcSize - hl;
-    if (bs[bsz - 1] == 0) {
+    if (src[hl + bsz - 1] == 0) {
         *err = 1;
         return;
     }
-    BitWin r = bwin_at(bs, (int64_t)(bsz - 1) * 8 + (int64_t)highbit32(bs[bsz - 1]));
-    uint32_t st[2];
-    st[0] = bwin_read(r, tl);
-    st[1] = bwin_read(r, tl);
-    for (uint32_t i = 0; i < len; i++) {
-        uint32_t& s = st[i & 1];
-        const DEntry e = L.dt[s];
-        dst[i] = e.symbol;
-        s = e.newState + bwin_read(r, e.nbBits);
-    }
+    fse_decode_run(G.dt, src + hl, bsz, tl, dst, len);
 }
 
 // ---------------------------------------------------------------- method 0 (RleFse:928-982)
@@ -912,7 +996,18 @@ __global__ void __launch_bounds__(256) k_o1_dtables(const uint8_t* stream, const
         if (t == 0) *err = 1;
         return;
     }
-    dtable_parallel(L, kO1Log, dt_all + ((size_t)blockIdx.y * kO1Ctx + blockIdx.x) * (1u << kO1Log));
+    DEntry* dt = dt_all + ((size_t)blockIdx.y * kO1Ctx + blockIdx.x) * (1u << kO1Log);
+    dtable_parallel(L, kO1Log, dt);
+    __syncthreads();
+    // the context that follows each entry's symbol rides in the top four bits of nbBits (15 after an
+    // escape, whose context depends on the escaped value); readers mask nbBits with 15
+    for (uint32_t u = t; u < (1u << kO1Log); u += blockDim.x) {
+        DEntry d = dt[u];
+        const unsigned sy = d.symbol;
+        const uint32_t nc = sy == 255 ? 15u : (uint32_t)o1_ctx((int)((sy >> 1) ^ (~(sy & 1) + 1)));
+        d.nbBits = (uint8_t)(d.nbBits | (nc << 4));
+        dt[u] = d;
+    }
 }
 
 
@@ -966,7 +1061,7 @@ __device__ inline int o1d_chain_of(const O1DecChain* ch, int nch, uint64_t t) {
 __device__ inline int o1d_step(const DEntry* dt, BitWin& r, uint32_t& X, uint32_t& ctx, uint64_t& e,
                                const int16_t* ovf, uint64_t ovf_lim, bool& bad) {
     const DEntry d = dt[(size_t)ctx * (1u << kO1Log) + (X & ((1u << kO1Log) - 1))];
-    X = d.newState + bwin_read(r, d.nbBits);
+    X = d.newState + bwin_read(r, d.nbBits & 15u);
     int code;
     if (d.symbol == 255) {
         if (e < ovf_lim) code = ovf[e];
@@ -979,6 +1074,106 @@ __device__ inline int o1d_step(const DEntry* dt, BitWin& r, uint32_t& X, uint32_
     ctx = (uint32_t)o1_ctx(code);
     return code;
 }
+
+// Sequential decoding of a stretch of one chain from a bitstream window staged in LDS, for the
+// walk and for dense chains. win[k + kWinPad] holds chain byte wlo + k; the kWinPad bytes below are
+// zero when the window starts at the chain's first byte. The decoding tables of contexts
+// kWalkCtx0 .. kWalkCtx0 + kWalkCtxN - 1 sit in LDS (hot), the others are read from device memory.
+constexpr int kWalkCtx0 = 11, kWalkCtxN = 4;  // contexts whose decoding tables are kept in LDS
+constexpr int kWinPad = 16;
+constexpr int kDenseStage = 12288;  // dense chains: bitstream bytes per window
+constexpr int kDenseOut = 4096;     // dense chains: codes per round
+constexpr int kDenseOvf = 2048;     // dense chains: escaped values per round
+
+// LDS pointers carry their address space so that every access is an LDS instruction.
+#define FZ_LDS __attribute__((address_space(3)))
+template <typename T>
+__device__ __forceinline__ FZ_LDS T* lds(T* p) { return (FZ_LDS T*)p; }
+
+struct SeqState {
+    int64_t B;      // read position (bits below are unread)
+    uint32_t X, cx;
+    uint64_t e;     // escapes consumed
+    bool bad;
+};
+
+// Decodes while the read position stays above `until` (bits), at most `max_n` symbols. Escaped
+// codes come from vbuf[e - ve0] (LDS) when vbuf is given, decoding stopping after the escape that
+// reaches ve1 unless ve1 is the end of the chain's escapes; without vbuf they are read from ov.
+// Writes each code to obuf[k] when obuf is given. Returns the symbols decoded.
+//
+// Per symbol: the table entry (state, context) and the two window words that hold the 32 bits below
+// the read position are independent LDS reads; the entry gives the bit count, the next state base,
+// the symbol and the next context (top nibble of nbBits; 15 after an escape). Positions are 32-bit
+// offsets into the window. One LDS round trip and a few dozen scalar operations per symbol.
+__device__ __forceinline__ uint64_t seq_decode(SeqState& S, const FZ_LDS uint8_t* win, int64_t wlo, int64_t until,
+                                               uint64_t max_n, const FZ_LDS uint32_t* hot, const uint32_t* dtg,
+                                               const FZ_LDS int16_t* vbuf, uint64_t ve0, uint64_t ve1,
+                                               const int16_t* ov, uint64_t novf, FZ_LDS int16_t* obuf) {
+    const FZ_LDS uint32_t* w32 = reinterpret_cast<const FZ_LDS uint32_t*>(win);
+    const int64_t base = (wlo - kWinPad) * 8;  // bit 0 of the window
+    int32_t p = (int32_t)(S.B - base);
+    // below 32 bits into the window the word reads would leave it
+    const bool raised = until < base + 32;
+    const int32_t lim = (int32_t)((raised ? base + 32 : until) - base);
+    uint32_t X = S.X, cx = S.cx;
+    uint64_t e = S.e;
+    const uint64_t elim = (vbuf && ve1 < novf) ? ve1 : ~0ull;
+    const uint32_t kmax = (uint32_t)min<uint64_t>(max_n, 0x7FFFFFFFull);
+    uint32_t k = 0;
+    while (p > lim && k < kmax) {
+        const uint32_t xi = X & ((1u << kO1Log) - 1);
+        const uint32_t hc = cx - kWalkCtx0;
+        const int32_t j = (p - 32) >> 5;  // words j, j+1 hold bits [32 j, 32 j + 64), which contain [p - 32, p)
+        uint32_t raw = hot[(min(hc, (uint32_t)kWalkCtxN - 1) << kO1Log) + xi];
+        const uint64_t w = (uint64_t)w32[j] | ((uint64_t)w32[j + 1] << 32);
+        if (hc >= (uint32_t)kWalkCtxN) raw = dtg[((size_t)cx << kO1Log) + xi];
+        const uint32_t nb = (raw >> 24) & 15u, sym = (raw >> 16) & 0xFFu;
+        p -= (int32_t)nb;
+        X = (raw & 0xFFFFu) + ((uint32_t)(w >> (uint32_t)(p - 32 * j)) & ((1u << nb) - 1u));
+        int code = (int)((sym >> 1) ^ (~(sym & 1) + 1));
+        cx = raw >> 28;
+        if (cx == 15u) {
+            if (e >= novf) { S.bad = true; code = 32767; }
+            else code = vbuf ? (int)vbuf[e - ve0 < (uint64_t)kDenseOvf ? e - ve0 : 0] : (int)ov[e];
+            cx = (uint32_t)o1_ctx(code);
+            e++;
+            if (e >= elim) {
+                if (obuf) obuf[k] = (int16_t)code;
+                k++;
+                break;
+            }
+        }
+        if (obuf) obuf[k] = (int16_t)code;
+        k++;
+    }
+    // stopped below the staged bytes while symbols remain: the stream runs past its first bit
+    if (raised && p <= lim && k < kmax && (vbuf == nullptr || e < elim)) S.bad = true;
+    S.B = base + p;
+    S.X = X;
+    S.cx = cx;
+    S.e = e;
+    return k;
+}
+
+// Copies chain bytes [lo - kWinPad, hi) into win (bytes below the chain's start read as zero), eight
+// independent loads per lane per round.
+__device__ __forceinline__ void stage_window(FZ_LDS uint8_t* win, const uint8_t* cb, int64_t lo, int64_t hi) {
+    const int64_t base = lo - kWinPad, len = hi - base;
+    for (int64_t k0 = (int64_t)threadIdx.x * 8; k0 < len; k0 += (int64_t)blockDim.x * 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int64_t i = base + k0 + j;
+            v[j] = (k0 + j < len && i >= 0) ? cb[i] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (k0 + j < len) win[k0 + j] = (uint8_t)v[j];
+    }
+}
+
+constexpr int kSeqLanes = 64;  // the sequential decoders run as one wave
 
 // Graph capacity. Where escapes are dense the branches that follow wrong signs keep meeting
 // spurious escapes and the graph grows without bound; such subsequences give up early and are
@@ -1117,7 +1312,7 @@ __global__ void __launch_bounds__(kGLanes) k_o1d_graph(const uint8_t* in, const 
             bool esc = false;
             while (rd.pos > stop && cnt < cap) {
                 const DEntry d = dt[(size_t)cx * (1u << kO1Log) + (X & ((1u << kO1Log) - 1))];
-                X = d.newState + bwin_read(rd, d.nbBits);
+                X = d.newState + bwin_read(rd, d.nbBits & 15u);
                 cnt++;
                 if (d.symbol == 255) { esc = true; break; }
                 const unsigned sy = d.symbol;
@@ -1138,7 +1333,7 @@ __global__ void __launch_bounds__(kGLanes) k_o1d_graph(const uint8_t* in, const 
         const uint32_t cx0 = bC[i][l], X0 = bX[i][l];
         const DEntry d = dt[(size_t)cx0 * (1u << kO1Log) + (X0 & ((1u << kO1Log) - 1))];
         BitWin rd = bwin_at(buf, bB[i][l]);
-        const uint32_t X = d.newState + bwin_read(rd, d.nbBits);
+        const uint32_t X = d.newState + bwin_read(rd, d.nbBits & 15u);
         const int64_t B = rd.pos;
         bB[i][l] = B; bX[i][l] = X; bN[i][l] = bN[i][l] + 1;
         if (d.symbol == 255) {
@@ -1192,78 +1387,131 @@ __global__ void __launch_bounds__(256) k_o1d_glink(const O1DecChain* ch, int nch
 // graph (capacity exceeded, or a start state missing from its set) is decoded exactly here, by one
 // lane, with the tables of the contexts after large codes (11..14, most of the symbols where
 // escapes are dense) staged in LDS.
-constexpr int kWalkCtx0 = 11, kWalkCtxN = 4;
+constexpr int64_t kWalkStage = 16384;          // bitstream bytes staged per window
 
-__global__ void __launch_bounds__(256) k_o1d_walk(const uint8_t* in, const O1DecChain* ch, int nch,
-                                                  const DEntry* dt_all, const int16_t* ovf, const O1StartSet* start,
-                                                  const O1Graph* graph, O1DecState* res_start, uint64_t* res_e,
-                                                  uint64_t* res_off, unsigned int* stats, int* err) {
-    __shared__ DEntry hot[kWalkCtxN << kO1Log];
+__global__ void __launch_bounds__(kSeqLanes) k_o1d_walk(const uint8_t* in, const O1DecChain* ch, int nch,
+                                                        const DEntry* dt_all, const int16_t* ovf,
+                                                        const O1StartSet* start, const O1Graph* graph,
+                                                        O1DecState* res_start, uint64_t* res_e, uint64_t* res_off,
+                                                        unsigned int* stats, int* err) {
+    __shared__ uint32_t hot[kWalkCtxN << kO1Log];
+    __shared__ uint32_t stage32[(kWalkStage + kWinPad) / 4 + 2];
+    uint8_t* stage = reinterpret_cast<uint8_t*>(stage32);
     const int q = blockIdx.x;
     if (q >= nch) return;
     const O1DecChain c = ch[q];
-    const DEntry* dt = dt_all + c.dset * kO1Ctx * (1u << kO1Log);
+    const uint32_t* dt = reinterpret_cast<const uint32_t*>(dt_all + c.dset * kO1Ctx * (1u << kO1Log));
     for (uint32_t i = threadIdx.x; i < (kWalkCtxN << kO1Log); i += blockDim.x)
         hot[i] = dt[((size_t)kWalkCtx0 << kO1Log) + i];
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    const uint8_t* cb = in + c.off;
     const int16_t* ov = ovf + c.ovf_base;
-    O1DecState z{c.pos0, c.x0, 0};
-    uint64_t e = 0, off = 0;
+    const bool lead = threadIdx.x == 0;
+    // every lane walks the same path (the values stay wave-uniform); lane 0 does the stores
+    SeqState z{c.pos0, c.x0, 0, 0, false};
+    uint64_t off = 0;
     const uint64_t m = c.hi - c.lo;
     for (uint64_t r = 0; r < c.nsub; r++) {
         const uint64_t t = c.tfirst + r;
-        res_start[t] = z;
-        res_e[t] = e;
-        res_off[t] = off;
+        if (lead) {
+            res_start[t] = O1DecState{z.B, z.X, z.cx};
+            res_e[t] = z.e;
+            res_off[t] = off;
+        }
         if (r + 1 == c.nsub) break;
         const O1Graph& g = graph[t];
         const O1StartSet& ss = start[t];
+        const O1DecState zs{z.B, z.X, z.cx};
         int o = -1;
         if (g.status == 0)
             for (int k = 0; k < ss.n; k++)
-                if (o1d_same(ss.s[k], z)) { o = k; break; }
+                if (o1d_same(ss.s[k], zs)) { o = k; break; }
         if (o >= 0) {
             O1GEdge ed = g.origin[o];
             off += ed.sym;
             int guard = 0;
             while (ed.tgt >= 0 && guard++ <= kGNode) {
-                const int sign = (e < c.novf) ? (ov[e] < 0) : 0;
-                e++;
+                const int sign = (z.e < c.novf) ? (ov[z.e] < 0) : 0;
+                z.e++;
                 ed = g.node[ed.tgt][sign];
                 off += ed.sym;
             }
-            if (ed.tgt >= 0 || -1 - ed.tgt >= g.nexit) { *err = 1; return; }
-            z = g.exit[-1 - ed.tgt];
-        } else {
+            if (ed.tgt >= 0 || -1 - ed.tgt >= g.nexit) {
+                if (lead) *err = 1;
+                return;
+            }
+            const O1DecState x = g.exit[-1 - ed.tgt];
+            z.B = x.B; z.X = x.X; z.cx = x.ctx;
+            continue;
+        }
+        if (lead) {
             atomicAdd(&stats[g.status ? 0 : 1], 1u);  // capacity exceeded / start state not in the set
             if (g.status) atomicAdd(&stats[1 + g.status], 1u);
-            const int64_t stop = c.pos0 - (int64_t)(r + 1) * c.sub_bits;
-            bool bad = false;
-            uint64_t n = 0;
-            BitWin rd = bwin_at(in + c.off, z.B);
-            while (rd.pos > stop && off + n < m) {
-                const uint32_t xi = z.X & ((1u << kO1Log) - 1);
-                const uint32_t hc = z.ctx - kWalkCtx0;
-                const DEntry d = hc < (uint32_t)kWalkCtxN ? hot[(hc << kO1Log) + xi] : dt[((size_t)z.ctx << kO1Log) + xi];
-                z.X = d.newState + bwin_read(rd, d.nbBits);
-                int code;
-                if (d.symbol == 255) {
-                    if (e < c.novf) code = ov[e];
-                    else { code = 32767; bad = true; }
-                    e++;
-                } else {
-                    const unsigned sy = d.symbol;
-                    code = (int)((sy >> 1) ^ (~(sy & 1) + 1));
-                }
-                z.ctx = (uint32_t)o1_ctx(code);
-                n++;
-            }
-            z.B = rd.pos;
-            off += n;
-            if (bad) { *err = 1; return; }
+            atomicAdd(&stats[6 + q], 1u);  // per chain
+        }
+        // exact decode of this subsequence down to `stop`, the bitstream staged in LDS window by
+        // window
+        const int64_t stop = c.pos0 - (int64_t)(r + 1) * c.sub_bits;
+        while (z.B > stop && off < m && !z.bad) {
+            const int64_t hi = min<int64_t>((int64_t)c.len, (z.B + 7) / 8 + 1);
+            const int64_t lo = max<int64_t>(0, hi - (int64_t)kWalkStage);
+            __syncthreads();
+            stage_window(lds(stage), cb, lo, hi);
+            __syncthreads();
+            // window reads reach 64 bits below the read position: stay above the staged bytes unless
+            // they start at the chain's first byte (bits below it read as zero)
+            const int64_t floor_bits = lo == 0 ? INT64_MIN : lo * 8 + 64;
+            off += seq_decode(z, lds(stage), lo, max<int64_t>(stop, floor_bits), m - off, lds(hot), dt, nullptr, 0, 0,
+                              ov, c.novf, nullptr);
+        }
+        if (z.bad) {
+            if (lead) *err = 1;
+            return;
         }
     }
+}
+
+// A chain whose escapes are dense (the coarse levels of a tight bound) cannot be split: every few
+// dozen symbols the next context depends on an escape whose index in the side stream is unknown
+// to a decoder that starts mid-chain. One workgroup decodes such a chain from its start: lane 0
+// runs the state machine against LDS only where it can (the decoding tables of contexts 11-14,
+// which hold most of the symbols after large codes, a window of the bitstream, a window of the
+// escaped values, and a buffer of decoded codes); the workgroup refills the windows and writes the
+// codes out between rounds.
+
+__global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, const O1DecChain* ch, const DEntry* dt_all,
+                                                         const int16_t* ovf, int16_t* out, int* err) {
+    __shared__ uint32_t hot[kWalkCtxN << kO1Log];
+    __shared__ uint32_t stage32[(kDenseStage + kWinPad) / 4 + 2];
+    uint8_t* stage = reinterpret_cast<uint8_t*>(stage32);
+    __shared__ int16_t obuf[kDenseOut];
+    __shared__ int16_t vbuf[kDenseOvf];
+    const O1DecChain c = ch[blockIdx.x];
+    const uint32_t* dt = reinterpret_cast<const uint32_t*>(dt_all + c.dset * kO1Ctx * (1u << kO1Log));
+    for (uint32_t i = threadIdx.x; i < (kWalkCtxN << kO1Log); i += blockDim.x)
+        hot[i] = dt[((size_t)kWalkCtx0 << kO1Log) + i];
+    const uint8_t* cb = in + c.off;
+    const int16_t* ov = ovf + c.ovf_base;
+    const uint64_t m = c.hi - c.lo;
+    // every lane runs the same state machine on the same data (the values stay wave-uniform)
+    SeqState z{c.pos0, c.x0, 0, 0, false};
+    uint64_t i = 0;
+    while (i < m && !z.bad) {
+        const int64_t hi = min<int64_t>((int64_t)c.len, (z.B + 7) / 8 + 1);
+        const int64_t lo = max<int64_t>(0, hi - (int64_t)kDenseStage);
+        const uint64_t e0 = z.e, ve = min<uint64_t>(c.novf, e0 + kDenseOvf);
+        __syncthreads();
+        stage_window(lds(stage), cb, lo, hi);
+        for (uint64_t k = e0 + threadIdx.x; k < ve; k += blockDim.x) vbuf[k - e0] = ov[k];
+        __syncthreads();
+        const int64_t floor_bits = lo == 0 ? INT64_MIN : lo * 8 + 64;
+        const uint64_t nk = seq_decode(z, lds(stage), lo, floor_bits, min<uint64_t>(m - i, kDenseOut), lds(hot), dt,
+                                       lds(vbuf), e0, ve, ov, c.novf, lds(obuf));
+        __syncthreads();
+        for (uint64_t k = threadIdx.x; k < nk; k += blockDim.x) out[c.lo + i + k] = obuf[k];
+        i += nk;
+    }
+    if (threadIdx.x == 0 && (z.bad || i != m || z.e != c.novf)) *err = 1;
 }
 
 // Final pass: every subsequence decodes from its settled start into its place in the code array.
@@ -1396,11 +1644,10 @@ static uint64_t fse_enc_blocks(const uint8_t* src, uint64_t n, uint8_t* out, uin
                                hipStream_t st) {
     if (n == 0) return 0;
     const uint64_t nb = (n + kFblk - 1) / kFblk;
-    uint8_t* tmp = ws.take<uint8_t>(nb * (kFblk + 1024));
     uint8_t* slots = ws.take<uint8_t>(nb * (kFblk + 1024));
     uint64_t* sizes = ws.take<uint64_t>(nb);
     uint64_t* offs = ws.take<uint64_t>(nb);
-    FZ_LAUNCH("fse_blocks", n * 2, k_fse_blocks, dim3((unsigned)nb), dim3(256), 0, st, src, n, tmp, slots, sizes);
+    FZ_LAUNCH("fse_blocks", n * 2, k_fse_blocks, dim3((unsigned)nb), dim3(256), 0, st, src, n, slots, sizes);
     FZ_HIP(hipGetLastError());
     std::vector<uint64_t> hs(nb), ho(nb);
     d2h(hs.data(), sizes, nb, st);
@@ -1806,12 +2053,26 @@ static void fse_dec_blocks(const uint8_t* hsrc, const uint8_t* dsrc, uint64_t n,
 
 // Order-1 tANS chains (7 values each: stream offset, bytes, out lo, out hi, escape offset, table
 // set, escape count) decoded in parallel subsequences; see k_o1d_round.
+// More than this many escapes per 4096 symbols on average: the chain is decoded from its start.
+constexpr uint64_t kDenseEscapes = 8;
+
 static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const std::vector<uint64_t>& chains,
                                const DEntry* dt, const int16_t* ovf, int16_t* out, int* err, DevArena& ws,
-                               hipStream_t st) {
+                               hipStream_t st, const AuxStream& aux) {
     const int nch = (int)(chains.size() / 7);
-    std::vector<O1DecChain> hc;
-    uint64_t nsub = 0, max_sub = 1, stream_bytes = 0, ncodes = 0;
+    std::vector<O1DecChain> hc, hd;
+    uint64_t nsub = 0, max_sub = 1, stream_bytes = 0, ncodes = 0, dense_bytes = 0, dense_codes = 0;
+    // segments holding a dense chain: their context-13/14 tables give escapes a high probability, so
+    // a decoder off the true path (a wrong sign, a wrong start) meets spurious escapes at every turn
+    // and graphs overflow even where real escapes are few; chains there that have escapes at all are
+    // decoded from their start as well
+    std::vector<uint64_t> dense_seg;
+    for (int q = 0; q < nch; q++) {
+        const uint64_t* c = &chains[7 * q];
+        const uint64_t m = c[3] - c[2];
+        if (m && c[6] * 4096 > kDenseEscapes * m) dense_seg.push_back(c[5]);
+    }
+    auto in_dense_seg = [&](uint64_t g) { return std::find(dense_seg.begin(), dense_seg.end(), g) != dense_seg.end(); };
     for (int q = 0; q < nch; q++) {
         const uint64_t* c = &chains[7 * q];
         const uint64_t m = c[3] - c[2];
@@ -1825,6 +2086,12 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
             x0 = (x0 << 1) | (bit >= 0 ? ((b[bit >> 3] >> (bit & 7)) & 1u) : 0u);
         }
         const int64_t pos0 = top - (int64_t)kO1Log;
+        if (c[6] * 4096 > kDenseEscapes * m || (in_dense_seg(c[5]) && c[6] * 16384 > m)) {
+            hd.push_back(O1DecChain{c[0], c[1], c[2], c[3], c[4], c[6], c[5], pos0, x0, 1, 0, 0});
+            dense_bytes += c[1];
+            dense_codes += m;
+            continue;
+        }
         // about 4096 symbols per subsequence, enough for a guessed start to fall into step
         const double bits_per_sym = std::max(1.0, (double)std::max<int64_t>(pos0, 1) / (double)m);
         const int64_t sub = std::max<int64_t>(8192, (int64_t)(bits_per_sym * 4096));
@@ -1835,6 +2102,30 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
         nsub += ns;
         max_sub = std::max(max_sub, ns);
     }
+    // dense chains run beside the rest, on the auxiliary stream when there is one
+    hipStream_t dst = aux.stream ? aux.stream : st;
+    if (!hd.empty()) {
+        O1DecChain* ddc = ws.take<O1DecChain>(hd.size());
+        h2d(ddc, hd.data(), hd.size(), st);
+        if (aux.stream) {
+            FZ_HIP(hipEventRecord(aux.fork, st));
+            FZ_HIP(hipStreamWaitEvent(aux.stream, aux.fork, 0));
+        }
+        FZ_LAUNCH("o1_decode_dense", dense_bytes + dense_codes * 2, k_o1d_dense, dim3((unsigned)hd.size()), dim3(kSeqLanes), 0,
+                  dst, dev, ddc, dt, ovf, out, err);
+        FZ_HIP(hipGetLastError());
+    }
+    struct Join {  // the caller's stream waits for the dense chains however this function exits
+        const AuxStream& a;
+        hipStream_t st;
+        bool on;
+        ~Join() {
+            if (on) {
+                (void)hipEventRecord(a.join, a.stream);
+                (void)hipStreamWaitEvent(st, a.join, 0);
+            }
+        }
+    } join{aux, st, !hd.empty() && aux.stream != nullptr};
     if (hc.empty()) return;
     const int nc = (int)hc.size();
     O1DecChain* dch = ws.take<O1DecChain>(nc);
@@ -1845,8 +2136,8 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
     uint64_t* res_off = ws.take<uint64_t>(nsub);
     uint8_t* run = ws.take<uint8_t>(nsub);
     unsigned int* nrun = ws.take<unsigned int>(1);
-    unsigned int* stats = ws.take<unsigned int>(6);
-    FZ_HIP(hipMemsetAsync(stats, 0, 6 * sizeof(unsigned int), st));
+    unsigned int* stats = ws.take<unsigned int>(6 + nc);
+    FZ_HIP(hipMemsetAsync(stats, 0, (6 + nc) * sizeof(unsigned int), st));
     h2d(dch, hc.data(), hc.size(), st);
     uint64_t rounds = 0;
     FZ_LAUNCH("o1_decode_init", 0, k_o1d_init, dim3(nblk(nsub)), dim3(256), 0, st, dch, nc, nsub, start, run);
@@ -1867,14 +2158,20 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
         rounds = round + 1;
         if (!h) break;
     }
-    FZ_LAUNCH("o1_decode_walk", 0, k_o1d_walk, dim3(nc), dim3(256), 0, st, dev, dch, nc, dt, ovf, start, graph,
+    FZ_LAUNCH("o1_decode_walk", 0, k_o1d_walk, dim3(nc), dim3(kSeqLanes), 0, st, dev, dch, nc, dt, ovf, start, graph,
               res_start, res_e, res_off, stats, err);
     FZ_HIP(hipGetLastError());
     FZ_LAUNCH("o1_decode_out", stream_bytes + ncodes * 2, k_o1d_out, dim3(nblk(nsub)), dim3(256), 0, st, dev, dch, nc,
               nsub, dt, ovf, res_start, res_e, res_off, out, err);
     if (getenv("FALCONS_DECODE_STATS")) {
-        unsigned int hs[6] = {0, 0, 0, 0, 0, 0};
-        d2h(hs, stats, 6, st);
+        std::vector<unsigned int> hsv(6 + nc);
+        d2h(hsv.data(), stats, 6 + nc, st);
+        const unsigned int* hs = hsv.data();
+        fprintf(stderr, "[o1 decode] exact subsequences per chain (chain:count, nonzero):");
+        for (int q = 0; q < nc; q++)
+            if (hs[6 + q]) fprintf(stderr, " %d:%u/%u/esc%llu", q, hs[6 + q], hc[q].nsub, (unsigned long long)hc[q].novf);
+        fprintf(stderr, "\n");
+        fprintf(stderr, "[o1 decode] %zu dense chains decoded from their start\n", hd.size());
         fprintf(stderr, "[o1 decode] %d chains, %llu subsequences, %llu rounds, walk decoded %u subsequences "
                         "exactly (%u graphs over capacity [exits %u, nodes %u, branches %u, open edges %u], %u start "
                         "states outside the set)\n",
@@ -1885,7 +2182,7 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
 }
 
 void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
-                    hipStream_t st) {
+                    hipStream_t st, const AuxStream& aux) {
     if (len < 5) throw std::invalid_argument("Truncated entropy stream");
     const uint8_t* const hend = host + len;
     auto need_in = [&](const uint8_t* at, uint64_t k) {
@@ -1989,7 +2286,7 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
             d2h(&herr, err, 1, st);
             if (herr) throw std::invalid_argument("corrupt o1 tANS tables");
         }
-        o1_decode_parallel(host, dev, chains, dt, ovf, out, err, ws, st);
+        o1_decode_parallel(host, dev, chains, dt, ovf, out, err, ws, st, aux);
     } else {
         const uint64_t nr = get64(hp), nv = get64(hp), s_rcl = get64(hp), s_vcl = get64(hp), rbp = get64(hp),
                        vbp = get64(hp);

@@ -21,17 +21,19 @@ constexpr uint64_t kFseErr = ~0ull;
 
 FZ_HD unsigned highbit32(uint32_t v) { return 31u - (unsigned)__builtin_clz(v); }
 
-// Encoding table of one alphabet (at most 256 symbols, tableLog <= 13).
-struct CTable {
+// Encoding table of one alphabet (at most 256 symbols, tableLog <= MaxLog).
+template <unsigned MaxLog>
+struct CTableT {
     uint32_t tableLog;
     uint32_t maxSym;
     uint32_t deltaNbBits[256];
     int32_t deltaFindState[256];
-    uint16_t stateTable[1u << kFseMaxTableLog];
+    uint16_t stateTable[1u << MaxLog];
 };
+using CTable = CTableT<kFseMaxTableLog>;
 
 // Decoding table entry: next-state base, symbol, bits to read.
-struct DEntry {
+struct alignas(4) DEntry {
     uint16_t newState;
     uint8_t symbol;
     uint8_t nbBits;
@@ -318,7 +320,8 @@ FZ_HD bool build_dtable(DEntry* dt, const int16_t* norm, unsigned maxSym, unsign
 }
 
 // FSE_initCState2 (zfse/fse.h:504-515)
-FZ_HD uint32_t init_state2(const CTable* ct, unsigned sym) {
+template <class CT>
+FZ_HD uint32_t init_state2(const CT* ct, unsigned sym) {
     const uint32_t dnb = ct->deltaNbBits[sym];
     const uint32_t nbOut = (dnb + (1u << 15)) >> 16;
     const uint32_t v = (nbOut << 16) - dnb;

@@ -83,7 +83,13 @@ size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out
                       EntropyStats* stats);
 size_t entropy_decode_workspace(uint64_t n, size_t len);
 // Decodes the encoder bytes (host copy `host`, device copy `dev`, length len) into n codes (device).
+// A second stream for work that can run beside the main one (null: run on the main stream), with the
+// two events that fork it from and join it back to the main stream.
+struct AuxStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
 void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
-                    hipStream_t st);
+                    hipStream_t st, const AuxStream& aux);
 
 }  // namespace fz
