@@ -1195,7 +1195,8 @@ struct O1StartSet {
 
 struct O1Graph {
     O1DecState exit[kGBr];
-    int nexit, nnode, status, pad;  // status 0: complete, 1: capacity exceeded
+    int nexit, nnode, status, norigin;  // status 0: complete, 1: capacity exceeded
+    O1DecState ostate[kGBr];            // the start states the graph was built from
     O1GEdge origin[kGBr];
     O1GEdge node[kGNode][2];
 };
@@ -1204,8 +1205,11 @@ __device__ inline bool o1d_same(const O1DecState& a, const O1DecState& b) {
     return a.B == b.B && a.X == b.X && a.ctx == b.ctx;
 }
 
-// Starts of round 1: the true start for subsequence 0, a guess elsewhere.
-__global__ void __launch_bounds__(256) k_o1d_init(const O1DecChain* ch, int nch, uint64_t nsub, O1StartSet* start,
+// Starts of round 1: the true start for subsequence 0; elsewhere the state a decoder reaches at the
+// subsequence boundary after warming up over the preceding `warm` bits from a guessed state (it has
+// almost always fallen into step by then; after an escape it follows the positive sign).
+__global__ void __launch_bounds__(256) k_o1d_init(const uint8_t* in, const O1DecChain* ch, int nch, uint64_t nsub,
+                                                  const DEntry* dt_all, int64_t warm_subs, O1StartSet* start,
                                                   uint8_t* run) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nsub) return;
@@ -1214,8 +1218,22 @@ __global__ void __launch_bounds__(256) k_o1d_init(const O1DecChain* ch, int nch,
     O1StartSet ss;
     ss.n = 1;
     ss.pad = 0;
-    if (r == 0) ss.s[0] = O1DecState{c.pos0, c.x0, 0};
-    else ss.s[0] = O1DecState{c.pos0 - (int64_t)r * c.sub_bits, (uint32_t)((r * 2654435761u) & ((1u << kO1Log) - 1)), 0};
+    if (r == 0) {
+        ss.s[0] = O1DecState{c.pos0, c.x0, 0};
+    } else {
+        const int64_t b = c.pos0 - (int64_t)r * c.sub_bits;
+        const int64_t w = min<int64_t>(c.pos0, b + warm_subs * c.sub_bits);
+        const DEntry* dt = dt_all + c.dset * kO1Ctx * (1u << kO1Log);
+        BitWin rd = bwin_at(in + c.off, w);
+        uint32_t X = (uint32_t)((r * 2654435761u) & ((1u << kO1Log) - 1)), cx = 0;
+        while (rd.pos > b) {
+            const DEntry d = dt[(size_t)cx * (1u << kO1Log) + (X & ((1u << kO1Log) - 1))];
+            X = d.newState + bwin_read(rd, d.nbBits & 15u);
+            const unsigned nc = d.nbBits >> 4;
+            cx = nc == 15 ? 13u : nc;
+        }
+        ss.s[0] = O1DecState{rd.pos, X, cx};
+    }
     start[t] = ss;
     run[t] = 1;
 }
@@ -1252,6 +1270,7 @@ __global__ void __launch_bounds__(kGLanes) k_o1d_graph(const uint8_t* in, const 
         const int n0 = min(ss.n, kGBr);
         for (int o = 0; o < n0; o++) {
             const O1DecState s0 = ss.s[o];
+            G.ostate[o] = s0;
             bB[nb][l] = s0.B; bX[nb][l] = s0.X; bC[nb][l] = (uint8_t)s0.ctx; bN[nb][l] = 0;
             pS[np][l] = o; pB[np][l] = (uint8_t)nb; pC[np][l] = 0;
             np++;
@@ -1355,6 +1374,7 @@ __global__ void __launch_bounds__(kGLanes) k_o1d_graph(const uint8_t* in, const 
     G.nexit = nexit;
     G.nnode = nnode;
     G.status = status;
+    G.norigin = min(start[t].n, kGBr);
 }
 
 // Next round's start sets from this round's exits; counts the subsequences to run again.
@@ -1391,9 +1411,8 @@ constexpr int64_t kWalkStage = 16384;          // bitstream bytes staged per win
 
 __global__ void __launch_bounds__(kSeqLanes) k_o1d_walk(const uint8_t* in, const O1DecChain* ch, int nch,
                                                         const DEntry* dt_all, const int16_t* ovf,
-                                                        const O1StartSet* start, const O1Graph* graph,
-                                                        O1DecState* res_start, uint64_t* res_e, uint64_t* res_off,
-                                                        unsigned int* stats, int* err) {
+                                                        const O1Graph* graph, O1DecState* res_start, uint64_t* res_e,
+                                                        uint64_t* res_off, unsigned int* stats, int* err) {
     __shared__ uint32_t hot[kWalkCtxN << kO1Log];
     __shared__ uint32_t stage32[(kWalkStage + kWinPad) / 4 + 2];
     uint8_t* stage = reinterpret_cast<uint8_t*>(stage32);
@@ -1419,13 +1438,14 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_walk(const uint8_t* in, const
             res_off[t] = off;
         }
         if (r + 1 == c.nsub) break;
+        // the graph is usable when it is complete and was built from a start set holding the true
+        // start, also when later rounds moved that subsequence's start set on
         const O1Graph& g = graph[t];
-        const O1StartSet& ss = start[t];
         const O1DecState zs{z.B, z.X, z.cx};
         int o = -1;
         if (g.status == 0)
-            for (int k = 0; k < ss.n; k++)
-                if (o1d_same(ss.s[k], zs)) { o = k; break; }
+            for (int k = 0; k < g.norigin; k++)
+                if (o1d_same(g.ostate[k], zs)) { o = k; break; }
         if (o >= 0) {
             O1GEdge ed = g.origin[o];
             off += ed.sym;
@@ -2055,13 +2075,22 @@ static void fse_dec_blocks(const uint8_t* hsrc, const uint8_t* dsrc, uint64_t n,
 // set, escape count) decoded in parallel subsequences; see k_o1d_round.
 // More than this many escapes per 4096 symbols on average: the chain is decoded from its start.
 constexpr uint64_t kDenseEscapes = 8;
+// Symbols per subsequence (about): each graph round decodes this many per lane, so it sets the
+// latency of a round; a decoder started at a guessed state falls into step within about a
+// thousand symbols, so start sets settle within a few rounds.
+constexpr double kSubSymbols = 1024;
+// Graph rounds before the walk; subsequences whose start sets are still moving are decoded exactly
+// by the walk.
+constexpr uint64_t kMaxRounds = 6;
+// Subsequences of warm-up before the first round's start states.
+constexpr int64_t kWarmSubs = 4;
 
 static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const std::vector<uint64_t>& chains,
                                const DEntry* dt, const int16_t* ovf, int16_t* out, int* err, DevArena& ws,
                                hipStream_t st, const AuxStream& aux) {
     const int nch = (int)(chains.size() / 7);
     std::vector<O1DecChain> hc, hd;
-    uint64_t nsub = 0, max_sub = 1, stream_bytes = 0, ncodes = 0, dense_bytes = 0, dense_codes = 0;
+    uint64_t nsub = 0, stream_bytes = 0, ncodes = 0, dense_bytes = 0, dense_codes = 0;
     // segments holding a dense chain: their context-13/14 tables give escapes a high probability, so
     // a decoder off the true path (a wrong sign, a wrong start) meets spurious escapes at every turn
     // and graphs overflow even where real escapes are few; chains there that have escapes at all are
@@ -2094,13 +2123,12 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
         }
         // about 4096 symbols per subsequence, enough for a guessed start to fall into step
         const double bits_per_sym = std::max(1.0, (double)std::max<int64_t>(pos0, 1) / (double)m);
-        const int64_t sub = std::max<int64_t>(8192, (int64_t)(bits_per_sym * 4096));
+        const int64_t sub = std::max<int64_t>(2048, (int64_t)(bits_per_sym * kSubSymbols));
         const uint64_t ns = std::max<int64_t>(1, (pos0 + sub - 1) / sub);
         hc.push_back(O1DecChain{c[0], c[1], c[2], c[3], c[4], c[6], c[5], pos0, x0, (uint32_t)ns, nsub, sub});
         stream_bytes += c[1];
         ncodes += m;
         nsub += ns;
-        max_sub = std::max(max_sub, ns);
     }
     // dense chains run beside the rest, on the auxiliary stream when there is one
     hipStream_t dst = aux.stream ? aux.stream : st;
@@ -2140,12 +2168,12 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
     FZ_HIP(hipMemsetAsync(stats, 0, (6 + nc) * sizeof(unsigned int), st));
     h2d(dch, hc.data(), hc.size(), st);
     uint64_t rounds = 0;
-    FZ_LAUNCH("o1_decode_init", 0, k_o1d_init, dim3(nblk(nsub)), dim3(256), 0, st, dch, nc, nsub, start, run);
+    FZ_LAUNCH("o1_decode_init", 0, k_o1d_init, dim3(nblk(nsub)), dim3(256), 0, st, dev, dch, nc, nsub, dt,
+              (int64_t)kWarmSubs, start, run);
     FZ_HIP(hipGetLastError());
     // start sets settle like the states of a single decoder: every round settles at least the next
     // subsequence of every chain
-    for (uint64_t round = 0;; round++) {
-        if (round > max_sub + 1) throw std::runtime_error("tANS subsequence decoding did not settle");
+    for (uint64_t round = 0; round < kMaxRounds; round++) {
         FZ_LAUNCH("o1_decode_graph", 0, k_o1d_graph, dim3(nblk(nsub, kGLanes)), dim3(kGLanes), 0, st, dev, dch, nc, nsub,
                   dt, run, start, graph);
         FZ_HIP(hipGetLastError());
@@ -2158,8 +2186,8 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
         rounds = round + 1;
         if (!h) break;
     }
-    FZ_LAUNCH("o1_decode_walk", 0, k_o1d_walk, dim3(nc), dim3(kSeqLanes), 0, st, dev, dch, nc, dt, ovf, start, graph,
-              res_start, res_e, res_off, stats, err);
+    FZ_LAUNCH("o1_decode_walk", 0, k_o1d_walk, dim3(nc), dim3(kSeqLanes), 0, st, dev, dch, nc, dt, ovf, graph, res_start,
+              res_e, res_off, stats, err);
     FZ_HIP(hipGetLastError());
     FZ_LAUNCH("o1_decode_out", stream_bytes + ncodes * 2, k_o1d_out, dim3(nblk(nsub)), dim3(256), 0, st, dev, dch, nc,
               nsub, dt, ovf, res_start, res_e, res_off, out, err);
