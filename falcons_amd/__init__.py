@@ -24,6 +24,18 @@ if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is missing: build it with `make -C falcons_amd/csrc` "
                       "(or __graft_entry__.build())")
 
+# Independent calls (the batched and serving forms, several contexts) overlap on the device only
+# as far as the runtime gives their streams hardware queues; its default is 4. Raised to 16 (unless
+# FALCONS_KEEP_HW_QUEUES is set); the setting takes effect when this process initialises the GPU
+# runtime, so it is made before the library loads.
+if not os.environ.get("FALCONS_KEEP_HW_QUEUES"):
+    try:
+        _hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+    except ValueError:
+        _hwq = 4
+    if _hwq < 16:
+        os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 _L = C.CDLL(LIB_PATH)
 
 FZ_OK, FZ_ERR_INVALID_ARGUMENT, FZ_ERR_UNSUPPORTED, FZ_ERR_GPU, FZ_ERR_RUNTIME = range(5)

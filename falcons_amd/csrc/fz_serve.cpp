@@ -52,12 +52,19 @@ class WorkerSet {
         for (fz_ctx* c : ctx_) fz_ctx_destroy(c);
     }
     int size() const { return (int)ctx_.size(); }
-    // body(worker context, item index) for every index of `order`, handed out in that order.
+    // device bytes the workers' workspaces hold
+    size_t held() const {
+        size_t h = 0;
+        for (const fz_ctx* c : ctx_) h += c->arena.capacity();
+        return h;
+    }
+    // body(worker context, item index) for every index of `order`, handed out in that order, on at
+    // most `active` workers at once.
     template <class F>
-    void run(const std::vector<int>& order, F body) {
+    void run(const std::vector<int>& order, F body, int active) {
         std::atomic<size_t> next{0};
         std::vector<std::thread> th;
-        const int nw = std::min<int>(size(), (int)order.size());
+        const int nw = std::max(1, std::min<int>(std::min(size(), active), (int)order.size()));
         for (int w = 0; w < nw; w++)
             th.emplace_back([&, w] {
                 (void)hipSetDevice(device_);
@@ -82,7 +89,17 @@ namespace {
 int default_workers() {
     const char* e = getenv("FALCONS_WORKERS");
     const int v = e ? atoi(e) : 0;
-    return v > 0 ? std::min(v, 64) : 8;
+    return v > 0 ? std::min(v, 64) : 24;
+}
+
+// Workers that may run at once: each holds a workspace for the largest item, and together they stay
+// within 60 % of the device memory free now (workspaces the workers already hold count as free).
+int active_workers(WorkerSet& ws, size_t max_need, size_t held) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 1;
+    const size_t budget = (free_b + held) / 10 * 6;
+    const size_t k = max_need ? budget / max_need : (size_t)ws.size();
+    return (int)std::max<size_t>(1, std::min<size_t>(k, (size_t)ws.size()));
 }
 
 WorkerSet& workers_of(fz_ctx* ctx) {
@@ -139,6 +156,9 @@ extern "C" int fz_compress_batch(fz_ctx* ctx, fz_batch_item* items, int n, fz_op
         const int simd = opts ? opts->simd : 1;
         const std::vector<int> order =
             largest_first(n, [&](int i) { return items[i].conf.num * es_of(items[i].dtype); });
+        size_t need = 0;
+        for (int i = 0; i < n; i++) need = std::max(need, compress_workspace_bytes(items[i].conf.num, items[i].dtype));
+        const int active = active_workers(ws, need, ws.held());
         FirstError fe;
         const double t0 = now_ms();
         ws.run(order, [&](fz_ctx* w, int i) {
@@ -149,7 +169,7 @@ extern "C" int fz_compress_batch(fz_ctx* ctx, fz_batch_item* items, int n, fz_op
             it.pre_len = 0;
             it.status = fz_compress_device(w, &it.conf, it.dtype, it.d_data, it.d_pre, it.pre_cap, &it.pre_len, &o);
             fe.note(it.status, fz_last_error(w));
-        });
+        }, active);
         if (opts) {
             opts->t_total_ms = now_ms() - t0;
             opts->t_device_ms = opts->t_total_ms;
@@ -166,6 +186,11 @@ extern "C" int fz_decompress_batch(fz_ctx* ctx, fz_batch_dec_item* items, int n,
         WorkerSet& ws = workers_of(ctx);
         const std::vector<int> order =
             largest_first(n, [&](int i) { return items[i].conf.num * es_of(items[i].dtype); });
+        size_t need = 0;
+        for (int i = 0; i < n; i++)
+            need = std::max(need, decompress_workspace_bytes(items[i].conf.num, items[i].dtype, items[i].conf.num,
+                                                             items[i].pre_len));
+        const int active = active_workers(ws, need, ws.held());
         FirstError fe;
         const double t0 = now_ms();
         ws.run(order, [&](fz_ctx* w, int i) {
@@ -175,7 +200,7 @@ extern "C" int fz_decompress_batch(fz_ctx* ctx, fz_batch_dec_item* items, int n,
             it.status = fz_decompress_device(w, &it.conf, it.dtype, it.h_pre, it.d_pre, it.pre_len, it.d_out,
                                              it.out_cap_elems, &o);
             fe.note(it.status, fz_last_error(w));
-        });
+        }, active);
         if (opts) {
             opts->t_total_ms = now_ms() - t0;
             opts->t_device_ms = opts->t_total_ms;

@@ -151,7 +151,11 @@ typedef struct {
     size_t out_cap_elems;
 } fz_batch_dec_item;
 
-/* Number of worker sub-contexts of ctx for the batched calls (default 8, or FALCONS_WORKERS). */
+/* Number of worker sub-contexts of ctx for the batched calls (default 24, or FALCONS_WORKERS); at
+ * most as many run at once as fit in 60% of the free device memory with a workspace each for the
+ * largest item. The calls overlap on the device as far as the runtime gives their streams hardware
+ * queues: set GPU_MAX_HW_QUEUES (e.g. 16; the runtime's default is 4) before the process first uses
+ * the GPU. */
 int fz_set_workers(fz_ctx *ctx, int workers);
 int fz_compress_batch(fz_ctx *ctx, fz_batch_item *items, int n, fz_opts *opts);
 int fz_decompress_batch(fz_ctx *ctx, fz_batch_dec_item *items, int n, fz_opts *opts);
