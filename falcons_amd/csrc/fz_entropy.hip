@@ -52,6 +52,15 @@ __host__ __device__ inline int o1_sym(int code) {
 }
 
 // ---------------------------------------------------------------- small device helpers
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b mod 8), and each XCD has
+// its own L2. Kernels whose neighbouring workgroups share tables (the tANS tables of one segment)
+// renumber their workgroups so that every XCD takes one contiguous range: XCD x gets the logical
+// blocks x*(n/8) + min(x, n%8) onward, in order.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t n = gridDim.x, b = blockIdx.x, x = b & 7u, k = b >> 3;
+    return x * (n >> 3) + min(x, n & 7u) + k;
+}
+
 struct BitW {  // LSB-first FSE bitstream writer (zfse/bitstream.h)
     uint8_t* buf;
     uint64_t pos;
@@ -813,7 +822,7 @@ __device__ inline uint32_t o1_step(uint32_t& state, const CTable& ct, unsigned s
 __global__ void __launch_bounds__(256) k_o1_spec(const int16_t* bins, O1Chains C, uint64_t npieces,
                                                  const CTable* ct_all, uint16_t* slog, uint32_t* end_state,
                                                  uint64_t* nbits) {
-    const uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t pid = (uint64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (pid >= npieces) return;
     const int q = o1_piece_chain(C, pid);
     const uint64_t lo = C.c[4 * q + 1], hi = C.c[4 * q + 2];
@@ -824,8 +833,9 @@ __global__ void __launch_bounds__(256) k_o1_spec(const int16_t* bins, O1Chains C
     // piece 0 starts from the true seed; the others from the seed their own first symbol would give
     uint32_t st = o1_seed_state(bins, lo, p == 0 ? hi - 1 : s_hi, ct);
     uint64_t bits = 0;
+    // the log is step-major (step j of every piece side by side), so a wave's writes are coalesced
     for (uint64_t i = s_hi + 1; i-- > s_lo;) {
-        slog[i] = (uint16_t)st;
+        slog[(s_hi - i) * npieces + pid] = (uint16_t)st;
         const int pc = (i > lo) ? bins[i - 1] : 0;
         bits += o1_step(st, ct[o1_ctx(pc)], o1_sym(bins[i]));
     }
@@ -845,7 +855,7 @@ __global__ void __launch_bounds__(256) k_o1_fix(const int16_t* bins, O1Chains C,
                                                 const uint64_t* spec_bits, const uint32_t* end_prev,
                                                 uint32_t* end_next, uint64_t* nbits, const uint8_t* changed_prev,
                                                 uint8_t* changed_next, unsigned int* any_changed) {
-    const uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t pid = (uint64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (pid >= npieces) return;
     const int q = o1_piece_chain(C, pid);
     const uint64_t p = pid - C.pfirst[q];
@@ -860,11 +870,12 @@ __global__ void __launch_bounds__(256) k_o1_fix(const int16_t* bins, O1Chains C,
         int64_t delta = 0;
         bool met = false;
         for (uint64_t i = s_hi + 1; i-- > s_lo;) {
-            if (st == slog[i]) { met = true; break; }
+            const uint32_t logged = slog[(s_hi - i) * npieces + pid];
+            if (st == logged) { met = true; break; }
             const int pc = (i > lo) ? bins[i - 1] : 0;
             const CTable& t = ct[o1_ctx(pc)];
             const unsigned sym = o1_sym(bins[i]);
-            uint32_t ls = slog[i];
+            uint32_t ls = logged;
             delta -= (int64_t)o1_step(ls, t, sym);
             delta += (int64_t)o1_step(st, t, sym);
         }
@@ -937,7 +948,7 @@ __device__ inline void ww_flush(WordWriter& W) {
 __global__ void __launch_bounds__(256) k_o1_emit(const int16_t* bins, O1Chains C, uint64_t npieces,
                                                  const CTable* ct_all, const uint32_t* end_state,
                                                  const uint64_t* bitoff, uint32_t* words, uint64_t stride_words) {
-    const uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t pid = (uint64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (pid >= npieces) return;
     const int q = o1_piece_chain(C, pid);
     const uint64_t lo = C.c[4 * q + 1], hi = C.c[4 * q + 2];
@@ -1254,7 +1265,7 @@ __global__ void __launch_bounds__(kGLanes) k_o1d_graph(const uint8_t* in, const 
     __shared__ uint8_t pB[kGPend][kGLanes]; // its branch
     __shared__ uint32_t pC[kGPend][kGLanes];// branch count where it started
     const int l = threadIdx.x;
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + l;
+    const uint64_t t = (uint64_t)xcd_block() * blockDim.x + l;
     if (t >= nsub || !run[t]) return;
     const O1DecChain c = ch[o1d_chain_of(ch, nch, t)];
     const uint64_t r = t - c.tfirst;
@@ -1539,7 +1550,7 @@ __global__ void __launch_bounds__(256) k_o1d_out(const uint8_t* in, const O1DecC
                                                  const DEntry* dt_all, const int16_t* ovf, const O1DecState* start,
                                                  const uint64_t* e_start, const uint64_t* res_off, int16_t* out,
                                                  int* err) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t t = (uint64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (t >= nsub) return;
     const O1DecChain c = ch[o1d_chain_of(ch, nch, t)];
     const uint64_t r = t - c.tfirst;
@@ -1635,7 +1646,7 @@ size_t entropy_encode_workspace(uint64_t n) {
     b += (n * 3 + 4096) * 2;             // mantissas
     b += 4 * blocks * (kFblk + 1024);    // block tmp + slots (two arrays)
     b += n * 2 * 2 + (1 << 20);          // o1 chain buffers (2 bytes per symbol + slack)
-    b += n * 2 + (n / kO1Piece + 1024) * 64;  // o1 piece state log and per-piece arrays
+    b += n * 2 + 64 * kO1Piece * 2 + (n / kO1Piece + 1024) * 64;  // o1 piece state log and per-piece arrays
     b += 16 * kO1Ctx * sizeof(CTable) + 16 * kO1Ctx * (1u << kO1Log);
     b += 3 * (n * 2 + 4 * (kFblk + 1024)) + (1 << 20);  // ovf
     b += 64u << 20;                      // cub scratch
@@ -1896,7 +1907,7 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
     uint64_t* dpfirst = ws.take<uint64_t>(nch + 1);
     h2d(dpfirst, pfirst.data(), nch + 1, st);
     const O1Chains CH{dchains, dpfirst, nch};
-    uint16_t* slog = ws.take<uint16_t>(n + 16);
+    uint16_t* slog = ws.take<uint16_t>(np * kO1Piece + 16);  // step-major: [step in piece][piece]
     uint32_t* spec_end = ws.take<uint32_t>(np + 1);
     uint32_t* end_a = ws.take<uint32_t>(np + 1);
     uint32_t* end_b = ws.take<uint32_t>(np + 1);
