@@ -1222,7 +1222,7 @@ __device__ inline bool o1d_same(const O1DecState& a, const O1DecState& b) {
 __global__ void __launch_bounds__(256) k_o1d_init(const uint8_t* in, const O1DecChain* ch, int nch, uint64_t nsub,
                                                   const DEntry* dt_all, int64_t warm_subs, O1StartSet* start,
                                                   uint8_t* run) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t t = (uint64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (t >= nsub) return;
     const O1DecChain c = ch[o1d_chain_of(ch, nch, t)];
     const uint64_t r = t - c.tfirst;
