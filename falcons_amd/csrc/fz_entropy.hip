@@ -1502,6 +1502,97 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_walk(const uint8_t* in, const
     }
 }
 
+// Decoding-table entries of the dense-chain decoder. Contexts are renumbered so that the four
+// kept in LDS (kWalkCtx0 ..) come first: 0..3 are those, 4..14 are contexts 0..10, 15 follows an
+// escape. LDS entry: [31:28] renumbered next context, [27:24] bit count, [23:15] the decoded code
+// (signed, 9 bits), [14:0] the next state base times four (a byte offset into one table).
+__device__ __forceinline__ uint32_t dense_ctx(uint32_t c) {
+    return c == 15u ? 15u : (c >= (uint32_t)kWalkCtx0 ? c - (uint32_t)kWalkCtx0 : c + (uint32_t)kWalkCtxN);
+}
+__device__ __forceinline__ uint32_t dense_entry(uint32_t raw) {
+    const uint32_t sy = (raw >> 16) & 0xFFu;
+    const uint32_t code = (sy >> 1) ^ (~(sy & 1u) + 1u);
+    return (dense_ctx(raw >> 28) << 28) | (((raw >> 24) & 15u) << 24) | ((code & 0x1FFu) << 15) |
+           ((raw & 0x1FFFu) << 2);
+}
+
+// The state machine of seq_decode for the dense-chain decoder (escaped values always from vbuf,
+// codes always to obuf, tables of the renumbered contexts 0..3 in LDS in the format above, the
+// others in device memory in the common format). Steps run in groups of four with one bounds check
+// per group while the group cannot reach the window floor, the output limit or the escape limit;
+// the last steps run one by one with every check.
+__device__ __forceinline__ uint64_t dense_decode(SeqState& S, const FZ_LDS uint8_t* win, int64_t wlo, int64_t until,
+                                                 uint64_t max_n, const FZ_LDS uint32_t* hot, const uint32_t* dtg,
+                                                 const FZ_LDS int16_t* vbuf, uint64_t ve0, uint64_t ve1,
+                                                 uint64_t novf, FZ_LDS int16_t* obuf) {
+    const FZ_LDS uint32_t* w32 = reinterpret_cast<const FZ_LDS uint32_t*>(win);
+    const FZ_LDS uint8_t* hotb = reinterpret_cast<const FZ_LDS uint8_t*>(hot);
+    const int64_t base = (wlo - kWinPad) * 8;
+    int32_t p = (int32_t)(S.B - base);
+    const bool raised = until < base + 32;
+    const int32_t lim = (int32_t)((raised ? base + 32 : until) - base);
+    uint32_t X4 = (S.X & ((1u << kO1Log) - 1)) << 2;
+    uint32_t r = dense_ctx(S.cx);
+    uint64_t e = S.e;
+    const uint64_t elim = ve1 < novf ? ve1 : ~0ull;
+    const uint32_t kmax = (uint32_t)min<uint64_t>(max_n, 0x7FFFFFFFull);
+    uint32_t k = 0;
+    bool bad = S.bad, stop = false;
+    // one step; returns false after the escape that reaches elim
+    auto step = [&]() {
+        const int32_t j = (p - 32) >> 5;
+        const uint32_t wlo32 = w32[j], whi32 = w32[j + 1];
+        uint32_t nb, base4, nr;
+        int code;
+        if (__builtin_expect(r < (uint32_t)kWalkCtxN, 1)) {
+            const uint32_t raw = *reinterpret_cast<const FZ_LDS uint32_t*>(hotb + ((r << (kO1Log + 2)) | (X4 & 0x7FFCu)));
+            nb = (raw >> 24) & 15u;
+            base4 = raw & 0x7FFFu;
+            code = __builtin_amdgcn_sbfe((int)raw, 15, 9);
+            nr = raw >> 28;
+        } else {
+            const uint32_t cx = r - (uint32_t)kWalkCtxN;  // r < 15 here
+            const uint32_t raw = dtg[((size_t)cx << kO1Log) + ((X4 >> 2) & ((1u << kO1Log) - 1))];
+            nb = (raw >> 24) & 15u;
+            base4 = (raw & 0x1FFFu) << 2;
+            const uint32_t sy = (raw >> 16) & 0xFFu;
+            code = (int)((sy >> 1) ^ (~(sy & 1u) + 1u));
+            nr = dense_ctx(raw >> 28);
+        }
+        p -= (int32_t)nb;
+        const uint64_t w = (uint64_t)wlo32 | ((uint64_t)whi32 << 32);
+        const uint32_t bits = __builtin_amdgcn_ubfe((uint32_t)(w >> (uint32_t)(p - 32 * j)), 0u, nb);
+        X4 = base4 + (bits << 2);
+        if (__builtin_expect(nr == 15u, 0)) {
+            const uint64_t ei = e - ve0;
+            if (e >= novf) { bad = true; code = 32767; }
+            else code = (int)vbuf[ei < (uint64_t)kDenseOvf ? ei : 0];
+            nr = dense_ctx((uint32_t)o1_ctx(code));
+            e++;
+            if (e >= elim) stop = true;
+        }
+        r = nr;
+        obuf[k] = (int16_t)code;
+        k++;
+    };
+    // four steps read at most 4 * 15 bits, write four codes and consume at most four escapes
+    while (p > lim + 60 && k + 4 <= kmax && (elim == ~0ull || e + 4 <= elim)) {
+        step();
+        step();
+        step();
+        step();
+        if (stop) break;
+    }
+    while (!stop && p > lim && k < kmax) step();
+    if (raised && p <= lim && k < kmax && !stop) bad = true;
+    S.B = base + p;
+    S.X = X4 >> 2;
+    S.cx = r == 15u ? 15u : (r < (uint32_t)kWalkCtxN ? r + (uint32_t)kWalkCtx0 : r - (uint32_t)kWalkCtxN);
+    S.e = e;
+    S.bad = bad;
+    return k;
+}
+
 // A chain whose escapes are dense (the coarse levels of a tight bound) cannot be split: every few
 // dozen symbols the next context depends on an escape whose index in the side stream is unknown
 // to a decoder that starts mid-chain. One workgroup decodes such a chain from its start: lane 0
@@ -1520,7 +1611,7 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, cons
     const O1DecChain c = ch[blockIdx.x];
     const uint32_t* dt = reinterpret_cast<const uint32_t*>(dt_all + c.dset * kO1Ctx * (1u << kO1Log));
     for (uint32_t i = threadIdx.x; i < (kWalkCtxN << kO1Log); i += blockDim.x)
-        hot[i] = dt[((size_t)kWalkCtx0 << kO1Log) + i];
+        hot[i] = dense_entry(dt[((size_t)kWalkCtx0 << kO1Log) + i]);
     const uint8_t* cb = in + c.off;
     const int16_t* ov = ovf + c.ovf_base;
     const uint64_t m = c.hi - c.lo;
@@ -1536,8 +1627,8 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, cons
         for (uint64_t k = e0 + threadIdx.x; k < ve; k += blockDim.x) vbuf[k - e0] = ov[k];
         __syncthreads();
         const int64_t floor_bits = lo == 0 ? INT64_MIN : lo * 8 + 64;
-        const uint64_t nk = seq_decode(z, lds(stage), lo, floor_bits, min<uint64_t>(m - i, kDenseOut), lds(hot), dt,
-                                       lds(vbuf), e0, ve, ov, c.novf, lds(obuf));
+        const uint64_t nk = dense_decode(z, lds(stage), lo, floor_bits, min<uint64_t>(m - i, kDenseOut), lds(hot), dt,
+                                         lds(vbuf), e0, ve, c.novf, lds(obuf));
         __syncthreads();
         for (uint64_t k = threadIdx.x; k < nk; k += blockDim.x) out[c.lo + i + k] = obuf[k];
         i += nk;
