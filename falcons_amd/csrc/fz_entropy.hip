@@ -1003,11 +1003,13 @@ __global__ void __launch_bounds__(256) k_o1_dtables(const uint8_t* stream, const
     const uint8_t* b = stream + nf_off[blockIdx.y] + 2 * ((size_t)blockIdx.x * kO1Sym + t);
     L.norm[t] = (int16_t)(uint16_t)(b[0] | (b[1] << 8));
     __syncthreads();
+    DEntry* dt = dt_all + ((size_t)blockIdx.y * kO1Ctx + blockIdx.x) * (1u << kO1Log);
     if (!spread_parallel(L, kO1Log)) {
         if (t == 0) *err = 1;
+        // a defined table (state 0, no bits, code 0) so that the decoders stay in range
+        for (uint32_t u = t; u < (1u << kO1Log); u += blockDim.x) dt[u] = DEntry{0, 0, 0};
         return;
     }
-    DEntry* dt = dt_all + ((size_t)blockIdx.y * kO1Ctx + blockIdx.x) * (1u << kO1Log);
     dtable_parallel(L, kO1Log, dt);
     __syncthreads();
     // the context that follows each entry's symbol rides in the top four bits of nbBits (15 after an
@@ -1167,21 +1169,39 @@ __device__ __forceinline__ uint64_t seq_decode(SeqState& S, const FZ_LDS uint8_t
     return k;
 }
 
-// Copies chain bytes [lo - kWinPad, hi) into win (bytes below the chain's start read as zero), eight
-// independent loads per lane per round.
-__device__ __forceinline__ void stage_window(FZ_LDS uint8_t* win, const uint8_t* cb, int64_t lo, int64_t hi) {
-    const int64_t base = lo - kWinPad, len = hi - base;
-    for (int64_t k0 = (int64_t)threadIdx.x * 8; k0 < len; k0 += (int64_t)blockDim.x * 8) {
-        uint32_t v[8];
+// Copies chain bytes [w0, hi) into win, where w0 <= lo - kWinPad is the nearest chain byte below
+// whose device address is 16-byte aligned, so that whole aligned 16-byte blocks move (one load per
+// block; blocks that cross the chain's first or last byte go byte by byte, and bytes outside
+// [0, len) read as zero). Returns w0: win[i] holds chain byte w0 + i. win must hold
+// hi - lo + kWinPad + 32 bytes.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int64_t stage_window(FZ_LDS uint8_t* win, const uint8_t* cb, int64_t len, int64_t lo,
+                                                int64_t hi) {
+    const int64_t s0 = lo - kWinPad;
+    const int64_t w0 = s0 - (int64_t)(((uintptr_t)cb + (uintptr_t)s0) & 15u);
+    const int64_t nblk = (hi - w0 + 15) / 16;
+    for (int64_t q = threadIdx.x; q < nblk; q += blockDim.x) {
+        const int64_t c0 = w0 + 16 * q;
+        u32x4 v;
+        if (c0 >= 0 && c0 + 16 <= len) {
+            v = *reinterpret_cast<const u32x4*>(cb + c0);
+        } else {
+            uint32_t w[4];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int64_t i = base + k0 + j;
-            v[j] = (k0 + j < len && i >= 0) ? cb[i] : 0u;
+            for (int j = 0; j < 4; j++) {
+                uint32_t x = 0;
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int64_t i = c0 + 4 * j + t;
+                    x |= (i >= 0 && i < len ? (uint32_t)cb[i] : 0u) << (8 * t);
+                }
+                w[j] = x;
+            }
+            v = u32x4{w[0], w[1], w[2], w[3]};
         }
-#pragma unroll
-        for (int j = 0; j < 8; j++)
-            if (k0 + j < len) win[k0 + j] = (uint8_t)v[j];
+        *reinterpret_cast<FZ_LDS u32x4*>(win + 16 * q) = v;
     }
+    return w0;
 }
 
 constexpr int kSeqLanes = 64;  // the sequential decoders run as one wave
@@ -1237,13 +1257,16 @@ __global__ void __launch_bounds__(256) k_o1d_init(const uint8_t* in, const O1Dec
         const DEntry* dt = dt_all + c.dset * kO1Ctx * (1u << kO1Log);
         BitWin rd = bwin_at(in + c.off, w);
         uint32_t X = (uint32_t)((r * 2654435761u) & ((1u << kO1Log) - 1)), cx = 0;
-        while (rd.pos > b) {
+        // symbols that read no bits do not move the position (a context with one symbol): bounded
+        uint64_t guard = (uint64_t)(w - b) * 4 + 4096;
+        while (rd.pos > b && guard--) {
             const DEntry d = dt[(size_t)cx * (1u << kO1Log) + (X & ((1u << kO1Log) - 1))];
             X = d.newState + bwin_read(rd, d.nbBits & 15u);
             const unsigned nc = d.nbBits >> 4;
             cx = nc == 15 ? 13u : nc;
         }
-        ss.s[0] = O1DecState{rd.pos, X, cx};
+        // a warm-up that never got there starts at the boundary (the walk then decodes it exactly)
+        ss.s[0] = O1DecState{rd.pos > b ? b : rd.pos, X & ((1u << kO1Log) - 1), cx};
     }
     start[t] = ss;
     run[t] = 1;
@@ -1425,7 +1448,7 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_walk(const uint8_t* in, const
                                                         const O1Graph* graph, O1DecState* res_start, uint64_t* res_e,
                                                         uint64_t* res_off, unsigned int* stats, int* err) {
     __shared__ uint32_t hot[kWalkCtxN << kO1Log];
-    __shared__ uint32_t stage32[(kWalkStage + kWinPad) / 4 + 2];
+    __shared__ __align__(16) uint32_t stage32[(kWalkStage + kWinPad + 32) / 4 + 4];
     uint8_t* stage = reinterpret_cast<uint8_t*>(stage32);
     const int q = blockIdx.x;
     if (q >= nch) return;
@@ -1487,12 +1510,12 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_walk(const uint8_t* in, const
             const int64_t hi = min<int64_t>((int64_t)c.len, (z.B + 7) / 8 + 1);
             const int64_t lo = max<int64_t>(0, hi - (int64_t)kWalkStage);
             __syncthreads();
-            stage_window(lds(stage), cb, lo, hi);
+            const int64_t w0 = stage_window(lds(stage), cb, (int64_t)c.len, lo, hi);
             __syncthreads();
             // window reads reach 64 bits below the read position: stay above the staged bytes unless
             // they start at the chain's first byte (bits below it read as zero)
             const int64_t floor_bits = lo == 0 ? INT64_MIN : lo * 8 + 64;
-            off += seq_decode(z, lds(stage), lo, max<int64_t>(stop, floor_bits), m - off, lds(hot), dt, nullptr, 0, 0,
+            off += seq_decode(z, lds(stage), w0 + kWinPad, max<int64_t>(stop, floor_bits), m - off, lds(hot), dt, nullptr, 0, 0,
                               ov, c.novf, nullptr);
         }
         if (z.bad) {
@@ -1504,90 +1527,134 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_walk(const uint8_t* in, const
 
 // Decoding-table entries of the dense-chain decoder. Contexts are renumbered so that the four
 // kept in LDS (kWalkCtx0 ..) come first: 0..3 are those, 4..14 are contexts 0..10, 15 follows an
-// escape. LDS entry: [31:28] renumbered next context, [27:24] bit count, [23:15] the decoded code
-// (signed, 9 bits), [14:0] the next state base times four (a byte offset into one table).
+// escape. The LDS tables of the four interleave by state: the entry of (state X, context r < 4)
+// sits at byte X * 16 + r * 4. Entry: [31] set when the next context is not in LDS (or the symbol
+// is an escape), [30:22] the decoded code (signed), [21:18] the bit count, [16:0] with [31] clear:
+// the byte offset of the next entry less the read bits times 16 (next state base * 16 + next
+// context * 4); with [31] set: next state base * 16 + (renumbered next context - 4).
+constexpr uint32_t kDnSlow = 1u << 31;
 __device__ __forceinline__ uint32_t dense_ctx(uint32_t c) {
     return c == 15u ? 15u : (c >= (uint32_t)kWalkCtx0 ? c - (uint32_t)kWalkCtx0 : c + (uint32_t)kWalkCtxN);
 }
 __device__ __forceinline__ uint32_t dense_entry(uint32_t raw) {
     const uint32_t sy = (raw >> 16) & 0xFFu;
     const uint32_t code = (sy >> 1) ^ (~(sy & 1u) + 1u);
-    return (dense_ctx(raw >> 28) << 28) | (((raw >> 24) & 15u) << 24) | ((code & 0x1FFu) << 15) |
-           ((raw & 0x1FFFu) << 2);
+    const uint32_t ns = raw & ((1u << kO1Log) - 1);
+    const uint32_t r = dense_ctx(raw >> 28);
+    const uint32_t low = r < (uint32_t)kWalkCtxN ? (ns << 4) | (r << 2) : kDnSlow | (ns << 4) | (r - kWalkCtxN);
+    return low | ((code & 0x1FFu) << 22) | (((raw >> 24) & 15u) << 18);
 }
 
-// The state machine of seq_decode for the dense-chain decoder (escaped values always from vbuf,
-// codes always to obuf, tables of the renumbered contexts 0..3 in LDS in the format above, the
-// others in device memory in the common format). Steps run in groups of four with one bounds check
-// per group while the group cannot reach the window floor, the output limit or the escape limit;
-// the last steps run one by one with every check.
-__device__ __forceinline__ uint64_t dense_decode(SeqState& S, const FZ_LDS uint8_t* win, int64_t wlo, int64_t until,
-                                                 uint64_t max_n, const FZ_LDS uint32_t* hot, const uint32_t* dtg,
+// Dense-format copies of the tables of contexts 0..10 for each dense chain: grid (chains, 11).
+__global__ void __launch_bounds__(256) k_o1d_dense_tables(const O1DecChain* ch, const DEntry* dt_all, uint32_t* dtd) {
+    const O1DecChain c = ch[blockIdx.x];
+    const uint32_t* dt = reinterpret_cast<const uint32_t*>(dt_all + c.dset * kO1Ctx * (1u << kO1Log)) +
+                         ((size_t)blockIdx.y << kO1Log);
+    uint32_t* o = dtd + ((size_t)blockIdx.x * kWalkCtx0 + blockIdx.y) * (1u << kO1Log);
+    for (uint32_t i = threadIdx.x; i < (1u << kO1Log); i += blockDim.x) o[i] = dense_entry(dt[i]);
+}
+
+__device__ __forceinline__ uint32_t lds_u32(const FZ_LDS uint8_t* base, uint32_t off) {
+    return *reinterpret_cast<const FZ_LDS uint32_t*>(base + off);
+}
+
+// The state machine of seq_decode for dense chains (escaped values from vbuf, codes to obuf).
+// The bits below the read position sit in a 64-bit buffer refilled from the LDS window once per
+// four steps (four steps read at most 60 bits); each step is one LDS read and a dozen scalar
+// operations unless the next context's table is in device memory or the symbol is an escape.
+// The last steps before a window, output or escape limit run one by one with every check.
+// E is the entry of the current state on entry and on return.
+__device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const FZ_LDS uint8_t* win, int64_t wlo,
+                                                 int64_t until,
+                                                 uint64_t max_n, const FZ_LDS uint8_t* hotb, const uint32_t* dtd,
                                                  const FZ_LDS int16_t* vbuf, uint64_t ve0, uint64_t ve1,
                                                  uint64_t novf, FZ_LDS int16_t* obuf) {
     const FZ_LDS uint32_t* w32 = reinterpret_cast<const FZ_LDS uint32_t*>(win);
-    const FZ_LDS uint8_t* hotb = reinterpret_cast<const FZ_LDS uint8_t*>(hot);
     const int64_t base = (wlo - kWinPad) * 8;
     int32_t p = (int32_t)(S.B - base);
     const bool raised = until < base + 32;
     const int32_t lim = (int32_t)((raised ? base + 32 : until) - base);
-    uint32_t X4 = (S.X & ((1u << kO1Log) - 1)) << 2;
-    uint32_t r = dense_ctx(S.cx);
     uint64_t e = S.e;
     const uint64_t elim = ve1 < novf ? ve1 : ~0ull;
     const uint32_t kmax = (uint32_t)min<uint64_t>(max_n, 0x7FFFFFFFull);
     uint32_t k = 0;
     bool bad = S.bad, stop = false;
-    // one step; returns false after the escape that reaches elim
+    // the entry of the current state
+    auto fetch = [&](uint32_t X, uint32_t r) -> uint32_t {
+        return r < (uint32_t)kWalkCtxN ? lds_u32(hotb, (X << 4) | (r << 2))
+                                       : dtd[((size_t)(r - kWalkCtxN) << kO1Log) + X];
+    };
+    asm volatile("" : "+v"(E));
+    uint64_t buf = 0;
+    int32_t prel = 0;
+    // bits [p - prel, p) of the window into buf (p >= 33 wherever this runs)
+    auto refill = [&]() {
+        const int32_t lo = max(p - 64, 0);
+        const int32_t j0 = lo >> 5;
+        const uint32_t w0 = w32[j0], w1 = w32[j0 + 1], w2 = w32[j0 + 2];
+        const uint32_t sh = (uint32_t)lo & 31u;
+        buf = (uint64_t)__builtin_amdgcn_alignbit(w1, w0, sh) | ((uint64_t)__builtin_amdgcn_alignbit(w2, w1, sh) << 32);
+        prel = p - lo;
+    };
+    // One step. The table entry, the bit buffer and the read position stay in vector registers (the
+    // values are the same in every lane): scalar arithmetic on them would put a lane-to-scalar move
+    // and its latency on the critical path of every step. Only the rare-path test reads E into a
+    // scalar register. stop is set after the escape that reaches elim (in a group of four, which
+    // starts with e + 4 <= elim, only by its last step).
     auto step = [&]() {
-        const int32_t j = (p - 32) >> 5;
-        const uint32_t wlo32 = w32[j], whi32 = w32[j + 1];
-        uint32_t nb, base4, nr;
-        int code;
-        if (__builtin_expect(r < (uint32_t)kWalkCtxN, 1)) {
-            const uint32_t raw = *reinterpret_cast<const FZ_LDS uint32_t*>(hotb + ((r << (kO1Log + 2)) | (X4 & 0x7FFCu)));
-            nb = (raw >> 24) & 15u;
-            base4 = raw & 0x7FFFu;
-            code = __builtin_amdgcn_sbfe((int)raw, 15, 9);
-            nr = raw >> 28;
-        } else {
-            const uint32_t cx = r - (uint32_t)kWalkCtxN;  // r < 15 here
-            const uint32_t raw = dtg[((size_t)cx << kO1Log) + ((X4 >> 2) & ((1u << kO1Log) - 1))];
-            nb = (raw >> 24) & 15u;
-            base4 = (raw & 0x1FFFu) << 2;
-            const uint32_t sy = (raw >> 16) & 0xFFu;
-            code = (int)((sy >> 1) ^ (~(sy & 1u) + 1u));
-            nr = dense_ctx(raw >> 28);
+        const uint32_t nb = (E >> 18) & 15u;
+        prel -= (int32_t)nb;
+        const uint32_t bits = (uint32_t)(buf >> ((uint32_t)prel & 63u)) & ((1u << nb) - 1u);
+        int code = __builtin_amdgcn_sbfe((int)E, 22, 9);
+        // the next hot entry, read before the rare-path test (its address stays inside the tables
+        // for every entry)
+        uint32_t nx = lds_u32(hotb, (E & 0x1FFFFu) + (bits << 4));
+        const uint32_t Es = __builtin_amdgcn_readfirstlane(E);
+        if (__builtin_expect((Es & kDnSlow) != 0, 0)) {
+            const uint32_t X = (((E >> 4) & ((1u << kO1Log) - 1)) + bits) & ((1u << kO1Log) - 1);
+            uint32_t r = (Es & 15u) + (uint32_t)kWalkCtxN;
+            if (r == 15u) {
+                const uint64_t ei = e - ve0;
+                if (e >= novf) { bad = true; code = 32767; }
+                else code = (int)vbuf[ei < (uint64_t)kDenseOvf ? ei : 0];
+                r = dense_ctx((uint32_t)o1_ctx(code));
+                e++;
+                if (e >= elim) stop = true;
+            }
+            nx = fetch(X, r);
         }
-        p -= (int32_t)nb;
-        const uint64_t w = (uint64_t)wlo32 | ((uint64_t)whi32 << 32);
-        const uint32_t bits = __builtin_amdgcn_ubfe((uint32_t)(w >> (uint32_t)(p - 32 * j)), 0u, nb);
-        X4 = base4 + (bits << 2);
-        if (__builtin_expect(nr == 15u, 0)) {
-            const uint64_t ei = e - ve0;
-            if (e >= novf) { bad = true; code = 32767; }
-            else code = (int)vbuf[ei < (uint64_t)kDenseOvf ? ei : 0];
-            nr = dense_ctx((uint32_t)o1_ctx(code));
-            e++;
-            if (e >= elim) stop = true;
-        }
-        r = nr;
         obuf[k] = (int16_t)code;
         k++;
+        E = nx;
+        asm volatile("" : "+v"(E));
     };
-    // four steps read at most 4 * 15 bits, write four codes and consume at most four escapes
-    while (p > lim + 60 && k + 4 <= kmax && (elim == ~0ull || e + 4 <= elim)) {
-        step();
-        step();
-        step();
-        step();
+    // groups of four steps after one refill; g groups at a time stay clear of the window floor (a
+    // group reads at most 60 bits), the output limit and the escape limit
+    for (;;) {
+        uint32_t g = p > lim + 60 ? (uint32_t)((p - lim - 61) / 60 + 1) : 0u;
+        g = min(g, (kmax - k) / 4);
+        if (elim != ~0ull) g = (uint32_t)min<uint64_t>(g, (elim - e) / 4);
+        if (g == 0) break;
+        for (; g; g--) {
+            refill();
+            const int32_t lo = p - prel;
+            step();
+            step();
+            step();
+            step();
+            p = lo + prel;
+            if (stop) break;
+        }
         if (stop) break;
     }
-    while (!stop && p > lim && k < kmax) step();
+    while (!stop && p > lim && k < kmax) {
+        refill();
+        const int32_t lo = p - prel;
+        step();
+        p = lo + prel;
+    }
     if (raised && p <= lim && k < kmax && !stop) bad = true;
     S.B = base + p;
-    S.X = X4 >> 2;
-    S.cx = r == 15u ? 15u : (r < (uint32_t)kWalkCtxN ? r + (uint32_t)kWalkCtx0 : r - (uint32_t)kWalkCtxN);
     S.e = e;
     S.bad = bad;
     return k;
@@ -1595,40 +1662,45 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, const FZ_LDS uint8
 
 // A chain whose escapes are dense (the coarse levels of a tight bound) cannot be split: every few
 // dozen symbols the next context depends on an escape whose index in the side stream is unknown
-// to a decoder that starts mid-chain. One workgroup decodes such a chain from its start: lane 0
-// runs the state machine against LDS only where it can (the decoding tables of contexts 11-14,
-// which hold most of the symbols after large codes, a window of the bitstream, a window of the
-// escaped values, and a buffer of decoded codes); the workgroup refills the windows and writes the
-// codes out between rounds.
+// to a decoder that starts mid-chain. One workgroup decodes such a chain from its start, every lane
+// running the same (wave-uniform) state machine: the tables of contexts 11-14, which hold most of
+// the symbols after large codes, a window of the bitstream, a window of the escaped values and a
+// buffer of decoded codes sit in LDS; the workgroup refills the windows and writes the codes out
+// between rounds.
 
 __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, const O1DecChain* ch, const DEntry* dt_all,
-                                                         const int16_t* ovf, int16_t* out, int* err) {
+                                                         const uint32_t* dtd_all, const int16_t* ovf, int16_t* out,
+                                                         int* err) {
     __shared__ uint32_t hot[kWalkCtxN << kO1Log];
-    __shared__ uint32_t stage32[(kDenseStage + kWinPad) / 4 + 2];
+    __shared__ __align__(16) uint32_t stage32[(kDenseStage + kWinPad + 32) / 4 + 4];
     uint8_t* stage = reinterpret_cast<uint8_t*>(stage32);
     __shared__ int16_t obuf[kDenseOut];
     __shared__ int16_t vbuf[kDenseOvf];
     const O1DecChain c = ch[blockIdx.x];
     const uint32_t* dt = reinterpret_cast<const uint32_t*>(dt_all + c.dset * kO1Ctx * (1u << kO1Log));
+    const uint32_t* dtd = dtd_all + (size_t)blockIdx.x * kWalkCtx0 * (1u << kO1Log);
+    // interleaved by state: hot[X * 4 + r] holds (state X, context kWalkCtx0 + r)
     for (uint32_t i = threadIdx.x; i < (kWalkCtxN << kO1Log); i += blockDim.x)
-        hot[i] = dense_entry(dt[((size_t)kWalkCtx0 << kO1Log) + i]);
+        hot[i] = dense_entry(dt[((size_t)(kWalkCtx0 + (i & 3u)) << kO1Log) + (i >> 2)]);
     const uint8_t* cb = in + c.off;
     const int16_t* ov = ovf + c.ovf_base;
     const uint64_t m = c.hi - c.lo;
-    // every lane runs the same state machine on the same data (the values stay wave-uniform)
     SeqState z{c.pos0, c.x0, 0, 0, false};
+    // the chain starts in context 0, whose table is in device memory
+    uint32_t E = dtd[((size_t)(dense_ctx(0) - kWalkCtxN) << kO1Log) + (c.x0 & ((1u << kO1Log) - 1))];
     uint64_t i = 0;
     while (i < m && !z.bad) {
         const int64_t hi = min<int64_t>((int64_t)c.len, (z.B + 7) / 8 + 1);
         const int64_t lo = max<int64_t>(0, hi - (int64_t)kDenseStage);
         const uint64_t e0 = z.e, ve = min<uint64_t>(c.novf, e0 + kDenseOvf);
         __syncthreads();
-        stage_window(lds(stage), cb, lo, hi);
+        const int64_t w0 = stage_window(lds(stage), cb, (int64_t)c.len, lo, hi);
         for (uint64_t k = e0 + threadIdx.x; k < ve; k += blockDim.x) vbuf[k - e0] = ov[k];
         __syncthreads();
         const int64_t floor_bits = lo == 0 ? INT64_MIN : lo * 8 + 64;
-        const uint64_t nk = dense_decode(z, lds(stage), lo, floor_bits, min<uint64_t>(m - i, kDenseOut), lds(hot), dt,
-                                         lds(vbuf), e0, ve, c.novf, lds(obuf));
+        const uint64_t nk = dense_decode(z, E, lds(stage), w0 + kWinPad, floor_bits, min<uint64_t>(m - i, kDenseOut),
+                                         lds(reinterpret_cast<const uint8_t*>(hot)), dtd, lds(vbuf), e0, ve, c.novf,
+                                         lds(obuf));
         __syncthreads();
         for (uint64_t k = threadIdx.x; k < nk; k += blockDim.x) out[c.lo + i + k] = obuf[k];
         i += nk;
@@ -2135,7 +2207,9 @@ size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out
 }
 
 size_t entropy_decode_workspace(uint64_t n, size_t len) {
-    return n * 8 * 3 + n * 2 * 2 + len * 2 + 16 * kO1Ctx * (sizeof(DEntry) + 1) * (1u << kO1Log) + (96u << 20);
+    // (the last term: dense-format tables of contexts 0..10 for up to 64 chains decoded from their start)
+    return n * 8 * 3 + n * 2 * 2 + len * 2 + 16 * kO1Ctx * (sizeof(DEntry) + 1) * (1u << kO1Log) + (96u << 20) +
+           (size_t)64 * kWalkCtx0 * 4 * (1u << kO1Log);
 }
 
 static uint64_t get64(const uint8_t*& p) {
@@ -2204,6 +2278,9 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
         if (m && c[6] * 4096 > kDenseEscapes * m) dense_seg.push_back(c[5]);
     }
     auto in_dense_seg = [&](uint64_t g) { return std::find(dense_seg.begin(), dense_seg.end(), g) != dense_seg.end(); };
+    // FALCONS_O1_SEQUENTIAL=1 decodes every chain from its start (the dense-chain decoder)
+    const char* seq_env = getenv("FALCONS_O1_SEQUENTIAL");
+    const bool all_seq = seq_env && atoi(seq_env) != 0;
     for (int q = 0; q < nch; q++) {
         const uint64_t* c = &chains[7 * q];
         const uint64_t m = c[3] - c[2];
@@ -2217,7 +2294,7 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
             x0 = (x0 << 1) | (bit >= 0 ? ((b[bit >> 3] >> (bit & 7)) & 1u) : 0u);
         }
         const int64_t pos0 = top - (int64_t)kO1Log;
-        if (c[6] * 4096 > kDenseEscapes * m || (in_dense_seg(c[5]) && c[6] * 16384 > m)) {
+        if (all_seq || c[6] * 4096 > kDenseEscapes * m || (in_dense_seg(c[5]) && c[6] * 16384 > m)) {
             hd.push_back(O1DecChain{c[0], c[1], c[2], c[3], c[4], c[6], c[5], pos0, x0, 1, 0, 0});
             dense_bytes += c[1];
             dense_codes += m;
@@ -2241,8 +2318,11 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
             FZ_HIP(hipEventRecord(aux.fork, st));
             FZ_HIP(hipStreamWaitEvent(aux.stream, aux.fork, 0));
         }
+        uint32_t* dtd = ws.take<uint32_t>(hd.size() * kWalkCtx0 * (1u << kO1Log));
+        FZ_LAUNCH("o1_decode_dense_tables", 0, k_o1d_dense_tables, dim3((unsigned)hd.size(), (unsigned)kWalkCtx0),
+                  dim3(256), 0, dst, ddc, dt, dtd);
         FZ_LAUNCH("o1_decode_dense", dense_bytes + dense_codes * 2, k_o1d_dense, dim3((unsigned)hd.size()), dim3(kSeqLanes), 0,
-                  dst, dev, ddc, dt, ovf, out, err);
+                  dst, dev, ddc, dt, dtd, ovf, out, err);
         FZ_HIP(hipGetLastError());
     }
     struct Join {  // the caller's stream waits for the dense chains however this function exits

@@ -190,3 +190,23 @@ def test_truncated_stream_raises(ctx):
     for cut in (3, 20, len(stream) // 2, len(stream) - 1):
         with pytest.raises(ValueError):
             ctx.decompress_ex(stream[:cut], np.float32)
+
+
+SEQ_CASES = [
+    ("no-delta", lambda: synth.power_law((96, 96, 96), -3.0, seed=5), dict(eb_mode=1, rel_eb=1e-4)),
+    ("delta", lambda: quad3(), dict(eb_mode=0, abs_eb=2e-3, interp_algo=0)),
+    ("cubic-steps", lambda: quad3(steps=True) ** 1.5, dict(eb_mode=0, abs_eb=1e-5, interp_algo=1)),
+    ("escapes", lambda: spiky((80, 72, 90), np.float32), dict(eb_mode=0, abs_eb=1e-3)),
+    ("nan-inf", lambda: nan_inf((40, 50, 61), np.float64), dict(eb_mode=0, abs_eb=1e-3)),
+    ("dense-escapes", lambda: synth.power_law((128, 128, 128), -3.0, seed=8), dict(eb_mode=1, rel_eb=2e-5)),
+]
+
+
+@pytest.mark.parametrize("case", SEQ_CASES, ids=[c[0] for c in SEQ_CASES])
+def test_o1_sequential_decoder(ctx, oracle, case, monkeypatch):
+    """Every order-1 tANS chain through the sequential decoder (FALCONS_O1_SEQUENTIAL=1), which the
+    library otherwise keeps for chains with dense escapes: same codes as the oracle."""
+    _, make, kw = case
+    data = make()
+    monkeypatch.setenv("FALCONS_O1_SEQUENTIAL", "1")
+    check(ctx, oracle, data, **kw)
