@@ -1601,7 +1601,7 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
     // and its latency on the critical path of every step. Only the rare-path test reads E into a
     // scalar register. stop is set after the escape that reaches elim (in a group of four, which
     // starts with e + 4 <= elim, only by its last step).
-    auto step = [&]() {
+    auto step = [&]() -> uint32_t {
         const uint32_t nb = (E >> 18) & 15u;
         prel -= (int32_t)nb;
         const uint32_t bits = (uint32_t)(buf >> ((uint32_t)prel & 63u)) & ((1u << nb) - 1u);
@@ -1623,10 +1623,9 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
             }
             nx = fetch(X, r);
         }
-        obuf[k] = (int16_t)code;
-        k++;
         E = nx;
         asm volatile("" : "+v"(E));
+        return (uint32_t)code & 0xFFFFu;
     };
     // groups of four steps after one refill; g groups at a time stay clear of the window floor (a
     // group reads at most 60 bits), the output limit and the escape limit
@@ -1635,13 +1634,16 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
         g = min(g, (kmax - k) / 4);
         if (elim != ~0ull) g = (uint32_t)min<uint64_t>(g, (elim - e) / 4);
         if (g == 0) break;
+        // k is a multiple of four here (groups start at 0 and take four codes each)
         for (; g; g--) {
             refill();
             const int32_t lo = p - prel;
-            step();
-            step();
-            step();
-            step();
+            const uint32_t c0 = step();
+            const uint32_t c1 = step();
+            const uint32_t c2 = step();
+            const uint32_t c3 = step();
+            *reinterpret_cast<FZ_LDS uint64_t*>(obuf + k) = (uint64_t)(c0 | (c1 << 16)) | ((uint64_t)(c2 | (c3 << 16)) << 32);
+            k += 4;
             p = lo + prel;
             if (stop) break;
         }
@@ -1650,7 +1652,8 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
     while (!stop && p > lim && k < kmax) {
         refill();
         const int32_t lo = p - prel;
-        step();
+        obuf[k] = (int16_t)step();
+        k++;
         p = lo + prel;
     }
     if (raised && p <= lim && k < kmax && !stop) bad = true;
@@ -1674,7 +1677,7 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, cons
     __shared__ uint32_t hot[kWalkCtxN << kO1Log];
     __shared__ __align__(16) uint32_t stage32[(kDenseStage + kWinPad + 32) / 4 + 4];
     uint8_t* stage = reinterpret_cast<uint8_t*>(stage32);
-    __shared__ int16_t obuf[kDenseOut];
+    __shared__ __align__(16) int16_t obuf[kDenseOut];
     __shared__ int16_t vbuf[kDenseOvf];
     const O1DecChain c = ch[blockIdx.x];
     const uint32_t* dt = reinterpret_cast<const uint32_t*>(dt_all + c.dset * kO1Ctx * (1u << kO1Log));
