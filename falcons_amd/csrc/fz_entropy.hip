@@ -1712,6 +1712,9 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, cons
 }
 
 // Final pass: every subsequence decodes from its settled start into its place in the code array.
+// Its code count is the distance to the next subsequence's output offset, so codes leave in
+// aligned 64-byte runs (32 codes packed in registers, four 16-byte stores) between a head and a
+// tail written one by one: whole lines reach L2 instead of two bytes at a time.
 __global__ void __launch_bounds__(256) k_o1d_out(const uint8_t* in, const O1DecChain* ch, int nch, uint64_t nsub,
                                                  const DEntry* dt_all, const int16_t* ovf, const O1DecState* start,
                                                  const uint64_t* e_start, const uint64_t* res_off, int16_t* out,
@@ -1723,16 +1726,40 @@ __global__ void __launch_bounds__(256) k_o1d_out(const uint8_t* in, const O1DecC
     const DEntry* dt = dt_all + c.dset * kO1Ctx * (1u << kO1Log);
     const bool last = r + 1 >= c.nsub;
     const int64_t stop = last ? INT64_MIN : c.pos0 - (int64_t)(r + 1) * c.sub_bits;
+    const uint64_t m = c.hi - c.lo;
+    const uint64_t o0 = res_off[t], o1 = last ? m : res_off[t + 1];
+    if (o0 > o1 || o1 > m) {  // offsets from a corrupt stream
+        *err = 1;
+        return;
+    }
     O1DecState s = start[t];
     uint64_t e = e_start[t];
-    uint64_t i = c.lo + res_off[t];
+    uint64_t i = c.lo + o0;
+    const uint64_t iend = c.lo + o1;
     bool bad = false;
     BitWin rd = bwin_at(in + c.off, s.B);
-    while (i < c.hi && rd.pos > stop) {
-        out[i++] = (int16_t)o1d_step(dt, rd, s.X, s.ctx, e, ovf + c.ovf_base, c.novf, bad);
+    int64_t before = rd.pos;  // read position before the latest symbol
+    auto step = [&]() -> uint32_t {
+        before = rd.pos;
+        return (uint32_t)o1d_step(dt, rd, s.X, s.ctx, e, ovf + c.ovf_base, c.novf, bad) & 0xFFFFu;
+    };
+    while (i < iend && (i & 31)) out[i++] = (int16_t)step();
+    while (i + 32 <= iend) {
+        uint32_t w[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const uint32_t a = step();
+            w[j] = a | (step() << 16);
+        }
+        uint32_t* o = reinterpret_cast<uint32_t*>(out + i);
+#pragma unroll
+        for (int j = 0; j < 16; j += 4)
+            *reinterpret_cast<u32x4*>(o + j) = u32x4{w[j], w[j + 1], w[j + 2], w[j + 3]};
+        i += 32;
     }
-    s.B = rd.pos;
-    if (bad || (last && (i != c.hi || e != c.novf)) || (!last && s.B > stop)) *err = 1;
+    while (i < iend) out[i++] = (int16_t)step();
+    // the subsequence's symbols are those that start above the next boundary
+    if (bad || (last && e != c.novf) || (!last && o1 > o0 && (rd.pos > stop || before <= stop))) *err = 1;
 }
 
 
