@@ -466,7 +466,8 @@ def bench_serving(a, fa, synth):
         if res["status"] != 0:
             raise SystemExit(f"request {res['tag']} failed: {res['msg']}")
         if res["tag"] >= a.warm_requests:
-            done.append((res["tag"], res["t_submit_ms"], res["t_done_ms"], res["field_bytes"]))
+            done.append((res["tag"], res["t_submit_ms"], res["t_done_ms"], res["field_bytes"], res["op"],
+                         res["t_done_ms"] - res["t_start_ms"], res["t_device_ms"], res["t_zstd_ms"]))
 
     t_start = time.perf_counter()
     due = t_start
@@ -519,6 +520,13 @@ def bench_serving(a, fa, synth):
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                        "mean": round(float(lat.mean()), 3), "max": round(float(lat.max()), 3)},
         "server": st,
+        # per operation: mean time in a slot, of which device time and host zstd time
+        "service_ms": {name: {"count": len(sel), "slot": round(float(np.mean([d[5] for d in sel])), 3),
+                              "device": round(float(np.mean([d[6] for d in sel])), 3),
+                              "zstd": round(float(np.mean([d[7] for d in sel])), 3),
+                              "field_MB": round(float(np.mean([d[3] for d in sel])) / 1e6, 2)}
+                       for name, op in (("compress", 0), ("decompress", 1))
+                       for sel in [[d for d in done if d[4] == op]] if sel},
     }
     print(json.dumps(line), flush=True)
 

@@ -419,16 +419,18 @@ struct Timer {
 
 }  // namespace
 
+// Workspace a call is expected to take: the peak measured on k^-3 fields with the order-1 tANS
+// coder (34 B per float element to compress, 16 B to decompress) with a margin. The arena grows
+// beyond it on demand (RLE+FSE streams and escape-heavy fields take more), so these size the first
+// allocation and the admission of served requests, not a hard limit.
 size_t fz::compress_workspace_bytes(uint64_t n, int dtype) {
     const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
-    return n * es * 3 + n * 2 + n * 8 + entropy_encode_bound(n) + select_escapes_scratch(n) +
-           entropy_encode_workspace(n) + tune_workspace_bytes(n, dtype) + (8u << 20);
+    return n * (3 * es + 24) + (64u << 20);
 }
 
 size_t fz::decompress_workspace_bytes(uint64_t n, int dtype, uint64_t nunpred, size_t pre_len) {
     const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
-    return n * es + n * 2 + n * 8 + nunpred * es + pre_len + select_escapes_scratch(n) +
-           entropy_decode_workspace(n, pre_len) + (16u << 20);
+    return n * (es + 14) + nunpred * es + 2 * pre_len + (64u << 20);
 }
 
 namespace {
@@ -438,6 +440,18 @@ namespace {
 // Byte size of decomposition.save without the unpredictable values (IDH:241-252,
 // LinearQuantizer.hpp:142-152).
 size_t decomp_head_bytes(int N) { return 8 * (size_t)N + 4 + 4 + 4 + 8 + 8 + 8 + 1 + 8 + 4 + 4; }
+
+// With FALCONS_ARENA_STATS set, each call reports the workspace it reserved and the most it took.
+struct ArenaReport {
+    DevArena& a;
+    const char* what;
+    uint64_t n;
+    ~ArenaReport() {
+        if (getenv("FALCONS_ARENA_STATS"))
+            fprintf(stderr, "[arena] %s n=%llu reserved %zu peak %zu\n", what, (unsigned long long)n, a.capacity(),
+                    a.peak());
+    }
+};
 
 // The compression pipeline on the GPU. d_data (device) holds the field and is overwritten with the
 // reconstructed values. The buffer handed to the zstd stage is assembled in device memory, at d_pre
@@ -450,6 +464,8 @@ size_t gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, int simd
     DevArena& ws = ctx->arena;
     ws.reset();
     ws.reserve(compress_workspace_bytes(n, dtype));
+    ws.clear_peak();
+    ArenaReport arena_report{ws, "compress", n};
     Timer tm{st, ctx->ev0, ctx->ev1};
     double* eb_dev = ws.take<double>(1);
     void* mm_scratch = ws.take<uint8_t>(minmax_scratch_bytes());
@@ -597,6 +613,8 @@ void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h
     DevArena& ws = ctx->arena;
     ws.reset();
     ws.reserve(decompress_workspace_bytes(n, dtype, (uint64_t)nun, pre_len));
+    ws.clear_peak();
+    ArenaReport arena_report{ws, "decompress", n};
     void* d_out = *d_out_io ? *d_out_io : ws.take<uint8_t>(n * es);
     *d_out_io = d_out;
     const uint8_t* d_enc;

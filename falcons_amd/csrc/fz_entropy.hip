@@ -1827,25 +1827,6 @@ static void h2d(T* d, const T* h, size_t n, hipStream_t st) {
     FZ_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, st));
 }
 
-size_t entropy_encode_workspace(uint64_t n) {
-    // delta copy, positions, class bytes, mantissas, block slots, scan scratch and o1 chain buffers
-    const uint64_t blocks = n / kFblk + 2;
-    size_t b = 0;
-    b += n * 2 + 4096;                   // delta
-    b += n * 8 + 4096;                   // positions
-    b += 2 * (n + 4096);                 // rcl, vcl
-    b += 2 * (n * 8 + 4096);             // bit counts
-    b += 2 * (n * 8 + 4096);             // bit offsets
-    b += (n * 3 + 4096) * 2;             // mantissas
-    b += 4 * blocks * (kFblk + 1024);    // block tmp + slots (two arrays)
-    b += n * 2 * 2 + (1 << 20);          // o1 chain buffers (2 bytes per symbol + slack)
-    b += n * 2 + 64 * kO1Piece * 2 + (n / kO1Piece + 1024) * 64;  // o1 piece state log and per-piece arrays
-    b += 16 * kO1Ctx * sizeof(CTable) + 16 * kO1Ctx * (1u << kO1Log);
-    b += 3 * (n * 2 + 4 * (kFblk + 1024)) + (1 << 20);  // ovf
-    b += 64u << 20;                      // cub scratch
-    return b;
-}
-
 // Scratch for cub calls is sized per call from the arena.
 static void* cub_scratch(DevArena& ws, size_t bytes) { return ws.take<uint8_t>(bytes ? bytes : 1); }
 
@@ -2234,12 +2215,6 @@ size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out
     const size_t body = (method == 1) ? encode_method1(bins, n, out + 5, out_cap - 5, ws, st)
                                       : encode_method0(bins, n, out + 5, out_cap - 5, ws, st);
     return 5 + body;
-}
-
-size_t entropy_decode_workspace(uint64_t n, size_t len) {
-    // (the last term: dense-format tables of contexts 0..10 for up to 64 chains decoded from their start)
-    return n * 8 * 3 + n * 2 * 2 + len * 2 + 16 * kO1Ctx * (sizeof(DEntry) + 1) * (1u << kO1Log) + (96u << 20) +
-           (size_t)64 * kWalkCtx0 * 4 * (1u << kO1Log);
 }
 
 static uint64_t get64(const uint8_t*& p) {

@@ -10,41 +10,82 @@
 
 namespace fz {
 
-// Bump allocator over one growing device allocation; reset at the start of each call.
+// Bump allocator over device blocks; reset at the start of each call. reserve() sizes the arena
+// for a call (one block when it grows while empty); take() adds a block when the current ones are
+// full, so an estimate below the true need costs an allocation but never a failure, and a grown
+// arena is not freed (hipFree synchronizes the whole device). Positions (used(), rewind(), peak())
+// count through the blocks in order.
 class DevArena {
    public:
     ~DevArena() { release(); }
     void reserve(size_t bytes) {
+        bytes = (bytes + 255) & ~size_t(255);
         if (bytes <= cap_) return;
-        release();
-        FZ_HIP(hipMalloc(&base_, bytes));
-        cap_ = bytes;
+        if (used_ == 0 && !blocks_.empty() && blocks_[0].cap < bytes) release();  // regrow as one block
+        if (blocks_.empty()) add_block(bytes);
+        else add_block(bytes - cap_);
     }
-    void reset() { used_ = 0; }
+    void reset() {
+        cur_ = 0;
+        off_ = 0;
+        used_ = 0;
+    }
     template <typename T>
     T* take(size_t count) {
         const size_t bytes = ((count * sizeof(T)) + 255) & ~size_t(255);
-        if (used_ + bytes > cap_) throw GpuError("device workspace exhausted");
-        T* p = reinterpret_cast<T*>(static_cast<char*>(base_) + used_);
+        while (cur_ < blocks_.size() && off_ + bytes > blocks_[cur_].cap) {  // the rest of this block is skipped
+            used_ += blocks_[cur_].cap - off_;
+            cur_++;
+            off_ = 0;
+        }
+        if (cur_ == blocks_.size()) add_block(std::max(bytes, std::max<size_t>(size_t(256) << 20, cap_ / 2)));
+        T* p = reinterpret_cast<T*>(static_cast<char*>(blocks_[cur_].p) + off_);
+        off_ += bytes;
         used_ += bytes;
+        peak_ = used_ > peak_ ? used_ : peak_;
         return p;
     }
     size_t used() const { return used_; }
+    // most taken at once since the last clear_peak()
+    size_t peak() const { return peak_; }
+    void clear_peak() { peak_ = used_; }
     // Gives back everything taken after `mark` (a value of used()); the caller has finished with it
     // in stream order.
     void rewind(size_t mark) {
-        if (mark <= used_) used_ = mark;
+        if (mark > used_) return;
+        size_t base = 0;
+        for (size_t k = 0; k < blocks_.size(); k++) {
+            if (mark <= base + blocks_[k].cap) {
+                cur_ = k;
+                off_ = mark - base;
+                used_ = mark;
+                return;
+            }
+            base += blocks_[k].cap;
+        }
     }
     size_t capacity() const { return cap_; }
     void release() {
-        if (base_) (void)hipFree(base_);
-        base_ = nullptr;
-        cap_ = used_ = 0;
+        for (Block& b : blocks_) (void)hipFree(b.p);
+        blocks_.clear();
+        cap_ = used_ = off_ = 0;
+        cur_ = 0;
     }
 
    private:
-    void* base_ = nullptr;
-    size_t cap_ = 0, used_ = 0;
+    struct Block {
+        void* p;
+        size_t cap;
+    };
+    void add_block(size_t bytes) {
+        bytes = (bytes + 255) & ~size_t(255);
+        void* p = nullptr;
+        FZ_HIP(hipMalloc(&p, bytes));
+        blocks_.push_back(Block{p, bytes});
+        cap_ += bytes;
+    }
+    std::vector<Block> blocks_;
+    size_t cap_ = 0, used_ = 0, off_ = 0, peak_ = 0, cur_ = 0;
 };
 
 // interpolation kernels (fz_interp.hip)
@@ -74,14 +115,11 @@ struct EntropyStats {
 // a symbol and 2 side-stream bytes on an escaped code; RLE+FSE at most 3 bytes per code; plus the
 // per-segment and per-block headers and tables.
 inline size_t entropy_encode_bound(uint64_t n) { return (size_t)n * 4 + (1u << 20); }
-// Device workspace the encoder needs for n codes.
-size_t entropy_encode_workspace(uint64_t n);
 // Encodes n codes (device) into out (device, capacity out_cap; std::length_error, before any write
 // past it, when the output would not fit). Returns the byte count of
 // [u8 method|delta<<7][u16 lo][u16 mode][payload] (RleFseEncoder::save + ::encode).
 size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out_cap, DevArena& ws, hipStream_t st,
                       EntropyStats* stats);
-size_t entropy_decode_workspace(uint64_t n, size_t len);
 // Decodes the encoder bytes (host copy `host`, device copy `dev`, length len) into n codes (device).
 // A second stream for work that can run beside the main one (null: run on the main stream), with the
 // two events that fork it from and join it back to the main stream.

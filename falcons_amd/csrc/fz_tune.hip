@@ -586,14 +586,6 @@ int tune_threads() {
     return e ? atoi(e) : 8;
 }
 
-size_t tune_workspace_bytes(uint64_t n, int dtype) {
-    // blocks hold at most 0.2 n values; per candidate: a work copy, codes, escape values and indices,
-    // the unpredictable list and the compaction scratch
-    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
-    const uint64_t m = n / 5 + 1;
-    return m * (3 * es + 2 + 8) + m * es + select_scratch_bytes(m) + n / 8 + (4u << 20);
-}
-
 int tune_interp(fz_config& c, int dtype, const void* d_data, DevArena& ws, hipStream_t st, TuneReport* rep) {
     if (rep) rep->n = 0;
     if (c.interp_anchor_stride < 0) {

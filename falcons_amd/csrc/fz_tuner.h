@@ -19,7 +19,6 @@ struct TuneReport {
 // Lorenzo/regression algorithm (1-D fields below ratio 50). Workspace comes from ws and is given
 // back before returning.
 int tune_interp(fz_config& c, int dtype, const void* d_data, DevArena& ws, hipStream_t st, TuneReport* rep);
-size_t tune_workspace_bytes(uint64_t n, int dtype);
 // omp_get_max_threads() of the reference build decides the 2-D candidate set
 // (FalconsAlgoInterp.hpp:391); FALCONS_TUNE_THREADS overrides the default of 8.
 int tune_threads();
