@@ -17,6 +17,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 
 #include "fz_common.h"
 #include "fz_fse.h"
@@ -464,23 +465,30 @@ __device__ uint64_t frame_block_wg(const uint8_t* src, uint32_t len, uint8_t* ou
     return sz;
 }
 
-// One workgroup per 64 K block of a class-byte array; framed blocks land in fixed slots.
-__global__ void __launch_bounds__(256) k_fse_blocks(const uint8_t* src, uint64_t n, uint8_t* slots, uint64_t* sizes) {
+// One FSE block of a batch: its input (class bytes, or a framed block to decode), its output and
+// its length in symbols (at most 64 K).
+struct FseBlk {
+    const uint8_t* src;
+    uint8_t* dst;
+    uint32_t len, pad;
+};
+
+// One workgroup per 64 K block (the blocks of every stream of a call in one launch); framed
+// blocks land in fixed slots.
+__global__ void __launch_bounds__(256) k_fse_blocks(const FseBlk* blks, uint8_t* slots, uint64_t* sizes) {
     __shared__ FseLds L;
     const uint64_t b = blockIdx.x;
-    const uint64_t off = b * kFblk;
-    const uint32_t len = (uint32_t)min<uint64_t>(kFblk, n - off);
-    const uint64_t sz = frame_block_wg(src + off, len, slots + b * (kFblk + 1024), L);
+    const uint64_t sz = frame_block_wg(blks[b].src, blks[b].len, slots + b * (kFblk + 1024), L);
     if (threadIdx.x == 0) sizes[b] = sz;
 }
 
-// Copies variable-size slots to their packed offsets (exclusive prefix of sizes, computed by the host).
-__global__ void k_pack_slots(const uint8_t* slots, uint64_t slot_stride, const uint64_t* sizes, const uint64_t* offs,
-                             uint8_t* out) {
+// Copies variable-size slots to their packed places (blks[b].dst, found by the host from the sizes).
+__global__ void k_pack_slots(const uint8_t* slots, uint64_t slot_stride, const uint64_t* sizes, const FseBlk* blks) {
     const uint64_t b = blockIdx.y;
     const uint64_t sz = sizes[b];
+    uint8_t* out = blks[b].dst;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sz; i += (uint64_t)gridDim.x * blockDim.x)
-        out[offs[b] + i] = slots[b * slot_stride + i];
+        out[i] = slots[b * slot_stride + i];
 }
 
 // FSE_decompress of one block (fse_decompress.c:254-282): the workgroup reads the header and builds
@@ -525,26 +533,26 @@ __device__ __forceinline__ void fse_decode_run(const DEntry* dt, const uint8_t* 
     if (i < len) dst[i] = dt[s0].symbol;
 }
 
-// Decodes framed class-byte blocks; block offsets were found by the host.
-__global__ void __launch_bounds__(256) k_fse_dec_blocks(const uint8_t* in, const uint64_t* offs, uint64_t n,
-                                                        uint8_t* out, int* err) {
+// Decodes framed class-byte blocks (the blocks of every stream of a call in one launch); the host
+// found where each block starts.
+__global__ void __launch_bounds__(256) k_fse_dec_blocks(const FseBlk* blks, int* err) {
     __shared__ FseDecLds L;
-    const uint64_t b = blockIdx.x;
-    const uint64_t o = offs[b];
-    const uint32_t len = (uint32_t)min<uint64_t>(kFblk, n - b * kFblk);
-    const uint8_t flag = in[o];
-    uint8_t* dst = out + b * kFblk;
+    const FseBlk B = blks[blockIdx.x];
+    const uint8_t* in = B.src;
+    const uint32_t len = B.len;
+    const uint8_t flag = in[0];
+    uint8_t* dst = B.dst;
     if (flag == 0) {
-        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = in[o + 1 + i];
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = in[1 + i];
         return;
     }
     if (flag == 2) {
-        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = in[o + 1];
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = in[1];
         return;
     }
-    const uint8_t* src = in + o + 5;
-    const uint32_t srcSize = (uint32_t)in[o + 1] | ((uint32_t)in[o + 2] << 8) | ((uint32_t)in[o + 3] << 16) |
-                             ((uint32_t)in[o + 4] << 24);
+    const uint8_t* src = in + 5;
+    const uint32_t srcSize = (uint32_t)in[1] | ((uint32_t)in[2] << 8) | ((uint32_t)in[3] << 16) |
+                             ((uint32_t)in[4] << 24);
     if (threadIdx.x == 0) {
         unsigned maxSym = 255, tl = 0;
         L.hl = read_ncount(L.norm, &maxSym, &tl, src, srcSize);
@@ -1585,6 +1593,13 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
                                        : dtd[((size_t)(r - kWalkCtxN) << kO1Log) + X];
     };
     asm volatile("" : "+v"(E));
+    // the next escaped value and the context it leads to, read before an escape needs them
+    auto esc_value = [&](uint64_t ei) -> int {
+        const uint64_t q = ei - ve0;
+        return ei < novf ? (int)vbuf[q < (uint64_t)kDenseOvf ? q : 0] : 32767;
+    };
+    int nval = esc_value(e);
+    uint32_t nctx = dense_ctx((uint32_t)o1_ctx(nval));
     uint64_t buf = 0;
     int32_t prel = 0;
     // bits [p - prel, p) of the window into buf (p >= 33 wherever this runs)
@@ -1614,12 +1629,13 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
             const uint32_t X = (((E >> 4) & ((1u << kO1Log) - 1)) + bits) & ((1u << kO1Log) - 1);
             uint32_t r = (Es & 15u) + (uint32_t)kWalkCtxN;
             if (r == 15u) {
-                const uint64_t ei = e - ve0;
-                if (e >= novf) { bad = true; code = 32767; }
-                else code = (int)vbuf[ei < (uint64_t)kDenseOvf ? ei : 0];
-                r = dense_ctx((uint32_t)o1_ctx(code));
+                if (e >= novf) bad = true;
+                code = nval;
+                r = nctx;
                 e++;
                 if (e >= elim) stop = true;
+                nval = esc_value(e);
+                nctx = dense_ctx((uint32_t)o1_ctx(nval));
             }
             nx = fetch(X, r);
         }
@@ -1843,27 +1859,55 @@ static void need_room(uint64_t want, uint64_t cap) {
     if (want > cap) throw std::length_error("The buffer for compressed data is not large enough.");
 }
 
-// fse_enc over a class-byte array (RleFse:150-176); returns the framed length written to out
-// (capacity cap).
-static uint64_t fse_enc_blocks(const uint8_t* src, uint64_t n, uint8_t* out, uint64_t cap, DevArena& ws,
-                               hipStream_t st) {
-    if (n == 0) return 0;
-    const uint64_t nb = (n + kFblk - 1) / kFblk;
+// fse_enc over class-byte arrays (RleFse:150-176), every block of every array in one launch and
+// one host round trip for the sizes. Item i writes its framed blocks at out (capacity cap), or,
+// with follow set, right after item i - 1's output (in what is left of its capacity); size
+// receives the framed length.
+struct FseEncItem {
+    const uint8_t* src;
+    uint64_t n;
+    uint8_t* out;
+    uint64_t cap;
+    bool follow;
+    uint64_t size;
+};
+static void fse_enc_multi(std::vector<FseEncItem>& items, DevArena& ws, hipStream_t st) {
+    std::vector<FseBlk> blks;
+    for (FseEncItem& it : items) {
+        it.size = 0;
+        for (uint64_t o = 0; o < it.n; o += kFblk)
+            blks.push_back(FseBlk{it.src + o, nullptr, (uint32_t)std::min<uint64_t>(kFblk, it.n - o), 0});
+    }
+    const uint64_t nb = blks.size();
+    if (nb == 0) return;
     uint8_t* slots = ws.take<uint8_t>(nb * (kFblk + 1024));
     uint64_t* sizes = ws.take<uint64_t>(nb);
-    uint64_t* offs = ws.take<uint64_t>(nb);
-    FZ_LAUNCH("fse_blocks", n * 2, k_fse_blocks, dim3((unsigned)nb), dim3(256), 0, st, src, n, slots, sizes);
+    FseBlk* dblks = ws.take<FseBlk>(nb);
+    h2d(dblks, blks.data(), nb, st);
+    FZ_LAUNCH("fse_blocks", 0, k_fse_blocks, dim3((unsigned)nb), dim3(256), 0, st, dblks, slots, sizes);
     FZ_HIP(hipGetLastError());
-    std::vector<uint64_t> hs(nb), ho(nb);
+    std::vector<uint64_t> hs(nb);
     d2h(hs.data(), sizes, nb, st);
-    uint64_t acc = 0;
-    for (uint64_t b = 0; b < nb; b++) { ho[b] = acc; acc += hs[b]; }
-    need_room(acc, cap);
-    h2d(offs, ho.data(), nb, st);
-    FZ_LAUNCH("fse_pack", 0, k_pack_slots, dim3(64, (unsigned)nb), dim3(256), 0, st, slots, (uint64_t)(kFblk + 1024), sizes,
-                       offs, out);
+    uint64_t b = 0;
+    for (size_t i = 0; i < items.size(); i++) {
+        FseEncItem& it = items[i];
+        if (it.follow && i > 0) {
+            it.out = items[i - 1].out + items[i - 1].size;
+            it.cap = items[i - 1].cap - items[i - 1].size;
+        }
+        uint64_t acc = 0;
+        for (uint64_t o = 0; o < it.n; o += kFblk, b++) {
+            blks[b].dst = it.out + acc;
+            acc += hs[b];
+        }
+        need_room(acc, it.cap);
+        it.size = acc;
+    }
+    h2d(dblks, blks.data(), nb, st);
+    FZ_LAUNCH("fse_pack", 0, k_pack_slots, dim3(64, (unsigned)nb), dim3(256), 0, st, slots, (uint64_t)(kFblk + 1024),
+              sizes, dblks);
     FZ_HIP(hipGetLastError());
-    return acc;
+    sync(st);  // blks is read by the copy above
 }
 
 static void put64(uint8_t*& p, uint64_t v) { memcpy(p, &v, 8); p += 8; }
@@ -1907,8 +1951,9 @@ static size_t encode_method0(const int16_t* bins, uint64_t n, uint8_t* out, uint
     FZ_HIP(hipGetLastError());
     // class streams go after the 48-byte head
     uint8_t* body = out + 48;
-    const uint64_t s_rcl = fse_enc_blocks(rcl, nr, body, cap - 48, ws, st);
-    const uint64_t s_vcl = fse_enc_blocks(vcl, nv, body + s_rcl, cap - 48 - s_rcl, ws, st);
+    std::vector<FseEncItem> fe{{rcl, nr, body, cap - 48, false, 0}, {vcl, nv, nullptr, 0, true, 0}};
+    fse_enc_multi(fe, ws, st);
+    const uint64_t s_rcl = fe[0].size, s_vcl = fe[1].size;
     need_room(48 + s_rcl + s_vcl + rbp + vbp, cap);
     FZ_HIP(hipMemcpyAsync(body + s_rcl + s_vcl, rman, rbp, hipMemcpyDeviceToDevice, st));
     FZ_HIP(hipMemcpyAsync(body + s_rcl + s_vcl + rbp, vman, vbp, hipMemcpyDeviceToDevice, st));
@@ -1967,20 +2012,35 @@ static void escape_streams(const int16_t* ovf, const std::vector<uint64_t>& off,
         for (uint64_t g = 0; g < S; g++)
             if (off[g] >= n) hb[g] = hb[S];
     }
+    // every segment's class bytes through FSE in one batch
+    std::vector<FseEncItem> fe;
+    std::vector<int> item_of(S, -1);
+    std::vector<uint64_t> vbps(S);
     for (uint64_t g = 0; g < S; g++) {
         const uint64_t lo = off[g], hi = off[g + 1], m = hi - lo;
         const uint64_t raw = 2 * m;
         // framed class bytes go right after the 17-byte head; the block is kept whole in the arena
         const uint64_t fse_cap = m + 5 * ((m + kFblk - 1) / kFblk + 1) + 64;
-        const uint64_t vbp = (hb[g + 1] - hb[g] + 7) / 8;
-        uint8_t* b = ws.take<uint8_t>(std::max<uint64_t>(1 + raw, 17 + fse_cap + vbp) + 8);
+        vbps[g] = (hb[g + 1] - hb[g] + 7) / 8;
+        uint8_t* b = ws.take<uint8_t>(std::max<uint64_t>(1 + raw, 17 + fse_cap + vbps[g]) + 8);
         block[g] = b;
+        if (m) {
+            item_of[g] = (int)fe.size();
+            fe.push_back(FseEncItem{vcl + lo, m, b + 17, fse_cap, false, 0});
+        }
+    }
+    fse_enc_multi(fe, ws, st);
+    std::vector<std::array<uint8_t, 17>> heads(S);
+    for (uint64_t g = 0; g < S; g++) {
+        const uint64_t lo = off[g], hi = off[g + 1], m = hi - lo;
+        const uint64_t raw = 2 * m, vbp = vbps[g];
+        uint8_t* b = block[g];
         if (m == 0) {
             FZ_HIP(hipMemcpyAsync(b, &kZeroByte, 1, hipMemcpyHostToDevice, st));
             len[g] = 1;
             continue;
         }
-        const uint64_t svcl = fse_enc_blocks(vcl + lo, m, b + 17, fse_cap, ws, st);
+        const uint64_t svcl = fe[item_of[g]].size;
         if (svcl + vbp + 16 >= raw) {
             FZ_HIP(hipMemcpyAsync(b, &kZeroByte, 1, hipMemcpyHostToDevice, st));
             FZ_HIP(hipMemcpyAsync(b + 1, ovf + lo, raw, hipMemcpyDeviceToDevice, st));
@@ -1992,14 +2052,27 @@ static void escape_streams(const int16_t* ovf, const std::vector<uint64_t>& off,
         FZ_LAUNCH("o1_escape_mantissas", m * 2 + vbp, k_ovf_mantissas, dim3(nblk(m)), dim3(256), 0, st, ovf, lo, hi,
                   vcl, voff, man);
         FZ_HIP(hipGetLastError());
-        uint8_t head[17];
+        uint8_t* head = heads[g].data();
         head[0] = 1;
         memcpy(head + 1, &svcl, 8);
         memcpy(head + 9, &vbp, 8);
         FZ_HIP(hipMemcpyAsync(b, head, 17, hipMemcpyHostToDevice, st));
         FZ_HIP(hipMemcpyAsync(b + 17 + svcl, man, vbp, hipMemcpyDeviceToDevice, st));
-        sync(st);  // head lives on this stack frame
         len[g] = 17 + svcl + vbp;
+    }
+    sync(st);  // heads live on this stack frame
+}
+
+// below[q] = the number of sorted positions pos[0..n) under ends[q] (one lane per end).
+__global__ void k_count_below(const uint64_t* pos, uint64_t n, const uint64_t* ends, int nq, uint64_t* below) {
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+        const uint64_t v = ends[q];
+        uint64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (pos[mid] < v) lo = mid + 1; else hi = mid;
+        }
+        below[q] = lo;
     }
 }
 
@@ -2018,8 +2091,6 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
     }
     uint64_t nesc = 0;
     d2h(&nesc, nsel, 1, st);
-    std::vector<uint64_t> hpos(nesc);
-    if (nesc) d2h(hpos.data(), epos, nesc, st);
     int16_t* ovf = ws.take<int16_t>(nesc + 1);
     if (nesc) FZ_LAUNCH("o1_gather_escapes", nesc * 10, k_gather_i16, dim3(nblk(nesc)), dim3(256), 0, st, bins, epos, nesc, ovf);
     // histograms and tables per segment
@@ -2029,7 +2100,7 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
     FZ_HIP(hipMemsetAsync(cnt, 0, S * kO1Ctx * kO1Sym * 4, st));
     std::vector<uint64_t> chains;  // {seg lo, chunk lo, chunk hi, segment}
     std::vector<uint64_t> novf(S * kO1K), ovf_off(S + 1);
-    std::vector<uint64_t> seg_n(S);
+    std::vector<uint64_t> seg_n(S), ends;
     uint64_t e = 0;
     for (uint64_t g = 0; g < S; g++) {
         const uint64_t lo = g * seg, hi = std::min(lo + seg, n), sn = hi - lo;
@@ -2047,13 +2118,31 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
             FZ_LAUNCH("o1_hist", 0, k_add_one, dim3(nblk(kO1Ctx * kO1Sym)), dim3(256), 0, st, c, kO1Ctx * kO1Sym);
         }
         FZ_HIP(hipGetLastError());
-        ovf_off[g] = e;
         for (int k = 0; k < kO1K; k++) {
             const uint64_t clo = lo + k * cs, chi = std::min(clo + cs, hi);
             chains.insert(chains.end(), {lo, clo, std::max(chi, clo), g});
-            uint64_t cntk = 0;
-            while (e < nesc && hpos[e] < chi) { e++; cntk++; }
-            novf[g * kO1K + k] = (chi > clo) ? cntk : 0;
+            ends.push_back(chi);  // below clo for the empty chunks at the end of a short segment
+        }
+    }
+    {
+        // escapes below each chain's end: binary searches in the sorted escape positions
+        const int nb = (int)(S * kO1K);
+        uint64_t* dends = ws.take<uint64_t>(nb);
+        uint64_t* dbelow = ws.take<uint64_t>(nb);
+        h2d(dends, ends.data(), nb, st);
+        FZ_LAUNCH("o1_escape_counts", 0, k_count_below, dim3(1), dim3(64), 0, st, epos, nesc, dends, nb, dbelow);
+        FZ_HIP(hipGetLastError());
+        std::vector<uint64_t> below(nb);
+        d2h(below.data(), dbelow, nb, st);
+        for (uint64_t g = 0; g < S; g++) {
+            ovf_off[g] = e;
+            for (int k = 0; k < kO1K; k++) {
+                const int q = (int)(g * kO1K + k);
+                const uint64_t clo = chains[4 * q + 1], chi = ends[q];
+                const uint64_t b = std::max(below[q], e);
+                novf[q] = (chi > clo) ? b - e : 0;
+                e = b;
+            }
         }
     }
     ovf_off[S] = e;
@@ -2240,16 +2329,30 @@ static uint64_t frame_offsets(const uint8_t* h, uint64_t n, std::vector<uint64_t
     return rp;
 }
 
-static void fse_dec_blocks(const uint8_t* hsrc, const uint8_t* dsrc, uint64_t n, uint8_t* out, int* err, DevArena& ws,
-                           hipStream_t st, uint64_t* consumed) {
+// Decodes framed class-byte streams (host copy hsrc for the block headers, device copy dsrc) of n
+// bytes each into out, every block of every stream in one launch.
+struct FseDecItem {
+    const uint8_t* hsrc;
+    const uint8_t* dsrc;
+    uint64_t n;
+    uint8_t* out;
+};
+static void fse_dec_multi(const std::vector<FseDecItem>& items, int* err, DevArena& ws, hipStream_t st) {
+    std::vector<FseBlk> blks;
     std::vector<uint64_t> offs;
-    const uint64_t used = frame_offsets(hsrc, n, offs);
-    if (consumed) *consumed = used;
-    if (n == 0) return;
-    uint64_t* doffs = ws.take<uint64_t>(offs.size());
-    h2d(doffs, offs.data(), offs.size(), st);
-    FZ_LAUNCH("fse_dec_blocks", n, k_fse_dec_blocks, dim3((unsigned)offs.size()), dim3(256), 0, st, dsrc, doffs, n, out, err);
+    for (const FseDecItem& it : items) {
+        if (it.n == 0) continue;
+        frame_offsets(it.hsrc, it.n, offs);
+        for (size_t k = 0; k < offs.size(); k++)
+            blks.push_back(FseBlk{it.dsrc + offs[k], it.out + k * kFblk,
+                                  (uint32_t)std::min<uint64_t>(kFblk, it.n - k * kFblk), 0});
+    }
+    if (blks.empty()) return;
+    FseBlk* dblks = ws.take<FseBlk>(blks.size());
+    h2d(dblks, blks.data(), blks.size(), st);
+    FZ_LAUNCH("fse_dec_blocks", 0, k_fse_dec_blocks, dim3((unsigned)blks.size()), dim3(256), 0, st, dblks, err);
     FZ_HIP(hipGetLastError());
+    sync(st);  // blks is read by the copy above
 }
 
 // Order-1 tANS chains (7 values each: stream offset, bytes, out lo, out hi, escape offset, table
@@ -2470,14 +2573,23 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
             total_ovf += ovfo;
         }
         int16_t* ovf = ws.take<int16_t>(total_ovf + 1);
+        // the class bytes of every segment in one batch
+        std::vector<uint8_t*> vcls(S, nullptr);
+        std::vector<FseDecItem> fd;
+        for (uint64_t g = 0; g < S; g++) {
+            const OvfPiece& pc = pieces[g];
+            if (!pc.n || pc.mode == 0) continue;
+            vcls[g] = ws.take<uint8_t>(pc.n);
+            fd.push_back(FseDecItem{host + pc.vcl_off, dev + pc.vcl_off, pc.n, vcls[g]});
+        }
+        fse_dec_multi(fd, err, ws, st);
         for (uint64_t g = 0; g < S; g++) {
             const OvfPiece& pc = pieces[g];
             if (!pc.n) continue;
             if (pc.mode == 0) {
                 FZ_HIP(hipMemcpyAsync(ovf + ovf_base[g], dev + pc.nraw_off, pc.n * 2, hipMemcpyDeviceToDevice, st));
             } else {
-                uint8_t* vcl = ws.take<uint8_t>(pc.n);
-                fse_dec_blocks(host + pc.vcl_off, dev + pc.vcl_off, pc.n, vcl, err, ws, st, nullptr);
+                uint8_t* vcl = vcls[g];
                 uint64_t* voff = ws.take<uint64_t>(pc.n + 1);
                 {
                     hipcub::TransformInputIterator<uint64_t, ToU64, const uint8_t*> it(vcl, ToU64{});
@@ -2510,8 +2622,7 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
         (void)vbp;
         uint8_t* rcl = ws.take<uint8_t>(nr + 1);
         uint8_t* vcl = ws.take<uint8_t>(nv + 1);
-        fse_dec_blocks(host + o_rcl, dev + o_rcl, nr, rcl, err, ws, st, nullptr);
-        if (nv) fse_dec_blocks(host + o_vcl, dev + o_vcl, nv, vcl, err, ws, st, nullptr);
+        fse_dec_multi({{host + o_rcl, dev + o_rcl, nr, rcl}, {host + o_vcl, dev + o_vcl, nv, vcl}}, err, ws, st);
         // bit offsets of the mantissas
         uint64_t* rk = ws.take<uint64_t>(nr + 1);
         uint64_t* vk = ws.take<uint64_t>(nv + 1);
