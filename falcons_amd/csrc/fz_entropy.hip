@@ -1105,6 +1105,7 @@ constexpr int kWinPad = 16;
 constexpr int kDenseStage = 12288;  // dense chains: bitstream bytes per window
 constexpr int kDenseOut = 4096;     // dense chains: codes per round
 constexpr int kDenseOvf = 2048;     // dense chains: escaped values per round
+constexpr int kDnOvfLast = kDenseOvf - 1;
 
 // LDS pointers carry their address space so that every access is an LDS instruction.
 #define FZ_LDS __attribute__((address_space(3)))
@@ -1536,11 +1537,12 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_walk(const uint8_t* in, const
 // Decoding-table entries of the dense-chain decoder. Contexts are renumbered so that the four
 // kept in LDS (kWalkCtx0 ..) come first: 0..3 are those, 4..14 are contexts 0..10, 15 follows an
 // escape. The LDS tables of the four interleave by state: the entry of (state X, context r < 4)
-// sits at byte X * 16 + r * 4. Entry: [31] set when the next context is not in LDS (or the symbol
-// is an escape), [30:22] the decoded code (signed), [21:18] the bit count, [16:0] with [31] clear:
-// the byte offset of the next entry less the read bits times 16 (next state base * 16 + next
-// context * 4); with [31] set: next state base * 16 + (renumbered next context - 4).
-constexpr uint32_t kDnSlow = 1u << 31;
+// sits at byte X * 16 + r * 4. Entry: [31] set when the next context is not in LDS, [30:22] the
+// decoded code (signed), [21:18] the bit count, [17] set for an escape, [16:0]: the byte offset of
+// the next entry less the read bits times 16 (next state base * 16 + next context * 4), where an
+// escape leaves the context term 0 (the escaped value's sign picks context 13 or 14, renumbered 2
+// or 3); with [31] set: next state base * 16 + (renumbered next context - 4).
+constexpr uint32_t kDnSlow = 1u << 31, kDnEsc = 1u << 17;
 __device__ __forceinline__ uint32_t dense_ctx(uint32_t c) {
     return c == 15u ? 15u : (c >= (uint32_t)kWalkCtx0 ? c - (uint32_t)kWalkCtx0 : c + (uint32_t)kWalkCtxN);
 }
@@ -1549,7 +1551,8 @@ __device__ __forceinline__ uint32_t dense_entry(uint32_t raw) {
     const uint32_t code = (sy >> 1) ^ (~(sy & 1u) + 1u);
     const uint32_t ns = raw & ((1u << kO1Log) - 1);
     const uint32_t r = dense_ctx(raw >> 28);
-    const uint32_t low = r < (uint32_t)kWalkCtxN ? (ns << 4) | (r << 2) : kDnSlow | (ns << 4) | (r - kWalkCtxN);
+    const uint32_t low = r == 15u ? kDnEsc | (ns << 4)
+                                  : r < (uint32_t)kWalkCtxN ? (ns << 4) | (r << 2) : kDnSlow | (ns << 4) | (r - kWalkCtxN);
     return low | ((code & 0x1FFu) << 22) | (((raw >> 24) & 15u) << 18);
 }
 
@@ -1572,6 +1575,7 @@ __device__ __forceinline__ uint32_t lds_u32(const FZ_LDS uint8_t* base, uint32_t
 // operations unless the next context's table is in device memory or the symbol is an escape.
 // The last steps before a window, output or escape limit run one by one with every check.
 // E is the entry of the current state on entry and on return.
+template <bool kEscInline>
 __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const FZ_LDS uint8_t* win, int64_t wlo,
                                                  int64_t until,
                                                  uint64_t max_n, const FZ_LDS uint8_t* hotb, const uint32_t* dtd,
@@ -1593,13 +1597,16 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
                                        : dtd[((size_t)(r - kWalkCtxN) << kO1Log) + X];
     };
     asm volatile("" : "+v"(E));
-    // the next escaped value and the context it leads to, read before an escape needs them
+    // The next escaped value and the byte term of the context it leads to (13 or 14 by its sign,
+    // renumbered 2 or 3, times 4), read before an escape needs them. Past the side stream's end
+    // the value is 32767 (the escape is then an error).
     auto esc_value = [&](uint64_t ei) -> int {
         const uint64_t q = ei - ve0;
         return ei < novf ? (int)vbuf[q < (uint64_t)kDenseOvf ? q : 0] : 32767;
     };
+    auto ctx_term = [](int v) -> uint32_t { return 8u + (((uint32_t)v >> 29) & 4u); };
     int nval = esc_value(e);
-    uint32_t nctx = dense_ctx((uint32_t)o1_ctx(nval));
+    uint32_t nterm = ctx_term(nval);
     uint64_t buf = 0;
     int32_t prel = 0;
     // bits [p - prel, p) of the window into buf (p >= 33 wherever this runs)
@@ -1614,8 +1621,9 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
     // One step. The table entry, the bit buffer and the read position stay in vector registers (the
     // values are the same in every lane): scalar arithmetic on them would put a lane-to-scalar move
     // and its latency on the critical path of every step. Only the rare-path test reads E into a
-    // scalar register. stop is set after the escape that reaches elim (in a group of four, which
-    // starts with e + 4 <= elim, only by its last step).
+    // scalar register; escapes and entries whose next table is in device memory take that path.
+    // stop is set after the escape that reaches elim (in a group of four, which starts with
+    // e + 4 <= elim, only by its last step).
     auto step = [&]() -> uint32_t {
         const uint32_t nb = (E >> 18) & 15u;
         prel -= (int32_t)nb;
@@ -1625,19 +1633,44 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
         // for every entry)
         uint32_t nx = lds_u32(hotb, (E & 0x1FFFFu) + (bits << 4));
         const uint32_t Es = __builtin_amdgcn_readfirstlane(E);
-        if (__builtin_expect((Es & kDnSlow) != 0, 0)) {
+        if (__builtin_expect((Es & (kDnSlow | kDnEsc)) != 0, 0)) {
             const uint32_t X = (((E >> 4) & ((1u << kO1Log) - 1)) + bits) & ((1u << kO1Log) - 1);
-            uint32_t r = (Es & 15u) + (uint32_t)kWalkCtxN;
-            if (r == 15u) {
+            if (Es & kDnEsc) {
+                // the next entry is read before the following escaped value
                 if (e >= novf) bad = true;
                 code = nval;
-                r = nctx;
+                nx = lds_u32(hotb, (X << 4) | nterm);
                 e++;
                 if (e >= elim) stop = true;
                 nval = esc_value(e);
-                nctx = dense_ctx((uint32_t)o1_ctx(nval));
+                nterm = ctx_term(nval);
+            } else {
+                nx = dtd[((size_t)(Es & 15u) << kO1Log) + X];
             }
-            nx = fetch(X, r);
+        }
+        E = nx;
+        asm volatile("" : "+v"(E));
+        return (uint32_t)code & 0xFFFFu;
+    };
+    // The step for escape-dense chains: an escape is handled without a branch. The escaped values
+    // come from vbuf by a running index kept in a vector register, read one step ahead; the index
+    // is folded back into e after every group.
+    uint32_t vi = (uint32_t)(e - ve0);  // index into vbuf of the next escaped value
+    asm volatile("" : "+v"(vi));
+    auto step_esc = [&]() -> uint32_t {
+        const uint32_t nb = (E >> 18) & 15u;
+        prel -= (int32_t)nb;
+        const uint32_t bits = (uint32_t)(buf >> ((uint32_t)prel & 63u)) & ((1u << nb) - 1u);
+        const bool esc = (E & kDnEsc) != 0;
+        const int code = esc ? nval : __builtin_amdgcn_sbfe((int)E, 22, 9);
+        uint32_t nx = lds_u32(hotb, (E & 0x1FFFFu) + (bits << 4) + (esc ? nterm : 0u));
+        vi += esc ? 1u : 0u;
+        nval = (int)vbuf[min(vi, (uint32_t)kDnOvfLast)];
+        nterm = ctx_term(nval);
+        const uint32_t Es = __builtin_amdgcn_readfirstlane(E);
+        if (__builtin_expect((Es & kDnSlow) != 0, 0)) {
+            const uint32_t X = (((E >> 4) & ((1u << kO1Log) - 1)) + bits) & ((1u << kO1Log) - 1);
+            nx = dtd[((size_t)(Es & 15u) << kO1Log) + X];
         }
         E = nx;
         asm volatile("" : "+v"(E));
@@ -1654,16 +1687,32 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
         for (; g; g--) {
             refill();
             const int32_t lo = p - prel;
-            const uint32_t c0 = step();
-            const uint32_t c1 = step();
-            const uint32_t c2 = step();
-            const uint32_t c3 = step();
+            uint32_t c0, c1, c2, c3;
+            if constexpr (kEscInline) {
+                c0 = step_esc();
+                c1 = step_esc();
+                c2 = step_esc();
+                c3 = step_esc();
+                e = ve0 + __builtin_amdgcn_readfirstlane(vi);
+                stop = e >= elim;
+            } else {
+                c0 = step();
+                c1 = step();
+                c2 = step();
+                c3 = step();
+            }
             *reinterpret_cast<FZ_LDS uint64_t*>(obuf + k) = (uint64_t)(c0 | (c1 << 16)) | ((uint64_t)(c2 | (c3 << 16)) << 32);
             k += 4;
             p = lo + prel;
             if (stop) break;
         }
         if (stop) break;
+    }
+    if (kEscInline) {
+        // the values consumed in groups past the side stream's end: an error, as in the step
+        if (e > novf) bad = true;
+        nval = esc_value(e);
+        nterm = ctx_term(nval);
     }
     while (!stop && p > lim && k < kmax) {
         refill();
@@ -1705,6 +1754,8 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, cons
     const int16_t* ov = ovf + c.ovf_base;
     const uint64_t m = c.hi - c.lo;
     SeqState z{c.pos0, c.x0, 0, 0, false};
+    // more than one escape in 16 symbols: the escape-dense step
+    const bool esc_dense = c.novf * 16 > m;
     // the chain starts in context 0, whose table is in device memory
     uint32_t E = dtd[((size_t)(dense_ctx(0) - kWalkCtxN) << kO1Log) + (c.x0 & ((1u << kO1Log) - 1))];
     uint64_t i = 0;
@@ -1717,9 +1768,13 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, cons
         for (uint64_t k = e0 + threadIdx.x; k < ve; k += blockDim.x) vbuf[k - e0] = ov[k];
         __syncthreads();
         const int64_t floor_bits = lo == 0 ? INT64_MIN : lo * 8 + 64;
-        const uint64_t nk = dense_decode(z, E, lds(stage), w0 + kWinPad, floor_bits, min<uint64_t>(m - i, kDenseOut),
-                                         lds(reinterpret_cast<const uint8_t*>(hot)), dtd, lds(vbuf), e0, ve, c.novf,
-                                         lds(obuf));
+        const uint64_t nk =
+            esc_dense ? dense_decode<true>(z, E, lds(stage), w0 + kWinPad, floor_bits, min<uint64_t>(m - i, kDenseOut),
+                                           lds(reinterpret_cast<const uint8_t*>(hot)), dtd, lds(vbuf), e0, ve, c.novf,
+                                           lds(obuf))
+                      : dense_decode<false>(z, E, lds(stage), w0 + kWinPad, floor_bits, min<uint64_t>(m - i, kDenseOut),
+                                            lds(reinterpret_cast<const uint8_t*>(hot)), dtd, lds(vbuf), e0, ve, c.novf,
+                                            lds(obuf));
         __syncthreads();
         for (uint64_t k = threadIdx.x; k < nk; k += blockDim.x) out[c.lo + i + k] = obuf[k];
         i += nk;
