@@ -1248,6 +1248,23 @@ __device__ inline bool o1d_same(const O1DecState& a, const O1DecState& b) {
 // Starts of round 1: the true start for subsequence 0; elsewhere the state a decoder reaches at the
 // subsequence boundary after warming up over the preceding `warm` bits from a guessed state (it has
 // almost always fallen into step by then; after an escape it follows the positive sign).
+// A decoder started in state X0 (context 0) at bit w and run down to the first symbol start at or
+// below b (after an escape it follows the positive sign). Symbols that read no bits do not move
+// the position (a context with one symbol), so the run is bounded; one that never gets there
+// returns position b.
+__device__ O1DecState o1d_warm(const uint8_t* cb, const DEntry* dt, int64_t w, int64_t b, uint32_t X0) {
+    BitWin rd = bwin_at(cb, w);
+    uint32_t X = X0 & ((1u << kO1Log) - 1), cx = 0;
+    uint64_t guard = (uint64_t)(w - b) * 4 + 4096;
+    while (rd.pos > b && guard--) {
+        const DEntry d = dt[(size_t)cx * (1u << kO1Log) + (X & ((1u << kO1Log) - 1))];
+        X = d.newState + bwin_read(rd, d.nbBits & 15u);
+        const unsigned nc = d.nbBits >> 4;
+        cx = nc == 15 ? 13u : nc;
+    }
+    return O1DecState{rd.pos > b ? b : rd.pos, X & ((1u << kO1Log) - 1), cx};
+}
+
 __global__ void __launch_bounds__(256) k_o1d_init(const uint8_t* in, const O1DecChain* ch, int nch, uint64_t nsub,
                                                   const DEntry* dt_all, int64_t warm_subs, O1StartSet* start,
                                                   uint8_t* run) {
@@ -1264,18 +1281,8 @@ __global__ void __launch_bounds__(256) k_o1d_init(const uint8_t* in, const O1Dec
         const int64_t b = c.pos0 - (int64_t)r * c.sub_bits;
         const int64_t w = min<int64_t>(c.pos0, b + warm_subs * c.sub_bits);
         const DEntry* dt = dt_all + c.dset * kO1Ctx * (1u << kO1Log);
-        BitWin rd = bwin_at(in + c.off, w);
-        uint32_t X = (uint32_t)((r * 2654435761u) & ((1u << kO1Log) - 1)), cx = 0;
-        // symbols that read no bits do not move the position (a context with one symbol): bounded
-        uint64_t guard = (uint64_t)(w - b) * 4 + 4096;
-        while (rd.pos > b && guard--) {
-            const DEntry d = dt[(size_t)cx * (1u << kO1Log) + (X & ((1u << kO1Log) - 1))];
-            X = d.newState + bwin_read(rd, d.nbBits & 15u);
-            const unsigned nc = d.nbBits >> 4;
-            cx = nc == 15 ? 13u : nc;
-        }
         // a warm-up that never got there starts at the boundary (the walk then decodes it exactly)
-        ss.s[0] = O1DecState{rd.pos > b ? b : rd.pos, X & ((1u << kO1Log) - 1), cx};
+        ss.s[0] = o1d_warm(in + c.off, dt, w, b, (uint32_t)(r * 2654435761u));
     }
     start[t] = ss;
     run[t] = 1;
@@ -1543,26 +1550,35 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_walk(const uint8_t* in, const
 // escape leaves the context term 0 (the escaped value's sign picks context 13 or 14, renumbered 2
 // or 3); with [31] set: next state base * 16 + (renumbered next context - 4).
 constexpr uint32_t kDnSlow = 1u << 31, kDnEsc = 1u << 17;
-__device__ __forceinline__ uint32_t dense_ctx(uint32_t c) {
-    return c == 15u ? 15u : (c >= (uint32_t)kWalkCtx0 ? c - (uint32_t)kWalkCtx0 : c + (uint32_t)kWalkCtxN);
-}
-__device__ __forceinline__ uint32_t dense_entry(uint32_t raw) {
+// The renumbering of one dense chain: the four contexts most often in use on it (by the
+// stationary distribution of its segment's tables) come first and have their tables in LDS.
+// inv[context] is the renumbered context (inv[15] = 15 for an escape), perm[r] the context of r;
+// esc marks chains decoded with the escape-dense step (contexts 13 and 14 then among the first four).
+struct DenseMap {
+    uint8_t inv[16];
+    uint8_t perm[16];
+    uint32_t esc;
+};
+__device__ __forceinline__ uint32_t dense_entry(uint32_t raw, const DenseMap& dm) {
     const uint32_t sy = (raw >> 16) & 0xFFu;
     const uint32_t code = (sy >> 1) ^ (~(sy & 1u) + 1u);
     const uint32_t ns = raw & ((1u << kO1Log) - 1);
-    const uint32_t r = dense_ctx(raw >> 28);
+    const uint32_t r = dm.inv[raw >> 28];
     const uint32_t low = r == 15u ? kDnEsc | (ns << 4)
                                   : r < (uint32_t)kWalkCtxN ? (ns << 4) | (r << 2) : kDnSlow | (ns << 4) | (r - kWalkCtxN);
     return low | ((code & 0x1FFu) << 22) | (((raw >> 24) & 15u) << 18);
 }
 
-// Dense-format copies of the tables of contexts 0..10 for each dense chain: grid (chains, 11).
-__global__ void __launch_bounds__(256) k_o1d_dense_tables(const O1DecChain* ch, const DEntry* dt_all, uint32_t* dtd) {
+// Dense-format copies of the tables of renumbered contexts 4..14 for each dense chain: grid
+// (chains, 11).
+__global__ void __launch_bounds__(256) k_o1d_dense_tables(const O1DecChain* ch, const DEntry* dt_all,
+                                                          const DenseMap* maps, uint32_t* dtd) {
     const O1DecChain c = ch[blockIdx.x];
+    const DenseMap dm = maps[blockIdx.x];
     const uint32_t* dt = reinterpret_cast<const uint32_t*>(dt_all + c.dset * kO1Ctx * (1u << kO1Log)) +
-                         ((size_t)blockIdx.y << kO1Log);
+                         ((size_t)dm.perm[kWalkCtxN + blockIdx.y] << kO1Log);
     uint32_t* o = dtd + ((size_t)blockIdx.x * kWalkCtx0 + blockIdx.y) * (1u << kO1Log);
-    for (uint32_t i = threadIdx.x; i < (1u << kO1Log); i += blockDim.x) o[i] = dense_entry(dt[i]);
+    for (uint32_t i = threadIdx.x; i < (1u << kO1Log); i += blockDim.x) o[i] = dense_entry(dt[i], dm);
 }
 
 __device__ __forceinline__ uint32_t lds_u32(const FZ_LDS uint8_t* base, uint32_t off) {
@@ -1580,7 +1596,7 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
                                                  int64_t until,
                                                  uint64_t max_n, const FZ_LDS uint8_t* hotb, const uint32_t* dtd,
                                                  const FZ_LDS int16_t* vbuf, uint64_t ve0, uint64_t ve1,
-                                                 uint64_t novf, FZ_LDS int16_t* obuf) {
+                                                 uint64_t novf, FZ_LDS int16_t* obuf, uint32_t r13, uint32_t r14) {
     const FZ_LDS uint32_t* w32 = reinterpret_cast<const FZ_LDS uint32_t*>(win);
     const int64_t base = (wlo - kWinPad) * 8;
     int32_t p = (int32_t)(S.B - base);
@@ -1597,14 +1613,14 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
                                        : dtd[((size_t)(r - kWalkCtxN) << kO1Log) + X];
     };
     asm volatile("" : "+v"(E));
-    // The next escaped value and the byte term of the context it leads to (13 or 14 by its sign,
-    // renumbered 2 or 3, times 4), read before an escape needs them. Past the side stream's end
-    // the value is 32767 (the escape is then an error).
+    // The next escaped value and the renumbered context it leads to (r13 or r14 by its sign; an
+    // escaped code has magnitude 128 or more), read before an escape needs them. Past the side
+    // stream's end the value is 32767 (the escape is then an error).
     auto esc_value = [&](uint64_t ei) -> int {
         const uint64_t q = ei - ve0;
         return ei < novf ? (int)vbuf[q < (uint64_t)kDenseOvf ? q : 0] : 32767;
     };
-    auto ctx_term = [](int v) -> uint32_t { return 8u + (((uint32_t)v >> 29) & 4u); };
+    auto ctx_term = [&](int v) -> uint32_t { return v < 0 ? r14 : r13; };
     int nval = esc_value(e);
     uint32_t nterm = ctx_term(nval);
     uint64_t buf = 0;
@@ -1639,7 +1655,7 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
                 // the next entry is read before the following escaped value
                 if (e >= novf) bad = true;
                 code = nval;
-                nx = lds_u32(hotb, (X << 4) | nterm);
+                nx = fetch(X, nterm);
                 e++;
                 if (e >= elim) stop = true;
                 nval = esc_value(e);
@@ -1652,9 +1668,9 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
         asm volatile("" : "+v"(E));
         return (uint32_t)code & 0xFFFFu;
     };
-    // The step for escape-dense chains: an escape is handled without a branch. The escaped values
-    // come from vbuf by a running index kept in a vector register, read one step ahead; the index
-    // is folded back into e after every group.
+    // The step for escape-dense chains (r13 and r14 then in LDS): an escape is handled without a
+    // branch. The escaped values come from vbuf by a running index kept in a vector register, read
+    // one step ahead; the index is folded back into e after every group.
     uint32_t vi = (uint32_t)(e - ve0);  // index into vbuf of the next escaped value
     asm volatile("" : "+v"(vi));
     auto step_esc = [&]() -> uint32_t {
@@ -1663,7 +1679,7 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
         const uint32_t bits = (uint32_t)(buf >> ((uint32_t)prel & 63u)) & ((1u << nb) - 1u);
         const bool esc = (E & kDnEsc) != 0;
         const int code = esc ? nval : __builtin_amdgcn_sbfe((int)E, 22, 9);
-        uint32_t nx = lds_u32(hotb, (E & 0x1FFFFu) + (bits << 4) + (esc ? nterm : 0u));
+        uint32_t nx = lds_u32(hotb, (E & 0x1FFFFu) + (bits << 4) + (esc ? nterm << 2 : 0u));
         vi += esc ? 1u : 0u;
         nval = (int)vbuf[min(vi, (uint32_t)kDnOvfLast)];
         nterm = ctx_term(nval);
@@ -1737,8 +1753,8 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
 // between rounds.
 
 __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, const O1DecChain* ch, const DEntry* dt_all,
-                                                         const uint32_t* dtd_all, const int16_t* ovf, int16_t* out,
-                                                         int* err) {
+                                                         const DenseMap* maps, const uint32_t* dtd_all,
+                                                         const int16_t* ovf, int16_t* out, int* err) {
     __shared__ uint32_t hot[kWalkCtxN << kO1Log];
     __shared__ __align__(16) uint32_t stage32[(kDenseStage + kWinPad + 32) / 4 + 4];
     uint8_t* stage = reinterpret_cast<uint8_t*>(stage32);
@@ -1747,17 +1763,20 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, cons
     const O1DecChain c = ch[blockIdx.x];
     const uint32_t* dt = reinterpret_cast<const uint32_t*>(dt_all + c.dset * kO1Ctx * (1u << kO1Log));
     const uint32_t* dtd = dtd_all + (size_t)blockIdx.x * kWalkCtx0 * (1u << kO1Log);
-    // interleaved by state: hot[X * 4 + r] holds (state X, context kWalkCtx0 + r)
+    const DenseMap dm = maps[blockIdx.x];
+    // interleaved by state: hot[X * 4 + r] holds (state X, renumbered context r)
     for (uint32_t i = threadIdx.x; i < (kWalkCtxN << kO1Log); i += blockDim.x)
-        hot[i] = dense_entry(dt[((size_t)(kWalkCtx0 + (i & 3u)) << kO1Log) + (i >> 2)]);
+        hot[i] = dense_entry(dt[((size_t)dm.perm[i & 3u] << kO1Log) + (i >> 2)], dm);
+    __syncthreads();
     const uint8_t* cb = in + c.off;
     const int16_t* ov = ovf + c.ovf_base;
     const uint64_t m = c.hi - c.lo;
     SeqState z{c.pos0, c.x0, 0, 0, false};
-    // more than one escape in 16 symbols: the escape-dense step
-    const bool esc_dense = c.novf * 16 > m;
-    // the chain starts in context 0, whose table is in device memory
-    uint32_t E = dtd[((size_t)(dense_ctx(0) - kWalkCtxN) << kO1Log) + (c.x0 & ((1u << kO1Log) - 1))];
+    const bool esc_dense = dm.esc != 0;
+    const uint32_t r13 = dm.inv[13], r14 = dm.inv[14];
+    // the chain starts in context 0
+    const uint32_t r0 = dm.inv[0], x0 = c.x0 & ((1u << kO1Log) - 1);
+    uint32_t E = r0 < (uint32_t)kWalkCtxN ? hot[(x0 << 2) | r0] : dtd[((size_t)(r0 - kWalkCtxN) << kO1Log) + x0];
     uint64_t i = 0;
     while (i < m && !z.bad) {
         const int64_t hi = min<int64_t>((int64_t)c.len, (z.B + 7) / 8 + 1);
@@ -1771,10 +1790,10 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, cons
         const uint64_t nk =
             esc_dense ? dense_decode<true>(z, E, lds(stage), w0 + kWinPad, floor_bits, min<uint64_t>(m - i, kDenseOut),
                                            lds(reinterpret_cast<const uint8_t*>(hot)), dtd, lds(vbuf), e0, ve, c.novf,
-                                           lds(obuf))
+                                           lds(obuf), r13, r14)
                       : dense_decode<false>(z, E, lds(stage), w0 + kWinPad, floor_bits, min<uint64_t>(m - i, kDenseOut),
                                             lds(reinterpret_cast<const uint8_t*>(hot)), dtd, lds(vbuf), e0, ve, c.novf,
-                                            lds(obuf));
+                                            lds(obuf), r13, r14);
         __syncthreads();
         for (uint64_t k = threadIdx.x; k < nk; k += blockDim.x) out[c.lo + i + k] = obuf[k];
         i += nk;
@@ -2423,10 +2442,61 @@ constexpr double kSubSymbols = 1024;
 constexpr uint64_t kMaxRounds = 6;
 // Subsequences of warm-up before the first round's start states.
 constexpr int64_t kWarmSubs = 4;
+// Chains below this many bits a symbol are decoded from their start.
+constexpr double kSeqBitsPerSymbol = 2.5;
+
+// The renumbering of the contexts of a dense chain in segment g: contexts by their share under the
+// stationary distribution of the segment's context chain (a code's context follows from the
+// previous code; an escape leads to 13 or 14, counted half each), the four largest first; with
+// esc set, 13 and 14 are among the four.
+static DenseMap dense_map(const uint8_t* host, uint64_t nf_off, bool esc) {
+    double T[kO1Ctx][kO1Ctx] = {};
+    for (int a = 0; a < kO1Ctx; a++) {
+        double tot = 0;
+        for (int sy = 0; sy < kO1Sym; sy++) {
+            int16_t v;
+            memcpy(&v, host + nf_off + 2 * ((size_t)a * kO1Sym + sy), 2);
+            const double w = v == -1 ? 1.0 : (v > 0 ? (double)v : 0.0);
+            if (w == 0) continue;
+            tot += w;
+            if (sy == 255) {
+                T[a][13] += w / 2;
+                T[a][14] += w / 2;
+            } else {
+                T[a][o1_ctx((int)((sy >> 1) ^ (~(sy & 1) + 1)))] += w;
+            }
+        }
+        for (int b = 0; b < kO1Ctx; b++) T[a][b] = tot > 0 ? T[a][b] / tot : 0.0;
+    }
+    double pi[kO1Ctx];
+    for (int a = 0; a < kO1Ctx; a++) pi[a] = 1.0 / kO1Ctx;
+    for (int it = 0; it < 64; it++) {
+        double nx[kO1Ctx] = {};
+        for (int a = 0; a < kO1Ctx; a++)
+            for (int b = 0; b < kO1Ctx; b++) nx[b] += pi[a] * T[a][b];
+        for (int b = 0; b < kO1Ctx; b++) pi[b] = nx[b];
+    }
+    int order[kO1Ctx];
+    for (int a = 0; a < kO1Ctx; a++) order[a] = a;
+    std::stable_sort(order, order + kO1Ctx, [&](int x, int y) {
+        const bool fx = esc && (x == 13 || x == 14), fy = esc && (y == 13 || y == 14);
+        if (fx != fy) return fx;
+        return pi[x] > pi[y];
+    });
+    DenseMap dm{};
+    for (int r = 0; r < kO1Ctx; r++) {
+        dm.perm[r] = (uint8_t)order[r];
+        dm.inv[order[r]] = (uint8_t)r;
+    }
+    dm.inv[15] = 15;
+    dm.perm[15] = 15;
+    dm.esc = esc ? 1u : 0u;
+    return dm;
+}
 
 static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const std::vector<uint64_t>& chains,
-                               const DEntry* dt, const int16_t* ovf, int16_t* out, int* err, DevArena& ws,
-                               hipStream_t st, const AuxStream& aux) {
+                               const std::vector<uint64_t>& nf_off, const DEntry* dt, const int16_t* ovf, int16_t* out,
+                               int* err, DevArena& ws, hipStream_t st, const AuxStream& aux) {
     const int nch = (int)(chains.size() / 7);
     std::vector<O1DecChain> hc, hd;
     uint64_t nsub = 0, stream_bytes = 0, ncodes = 0, dense_bytes = 0, dense_codes = 0;
@@ -2463,29 +2533,58 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
             dense_codes += m;
             continue;
         }
-        // about 4096 symbols per subsequence, enough for a guessed start to fall into step
+        // about 1024 symbols per subsequence
         const double bits_per_sym = std::max(1.0, (double)std::max<int64_t>(pos0, 1) / (double)m);
         const int64_t sub = std::max<int64_t>(2048, (int64_t)(bits_per_sym * kSubSymbols));
         const uint64_t ns = std::max<int64_t>(1, (pos0 + sub - 1) / sub);
-        hc.push_back(O1DecChain{c[0], c[1], c[2], c[3], c[4], c[6], c[5], pos0, x0, (uint32_t)ns, nsub, sub});
-        stream_bytes += c[1];
-        ncodes += m;
-        nsub += ns;
+        hc.push_back(O1DecChain{c[0], c[1], c[2], c[3], c[4], c[6], c[5], pos0, x0, (uint32_t)ns, 0, sub});
+    }
+    // Chains of fewer than 2.5 bits a symbol go to the sequential decoder as well: there a decoder
+    // started from a guessed state seldom falls into step with the true one (few bits are read per
+    // symbol), so subsequence start sets do not settle and the walk would decode most subsequences
+    // one after another anyway, more slowly.
+    {
+        std::vector<O1DecChain> keep;
+        for (O1DecChain c : hc) {
+            const double bps = (double)c.pos0 / (double)(c.hi - c.lo);
+            if (c.nsub > 2 && bps < kSeqBitsPerSymbol) {
+                c.nsub = 1;
+                c.sub_bits = 0;
+                hd.push_back(c);
+                dense_bytes += c.len;
+                dense_codes += c.hi - c.lo;
+                continue;
+            }
+            c.tfirst = nsub;
+            keep.push_back(c);
+            stream_bytes += c.len;
+            ncodes += c.hi - c.lo;
+            nsub += c.nsub;
+        }
+        hc.swap(keep);
     }
     // dense chains run beside the rest, on the auxiliary stream when there is one
     hipStream_t dst = aux.stream ? aux.stream : st;
+    std::vector<DenseMap> maps;  // read by a copy in flight until the function returns
     if (!hd.empty()) {
         O1DecChain* ddc = ws.take<O1DecChain>(hd.size());
         h2d(ddc, hd.data(), hd.size(), st);
+        // more than one escape in 16 symbols: the escape-dense step
+        maps.resize(hd.size());
+        for (size_t q = 0; q < hd.size(); q++)
+            maps[q] = dense_map(host, nf_off[hd[q].dset], hd[q].novf * 16 > hd[q].hi - hd[q].lo);
+        DenseMap* dmaps = ws.take<DenseMap>(hd.size());
+        h2d(dmaps, maps.data(), maps.size(), st);
+        // the second stream starts after every copy above
         if (aux.stream) {
             FZ_HIP(hipEventRecord(aux.fork, st));
             FZ_HIP(hipStreamWaitEvent(aux.stream, aux.fork, 0));
         }
         uint32_t* dtd = ws.take<uint32_t>(hd.size() * kWalkCtx0 * (1u << kO1Log));
         FZ_LAUNCH("o1_decode_dense_tables", 0, k_o1d_dense_tables, dim3((unsigned)hd.size(), (unsigned)kWalkCtx0),
-                  dim3(256), 0, dst, ddc, dt, dtd);
+                  dim3(256), 0, dst, ddc, dt, dmaps, dtd);
         FZ_LAUNCH("o1_decode_dense", dense_bytes + dense_codes * 2, k_o1d_dense, dim3((unsigned)hd.size()), dim3(kSeqLanes), 0,
-                  dst, dev, ddc, dt, dtd, ovf, out, err);
+                  dst, dev, ddc, dt, dmaps, dtd, ovf, out, err);
         FZ_HIP(hipGetLastError());
     }
     struct Join {  // the caller's stream waits for the dense chains however this function exits
@@ -2668,7 +2767,7 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
             d2h(&herr, err, 1, st);
             if (herr) throw std::invalid_argument("corrupt o1 tANS tables");
         }
-        o1_decode_parallel(host, dev, chains, dt, ovf, out, err, ws, st, aux);
+        o1_decode_parallel(host, dev, chains, nf_off, dt, ovf, out, err, ws, st, aux);
     } else {
         const uint64_t nr = get64(hp), nv = get64(hp), s_rcl = get64(hp), s_vcl = get64(hp), rbp = get64(hp),
                        vbp = get64(hp);
