@@ -52,6 +52,11 @@ __host__ __device__ inline int o1_sym(int code) {
     return zz < 255 ? (int)zz : 255;
 }
 
+// LDS pointers carry their address space so that every access is an LDS instruction.
+#define FZ_LDS __attribute__((address_space(3)))
+template <typename T>
+__device__ __forceinline__ FZ_LDS T* lds(T* p) { return (FZ_LDS T*)p; }
+
 // ---------------------------------------------------------------- small device helpers
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b mod 8), and each XCD has
 // its own L2. Kernels whose neighbouring workgroups share tables (the tANS tables of one segment)
@@ -498,8 +503,8 @@ __global__ void k_pack_slots(const uint8_t* slots, uint64_t slot_stride, const u
 struct FseDecSmall {
     TableLdsT<kFblkLog> tab;
     DEntry dt[1u << kFblkLog];
-    uint8_t in[kFblk + 64];
-    uint8_t out[kFblk];
+    alignas(16) uint8_t in[kFblk + 64];
+    alignas(16) uint8_t out[kFblk];
 };
 struct FseDecBig {
     TableLdsT<kFseMaxTableLog> tab;
@@ -531,6 +536,61 @@ __device__ __forceinline__ void fse_decode_run(const DEntry* dt, const uint8_t* 
         s1 = e1.newState + bwin_read(r, e1.nbBits);
     }
     if (i < len) dst[i] = dt[s0].symbol;
+}
+
+// The same loop with the table, the bitstream and the output in LDS (in: the bitstream, bsz
+// bytes, 4-byte aligned with 12 readable bytes past its end; out: 4-byte aligned). The bits below
+// the read position sit in a 64-bit buffer refilled from aligned LDS words once per two symbol
+// pairs (two pairs read at most 4 * 11 bits); the state values stay in vector registers.
+__device__ __forceinline__ void fse_decode_run_lds(const FZ_LDS DEntry* dt, const FZ_LDS uint8_t* in, uint64_t bsz,
+                                                   unsigned tl, FZ_LDS uint8_t* out, uint32_t len) {
+    const FZ_LDS uint32_t* w32 = reinterpret_cast<const FZ_LDS uint32_t*>(in);
+    // entries as words: [15:0] next state base, [23:16] symbol, [31:24] bit count
+    const FZ_LDS uint32_t* dtw = reinterpret_cast<const FZ_LDS uint32_t*>(dt);
+    int32_t p = (int32_t)((bsz - 1) * 8) + (int32_t)highbit32(in[bsz - 1]);
+    uint64_t buf = 0;
+    int32_t prel = 0;
+    auto refill = [&]() {  // bits [p - prel, p); bytes below the start read as zero
+        const int32_t lo = p - 64;
+        const int32_t lc = max(lo, 0);
+        const int32_t j0 = lc >> 5;
+        const uint32_t sh = (uint32_t)lc & 31u;
+        uint64_t b = (uint64_t)__builtin_amdgcn_alignbit(w32[j0 + 1], w32[j0], sh) |
+                     ((uint64_t)__builtin_amdgcn_alignbit(w32[j0 + 2], w32[j0 + 1], sh) << 32);
+        if (lo < 0) b = (p > 0 ? b << (uint32_t)(-lo) : 0ull);  // move bit p-1 to the top, zeros below
+        buf = b;
+        prel = 64;
+        (void)lo;
+    };
+    auto read = [&](uint32_t nb) -> uint32_t {
+        prel -= (int32_t)nb;
+        p -= (int32_t)nb;
+        return (uint32_t)(buf >> ((uint32_t)prel & 63u)) & ((1u << nb) - 1u);
+    };
+    refill();
+    uint32_t s0 = read(tl);
+    uint32_t s1 = read(tl);
+    uint32_t i = 0;
+    for (; i + 4 <= len; i += 4) {
+        refill();
+        const uint32_t e0 = dtw[s0], e1 = dtw[s1];
+        s0 = (e0 & 0xFFFFu) + read(e0 >> 24);
+        s1 = (e1 & 0xFFFFu) + read(e1 >> 24);
+        const uint32_t f0 = dtw[s0], f1 = dtw[s1];
+        s0 = (f0 & 0xFFFFu) + read(f0 >> 24);
+        s1 = (f1 & 0xFFFFu) + read(f1 >> 24);
+        *reinterpret_cast<FZ_LDS uint32_t*>(out + i) = ((e0 >> 16) & 0xFFu) | (((e1 >> 16) & 0xFFu) << 8) |
+                                                       (((f0 >> 16) & 0xFFu) << 16) | (((f1 >> 16) & 0xFFu) << 24);
+    }
+    for (; i + 2 <= len; i += 2) {
+        refill();
+        const uint32_t e0 = dtw[s0], e1 = dtw[s1];
+        out[i] = (uint8_t)(e0 >> 16);
+        out[i + 1] = (uint8_t)(e1 >> 16);
+        s0 = (e0 & 0xFFFFu) + read(e0 >> 24);
+        s1 = (e1 & 0xFFFFu) + read(e1 >> 24);
+    }
+    if (i < len) out[i] = (uint8_t)(dtw[s0] >> 16);
 }
 
 // Decodes framed class-byte blocks (the blocks of every stream of a call in one launch); the host
@@ -580,7 +640,7 @@ __global__ void __launch_bounds__(256) k_fse_dec_blocks(const FseBlk* blks, int*
         __syncthreads();
         if (threadIdx.x == 0) {
             if (S.in[bsz - 1] == 0) *err = 1;
-            else fse_decode_run(S.dt, S.in, bsz, tl, S.out, len);
+            else fse_decode_run_lds(lds(S.dt), lds(S.in), bsz, tl, lds(S.out), len);
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = S.out[i];
@@ -1107,10 +1167,6 @@ constexpr int kDenseOut = 4096;     // dense chains: codes per round
 constexpr int kDenseOvf = 2048;     // dense chains: escaped values per round
 constexpr int kDnOvfLast = kDenseOvf - 1;
 
-// LDS pointers carry their address space so that every access is an LDS instruction.
-#define FZ_LDS __attribute__((address_space(3)))
-template <typename T>
-__device__ __forceinline__ FZ_LDS T* lds(T* p) { return (FZ_LDS T*)p; }
 
 struct SeqState {
     int64_t B;      // read position (bits below are unread)
