@@ -339,7 +339,7 @@ __device__ void dtable_parallel(const TL& L, unsigned tl, DEntry* dt) {
 constexpr unsigned kFblkLog = kFseDefaultTableLog;
 
 struct FseLds {
-    uint8_t in[kFblk];
+    alignas(16) uint8_t in[kFblk];
     uint8_t out[kFblk + 1024];
     uint32_t count[256];
     TableLdsT<kFblkLog> tab;
@@ -409,27 +409,76 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, FseLds
             fse_encode(w, s2, L.ct, in[--ip]);
             fse_encode(w, s1, L.ct, in[--ip]);
         }
-        // a block whose output reaches n - 1 bytes is stored raw: stop there (the LDS buffer holds
-        // n + 1024 bytes; one round writes at most 8)
-        // the two states are independent chains: each round forms both output widths, writes the
-        // bits in the reference's order (s2 then s1), then looks both next states up together
+        // A block whose output reaches n - 1 bytes is stored raw: stop once the flushed bytes reach
+        // n (the LDS buffer holds n + 1024 bytes, the check lags the bits by under 4 bytes and one
+        // round flushes 8). The two states are independent chains; the bits go out in the
+        // reference's order (s2 then s1). Four symbols per round (ip is a multiple of four here): one word of input, their table
+        // values read a round ahead (they do not depend on the states), the two state chains
+        // through LDS address-space pointers. The bits go to a 64-bit accumulator flushed 32 bits at
+        // a time (the byte stream is that of a byte-wise flush), after each pair of symbols.
         bool full = false;
-        const CTableT<kFblkLog>& ct = L.ct;
-        while (ip > 0) {
-            if (nc + w.pos >= n) { full = true; break; }
+        const FZ_LDS uint32_t* in32 = reinterpret_cast<const FZ_LDS uint32_t*>(lds(L.in));
+        const FZ_LDS uint32_t* dnb = lds(L.ct.deltaNbBits);
+        const FZ_LDS int32_t* dfs = lds(L.ct.deltaFindState);
+        const FZ_LDS uint16_t* stt = lds(L.ct.stateTable);
+        FZ_LDS uint8_t* ob = lds(w.buf);
+        uint64_t acc = w.acc;
+        uint32_t an = w.n;
+        uint32_t pos = (uint32_t)w.pos;
+        uint32_t da[4], fa[4];  // [0]: in[ip-1] (s2), [1]: in[ip-2] (s1), [2]: in[ip-3] (s2), [3]: in[ip-4] (s1)
+        auto load = [&](uint32_t ipx) {
+            const uint32_t v = in32[(ipx - 4) >> 2];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const unsigned a = in[ip - 1], b = in[ip - 2];
-                ip -= 2;
-                const uint32_t da = ct.deltaNbBits[a], db = ct.deltaNbBits[b];
-                const int32_t fa = ct.deltaFindState[a], fb = ct.deltaFindState[b];
-                const uint32_t na = (uint32_t)(((uint64_t)s2 + da) >> 16), nbb = (uint32_t)(((uint64_t)s1 + db) >> 16);
-                bw_add(w, s2, na);
-                bw_add(w, s1, nbb);
-                const uint32_t t2 = ct.stateTable[(s2 >> na) + fa], t1 = ct.stateTable[(s1 >> nbb) + fb];
-                s2 = t2;
-                s1 = t1;
+            for (int q = 0; q < 4; q++) {
+                const uint32_t sym = (v >> (8 * (3 - q))) & 0xFFu;
+                da[q] = dnb[sym];
+                fa[q] = (uint32_t)dfs[sym];
             }
+        };
+        auto put = [&](uint32_t st, uint32_t nbo) {
+            acc |= (uint64_t)(st & ((1u << nbo) - 1u)) << an;
+            an += nbo;
+        };
+        // under 32 bits pending before a pair, at most 2 * 12 added: the 64-bit accumulator holds it
+        auto flush = [&]() {
+            if (an >= 32) {
+                const uint32_t v = (uint32_t)acc;
+                ob[pos] = (uint8_t)v;
+                ob[pos + 1] = (uint8_t)(v >> 8);
+                ob[pos + 2] = (uint8_t)(v >> 16);
+                ob[pos + 3] = (uint8_t)(v >> 24);
+                pos += 4;
+                acc >>= 32;
+                an -= 32;
+            }
+        };
+        if (ip > 0) load(ip);
+        while (ip > 0) {
+            if (nc + pos >= n) { full = true; break; }
+            const uint32_t d0 = da[0], d1 = da[1], d2 = da[2], d3 = da[3];
+            const int32_t f0 = (int32_t)fa[0], f1 = (int32_t)fa[1], f2 = (int32_t)fa[2], f3 = (int32_t)fa[3];
+            ip -= 4;
+            if (ip > 0) load(ip);
+            const uint32_t n2 = (s2 + d0) >> 16, n1 = (s1 + d1) >> 16;
+            put(s2, n2);
+            put(s1, n1);
+            s2 = stt[(s2 >> n2) + f0];
+            s1 = stt[(s1 >> n1) + f1];
+            flush();
+            const uint32_t m2 = (s2 + d2) >> 16, m1 = (s1 + d3) >> 16;
+            put(s2, m2);
+            put(s1, m1);
+            s2 = stt[(s2 >> m2) + f2];
+            s1 = stt[(s1 >> m1) + f3];
+            flush();
+        }
+        w.acc = acc;
+        w.n = an;
+        w.pos = pos;
+        while (w.n >= 8) {
+            w.buf[w.pos++] = (uint8_t)w.acc;
+            w.acc >>= 8;
+            w.n -= 8;
         }
         if (full) {
             L.result = 0;
