@@ -78,6 +78,10 @@ _L.fz_config_save.argtypes = [C.POINTER(fz_config), _u8p]
 _L.fz_config_save.restype = C.c_size_t
 _L.fz_read_config.argtypes = [_u8p, C.c_size_t, C.POINTER(fz_config), C.c_char_p, C.c_size_t]
 _L.fz_read_config.restype = C.c_int
+_L.fz_config_set_dims.argtypes = [C.POINTER(fz_config), C.c_int, C.POINTER(C.c_uint64)]
+_L.fz_config_set_dims.restype = C.c_int
+_L.fz_config_load.argtypes = [_u8p, C.c_size_t, C.POINTER(fz_config), C.c_char_p, C.c_size_t]
+_L.fz_config_load.restype = C.c_int
 _L.fz_compress_size_bound.argtypes = [C.POINTER(fz_config), C.c_int]
 _L.fz_compress_size_bound.restype = C.c_size_t
 _L.fz_compress.argtypes = [C.c_void_p, C.POINTER(fz_config), C.c_int, C.c_void_p, C.c_int, _u8p, C.c_size_t,
@@ -163,7 +167,8 @@ _L.fz_server_get_stats.restype = C.c_int
 _L.fz_server_destroy.argtypes = [C.c_void_p]
 
 EXPORTED_SYMBOLS = ("fz_ctx_create", "fz_ctx_destroy", "fz_last_error", "fz_opts_init", "fz_config_init",
-                    "fz_config_size_est", "fz_config_save", "fz_read_config", "fz_compress_size_bound",
+                    "fz_config_size_est", "fz_config_save", "fz_read_config", "fz_config_set_dims",
+                    "fz_config_load", "fz_compress_size_bound",
                     "fz_compress", "fz_decompress", "fz_prezstd_bound", "fz_compress_device",
                     "fz_decompress_device", "fz_device_alloc", "fz_device_free", "fz_memcpy", "fz_synchronize",
                     "fz_zstd_version", "fz_profile_enable", "fz_profile_reset", "fz_profile_count",
@@ -314,6 +319,146 @@ def _is_device_tensor(x):
     return type(x).__module__.startswith("torch") and getattr(x, "is_cuda", False)
 
 
+_DL_CPU, _DL_CUDA, _DL_ROCM = 1, 2, 10  # DLDeviceType (dlpack.h)
+_DL_FLOAT = 2  # DLDataTypeCode kDLFloat
+
+
+class _DLTensor(C.Structure):  # DLTensor (dlpack.h)
+    _fields_ = [("data", C.c_void_p), ("device_type", C.c_int32), ("device_id", C.c_int32), ("ndim", C.c_int32),
+                ("code", C.c_uint8), ("bits", C.c_uint8), ("lanes", C.c_uint16),
+                ("shape", C.POINTER(C.c_int64)), ("strides", C.POINTER(C.c_int64)), ("byte_offset", C.c_uint64)]
+
+
+class _DLManagedTensor(C.Structure):
+    _fields_ = [("dl_tensor", _DLTensor), ("manager_ctx", C.c_void_p), ("deleter", C.c_void_p)]
+
+
+_DL_DELETER = C.CFUNCTYPE(None, C.c_void_p)
+_capi = C.pythonapi
+_capi.PyCapsule_GetPointer.restype = C.c_void_p
+_capi.PyCapsule_GetPointer.argtypes = [C.py_object, C.c_char_p]
+_capi.PyCapsule_SetName.restype = C.c_int
+_capi.PyCapsule_SetName.argtypes = [C.py_object, C.c_char_p]
+_capi.PyCapsule_IsValid.restype = C.c_int
+_capi.PyCapsule_IsValid.argtypes = [C.py_object, C.c_char_p]
+_capi.PyCapsule_New.restype = C.py_object
+_capi.PyCapsule_New.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p]
+
+
+class DLDeviceArray:
+    """A GPU array taken from a DLPack capsule: its device pointer, shape and dtype, with no copy and
+    no other GPU runtime involved. Owns the capsule (consumed, as the protocol asks) and calls the
+    producer's deleter on release()."""
+
+    def __init__(self, producer):
+        cap = producer.__dlpack__()
+        if not _capi.PyCapsule_IsValid(cap, b"dltensor"):
+            raise ValueError("not an unconsumed DLPack capsule")
+        mt = C.cast(_capi.PyCapsule_GetPointer(cap, b"dltensor"), C.POINTER(_DLManagedTensor))
+        _capi.PyCapsule_SetName(cap, b"used_dltensor")
+        self._cap, self._mt = cap, mt
+        t = mt.contents.dl_tensor
+        self.device_type, self.device = int(t.device_type), int(t.device_id)
+        self.shape = tuple(int(t.shape[i]) for i in range(t.ndim))
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        if t.code != _DL_FLOAT or t.bits not in (32, 64) or t.lanes != 1:
+            self.release()
+            raise TypeError("Unsupported dtype: only float32 and float64 arrays are compressed")
+        self.dtype = np.dtype(np.float32 if t.bits == 32 else np.float64)
+        if t.strides:
+            want, st = 1, []
+            for d in reversed(self.shape):
+                st.append(want)
+                want *= d
+            if any(int(t.strides[i]) != st[len(st) - 1 - i] for i in range(t.ndim) if self.shape[i] > 1):
+                self.release()
+                raise ValueError("DLPack array must be C-contiguous")
+        self.ptr = int(t.data or 0) + int(t.byte_offset)
+        self.nbytes = self.size * self.dtype.itemsize
+
+    def release(self):
+        mt, self._mt = getattr(self, "_mt", None), None
+        if mt and mt.contents.deleter:
+            _DL_DELETER(mt.contents.deleter)(C.cast(mt, C.c_void_p))
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def from_dlpack(x):
+    """Any array that speaks DLPack (__dlpack__ / __dlpack_device__), without a copy: a GPU array
+    becomes a DLDeviceArray (device pointer, shape, dtype), a host array a numpy array. numpy
+    arrays and torch tensors pass through unchanged."""
+    if isinstance(x, (np.ndarray, DLDeviceArray)) or type(x).__module__.startswith("torch"):
+        return x
+    if not (hasattr(x, "__dlpack__") and hasattr(x, "__dlpack_device__")):
+        return x
+    dev_type, _ = x.__dlpack_device__()
+    if int(dev_type) in (_DL_CUDA, _DL_ROCM):
+        return DLDeviceArray(x)
+    if int(dev_type) == _DL_CPU:
+        return np.from_dlpack(x)
+    raise TypeError(f"unsupported DLPack device type {int(dev_type)}")
+
+
+def _device_view(x):
+    """(pointer, element count, dtype code) of a device array, or None for host arrays."""
+    if isinstance(x, DLDeviceArray):
+        return x.ptr, x.size, _DT[x.dtype]
+    if _is_device_tensor(x):
+        import torch
+        dt = {torch.float32: 0, torch.float64: 1}.get(x.dtype)
+        if dt is None or not x.is_contiguous():
+            raise TypeError("device input must be a contiguous float32/float64 tensor")
+        return x.data_ptr(), x.numel(), dt
+    return None
+
+
+# DLManagedTensors handed out by DeviceArray.__dlpack__, by address, with what they point into;
+# the consumer's call of the deleter removes an entry (a capsule that is never consumed keeps its
+# few hundred bytes until the process ends).
+_DL_EXPORTS = {}
+
+
+def _dl_export_deleter(addr):
+    _DL_EXPORTS.pop(int(addr or 0), None)
+
+
+_DL_EXPORT_DELETER = _DL_DELETER(_dl_export_deleter)
+
+
+class DeviceArray:
+    """A typed view of a DeviceBuffer that other libraries can take through DLPack (ROCm device,
+    no copy). The buffer must not be freed while a consumer uses it."""
+
+    def __init__(self, buf, shape, dtype):
+        dtype = np.dtype(dtype)
+        if dtype not in _DT:
+            raise TypeError(f"Unsupported dtype: {dtype}")
+        n = int(np.prod(shape))
+        if n * dtype.itemsize > buf.nbytes:
+            raise ValueError("shape larger than the device buffer")
+        self.buf, self.shape, self.dtype = buf, tuple(int(s) for s in shape), dtype
+
+    def __dlpack_device__(self):
+        return (_DL_ROCM, int(self.buf.ctx.device))
+
+    def __dlpack__(self, stream=None, **kw):
+        shp = (C.c_int64 * max(1, len(self.shape)))(*self.shape)
+        mt = _DLManagedTensor()
+        t = mt.dl_tensor
+        t.data = self.buf.ptr.value if isinstance(self.buf.ptr, C.c_void_p) else self.buf.ptr
+        t.device_type, t.device_id, t.ndim = _DL_ROCM, int(self.buf.ctx.device), len(self.shape)
+        t.code, t.bits, t.lanes = _DL_FLOAT, 8 * self.dtype.itemsize, 1
+        t.shape, t.strides, t.byte_offset = C.cast(shp, C.POINTER(C.c_int64)), None, 0
+        mt.deleter = C.cast(_DL_EXPORT_DELETER, C.c_void_p)
+        _DL_EXPORTS[C.addressof(mt)] = (mt, shp, self.buf)
+        return _capi.PyCapsule_New(C.addressof(mt), b"dltensor", None)
+
+
 class DeviceBuffer:
     """Device memory owned by a Context (freed with it or by free())."""
 
@@ -339,6 +484,10 @@ class DeviceBuffer:
         if rc:
             _raise(rc, self.ctx._err())
         return out
+
+    def as_array(self, shape, dtype):
+        """This buffer as a typed array other libraries can take through DLPack."""
+        return DeviceArray(self, shape, dtype)
 
     def free(self):
         if self.ptr and self.ctx.h:
@@ -374,13 +523,12 @@ class Context:
     def compress_ex(self, data, config, simd=1, want_prezstd=False, copy_back=True):
         """Compress a numpy array (host) or a torch GPU tensor (device, overwritten in place).
         Returns (stream bytes as uint8 ndarray, fz_opts with timings, pre-zstd bytes or None)."""
-        on_dev = _is_device_tensor(data)
+        data = from_dlpack(data)
+        dv = _device_view(data)
+        on_dev = dv is not None
         if on_dev:
-            import torch
-            dt = {torch.float32: 0, torch.float64: 1}.get(data.dtype)
-            if dt is None or not data.is_contiguous():
-                raise TypeError("device input must be a contiguous float32/float64 tensor")
-            ptr, nbytes, size = data.data_ptr(), data.numel() * data.element_size(), data.numel()
+            ptr, size, dt = dv
+            nbytes = size * (4 if dt == 0 else 8)
         else:
             if data.dtype not in _DT:
                 raise TypeError(f"Unsupported dtype: {data.dtype}. Supported: float32, float64")
@@ -422,9 +570,10 @@ class Context:
         if out is None:
             hdr = read_config(b)
             out = np.empty(hdr.num_elements, dtype=dtype)
-        on_dev = _is_device_tensor(out)
-        ptr = out.data_ptr() if on_dev else out.ctypes.data
-        cap = out.numel() if on_dev else out.size
+        out = from_dlpack(out)
+        dv = _device_view(out)
+        on_dev = dv is not None
+        ptr, cap = (dv[0], dv[1]) if on_dev else (out.ctypes.data, out.size)
         rc = _L.fz_decompress(self.h, b.ctypes.data_as(_u8p), b.size, dt, C.c_void_p(ptr), cap, 1 if on_dev else 0,
                               C.byref(cfg.conf), C.byref(opts))
         if rc:
@@ -633,9 +782,11 @@ class falcons:
 
     @staticmethod
     def compress(data, config, copy=False):
-        if _is_device_tensor(data):
+        data = from_dlpack(data)
+        dv = _device_view(data)
+        if dv is not None:
             out, _, _ = default_context().compress_ex(data, config)
-            return out, data.numel() * data.element_size() / float(out.size)
+            return out, dv[1] * (4 if dv[2] == 0 else 8) / float(out.size)
         if data.dtype not in falcons._SUPPORTED_DTYPES:
             raise TypeError(f"Unsupported dtype: {data.dtype}. Supported: float32, float64")
         if data.size == 0:
@@ -670,6 +821,31 @@ class falcons:
         return out.reshape(shape), cfg
 
     @staticmethod
+    def decompress_into(compressed, out):
+        """Decompresses into an existing array: a DLPack producer or torch tensor on the GPU (the
+        field is written in HBM, no host copy), or a host array. Its element count must be the
+        stream's; its dtype gives the value type. Returns (out, config)."""
+        compressed = np.asarray(compressed)
+        if compressed.dtype != np.uint8:
+            raise TypeError(f"Compressed data must be uint8, got {compressed.dtype}")
+        out = from_dlpack(out)
+        dv = _device_view(out)
+        if dv is not None:
+            dt, n = (np.float32 if dv[2] == 0 else np.float64), dv[1]
+        else:
+            dt = out.dtype.type if out.dtype in falcons._SUPPORTED_DTYPES else None
+            n = out.size
+            if not out.flags["C_CONTIGUOUS"] or not out.flags["WRITEABLE"]:
+                raise ValueError("the output array must be C-contiguous and writable")
+        if dt is None:
+            raise TypeError(f"Unsupported dtype: {out.dtype}")
+        hdr = read_config(compressed)
+        if n != hdr.num_elements:
+            raise ValueError("Output size does not match the compressed element count")
+        _, cfg, _ = default_context().decompress_ex(compressed, dt, out)
+        return out, cfg
+
+    @staticmethod
     def verify(src_data, dec_data):
         if np.shape(src_data) != np.shape(dec_data):
             raise ValueError("Shape mismatch between src_data and dec_data")
@@ -692,5 +868,5 @@ class falcons:
         return max_diff, psnr, nrmse
 
 
-__all__ = ["falcons", "falconsConfig", "falconsErrorBoundMode", "falconsAlgorithm", "falconsInterpAlgorithm",
+__all__ = ["falcons", "from_dlpack", "DLDeviceArray", "DeviceArray", "falconsConfig", "falconsErrorBoundMode", "falconsAlgorithm", "falconsInterpAlgorithm",
            "Context", "DeviceBuffer", "zstd_version", "read_config", "FalconsError", "FalconsUnsupported", "LIB_PATH", "EXPORTED_SYMBOLS"]

@@ -144,9 +144,28 @@ extern "C" size_t fz_config_size_est(const fz_config* c) {
     return fz_config_save(c, tmp);
 }
 
+extern "C" int fz_config_set_dims(fz_config* c, int N, const uint64_t* dims) {
+    int n = 0;
+    uint64_t num = 1, kept[4] = {0, 0, 0, 0};
+    for (int i = 0; i < N; i++)
+        if (dims[i] > 1) {
+            if (n == 4) return FZ_ERR_INVALID_ARGUMENT;  // the compressors take at most four dimensions
+            kept[n++] = dims[i];
+            num *= dims[i];
+        }
+    if (n == 0) { kept[0] = 1; n = 1; }
+    for (int i = 0; i < 4; i++) c->dims[i] = kept[i];
+    c->N = n;
+    c->num = num;
+    c->pred_dim = n;
+    c->block_size = (n == 1 ? 128 : (n == 2 ? 16 : 6));
+    return FZ_OK;
+}
+
 namespace {
-// Config::load (Config.hpp:372-454) then the checks of Falcons_read_config (falcons.hpp:122-141).
-size_t load_config(const uint8_t* in, size_t avail, fz_config* c) {
+// Config::load (Config.hpp:372-454); with `stream`, also the checks of Falcons_read_config
+// (falcons.hpp:122-141): magic number, data version, and a payload after the header.
+size_t load_config(const uint8_t* in, size_t avail, fz_config* c, bool stream = true) {
     if (!in || avail < 1) throw ArgError("Truncated Falcons configuration");
     const size_t hs = in[0];
     if (hs < 1 || hs > avail) throw ArgError("Truncated Falcons configuration");
@@ -201,6 +220,11 @@ size_t load_config(const uint8_t* in, size_t avail, fz_config* c) {
     if (p < end) { need(4); c->quantbin_cnt = get<int32_t>(p); }
     if (p < end) { need(4); c->block_size = get<int32_t>(p); }
     if (p < end) { need(1); c->pred_dim = get<uint8_t>(p); }
+    c->interp_algo = 1;
+    c->interp_anchor_stride = -1;
+    c->interp_alpha = 1.25;
+    c->interp_beta = 2.0;
+    if (!stream) return hs;
     if (magic != kMagic) throw ArgError("magic number mismatch, the input data is not compressed by Falcons");
     if (ver != kDataVer) {
         char m[96];
@@ -209,13 +233,19 @@ size_t load_config(const uint8_t* in, size_t avail, fz_config* c) {
         throw ArgError(m);
     }
     if (avail <= hs) throw ArgError("Missing Falcons compressed payload");
-    c->interp_algo = 1;
-    c->interp_anchor_stride = -1;
-    c->interp_alpha = 1.25;
-    c->interp_beta = 2.0;
     return hs;
 }
 }  // namespace
+
+extern "C" int fz_config_load(const uint8_t* in, size_t len, fz_config* out, char* err, size_t errlen) {
+    try {
+        load_config(in, len, out, false);
+        return FZ_OK;
+    } catch (const std::exception& e) {
+        if (err && errlen) snprintf(err, errlen, "%s", e.what());
+        return FZ_ERR_INVALID_ARGUMENT;
+    }
+}
 
 extern "C" int fz_read_config(const uint8_t* cmp, size_t len, fz_config* out, char* err, size_t errlen) {
     try {

@@ -78,6 +78,13 @@ void fz_config_init(fz_config *c, int N, const uint64_t *dims);
 size_t fz_config_size_est(const fz_config *c);
 /* Config::save (Config.hpp:316-362); out needs 256 bytes */
 size_t fz_config_save(const fz_config *c, uint8_t *out);
+/* Config::setDims (Config.hpp:163-181): dimensions of extent 1 are dropped; N, num, pred_dim and
+ * block_size follow; the other fields are kept. FZ_ERR_INVALID_ARGUMENT (config unchanged) when more
+ * than four dimensions exceed 1, which Falcons_compress rejects (falcons.hpp:78-80). */
+int fz_config_set_dims(fz_config *c, int N, const uint64_t *dims);
+/* Config::load (Config.hpp:372-454): parse Config::save bytes (a stream header or the HDF5 filter's
+ * cd_values); unlike fz_read_config no magic, version or payload check */
+int fz_config_load(const uint8_t *in, size_t len, fz_config *out, char *err, size_t errlen);
 /* Falcons_read_config (falcons.hpp:122-141): validate and parse the stream header */
 int fz_read_config(const uint8_t *cmp, size_t len, fz_config *out, char *err, size_t errlen);
 /* Falcons_compress_size_bound<T> (FalconsImpl.hpp:44-54) */
