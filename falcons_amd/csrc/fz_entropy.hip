@@ -2744,11 +2744,17 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
         std::vector<unsigned int> hsv(6 + nc);
         d2h(hsv.data(), stats, 6 + nc, st);
         const unsigned int* hs = hsv.data();
-        fprintf(stderr, "[o1 decode] exact subsequences per chain (chain:count, nonzero):");
+        fprintf(stderr, "[o1 decode] chains (chain:exact subsequences/subsequences/escapes/bits a symbol/segment):");
         for (int q = 0; q < nc; q++)
-            if (hs[6 + q]) fprintf(stderr, " %d:%u/%u/esc%llu", q, hs[6 + q], hc[q].nsub, (unsigned long long)hc[q].novf);
+            fprintf(stderr, " %d:%u/%u/esc%llu/b%.2f/s%llu", q, hs[6 + q], hc[q].nsub, (unsigned long long)hc[q].novf,
+                    (double)hc[q].pos0 / (double)std::max<uint64_t>(1, hc[q].hi - hc[q].lo),
+                    (unsigned long long)hc[q].dset);
         fprintf(stderr, "\n");
-        fprintf(stderr, "[o1 decode] %zu dense chains decoded from their start\n", hd.size());
+        fprintf(stderr, "[o1 decode] %zu dense chains decoded from their start:", hd.size());
+        for (const O1DecChain& c : hd)
+            fprintf(stderr, " b%.2f/s%llu/esc%llu", (double)c.pos0 / (double)std::max<uint64_t>(1, c.hi - c.lo),
+                    (unsigned long long)c.dset, (unsigned long long)c.novf);
+        fprintf(stderr, "\n");
         fprintf(stderr, "[o1 decode] %d chains, %llu subsequences, %llu rounds, walk decoded %u subsequences "
                         "exactly (%u graphs over capacity [exits %u, nodes %u, branches %u, open edges %u], %u start "
                         "states outside the set)\n",
