@@ -57,10 +57,11 @@ KERNEL_SYMBOLS = {"o1_decode_dense": "k_o1d_dense", "o1_decode_graph": "k_o1d_gr
 def measured_traffic(name, a):
     """HBM bytes per launch of kernel `name` (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy) from
     the committed counter profile of the default field; None for other workloads or kernels."""
-    if (a.side, a.rel, a.seed) != (512, 1e-3, 2):
+    prof = {1e-3: "traffic_512cube_rel1e-3.json", 1e-4: "traffic_512cube_rel1e-4.json"}.get(a.rel)
+    if (a.side, a.seed) != (512, 2) or prof is None:
         return None
     try:
-        with open(os.path.join(ROOT, "profiles", "traffic_512cube_rel1e-3.json")) as f:
+        with open(os.path.join(ROOT, "profiles", prof)) as f:
             k = json.load(f)["kernels"].get(KERNEL_SYMBOLS.get(name, name))
     except (OSError, ValueError, KeyError):
         return None
