@@ -1170,6 +1170,8 @@ struct O1DecChain {
     uint32_t nsub;          // subsequences
     uint64_t tfirst;        // first subsequence (global index)
     int64_t sub_bits;
+    uint32_t hot0;          // the walk keeps the tables of contexts hot0 .. hot0 + 3 in LDS
+    uint32_t pad;
 };
 
 struct O1DecState {  // where a subsequence starts, or where it left off
@@ -1208,8 +1210,8 @@ __device__ inline int o1d_step(const DEntry* dt, BitWin& r, uint32_t& X, uint32_
 // Sequential decoding of a stretch of one chain from a bitstream window staged in LDS, for the
 // walk and for dense chains. win[k + kWinPad] holds chain byte wlo + k; the kWinPad bytes below are
 // zero when the window starts at the chain's first byte. The decoding tables of contexts
-// kWalkCtx0 .. kWalkCtx0 + kWalkCtxN - 1 sit in LDS (hot), the others are read from device memory.
-constexpr int kWalkCtx0 = 11, kWalkCtxN = 4;  // contexts whose decoding tables are kept in LDS
+// hot0 .. hot0 + kWalkCtxN - 1 sit in LDS (hot), the others are read from device memory.
+constexpr int kWalkCtx0 = 11, kWalkCtxN = 4;  // contexts 11..14: after large codes, where escapes are dense
 constexpr int kWinPad = 16;
 constexpr int kDenseStage = 12288;  // dense chains: bitstream bytes per window
 constexpr int kDenseOut = 4096;     // dense chains: codes per round
@@ -1234,9 +1236,9 @@ struct SeqState {
 // the symbol and the next context (top nibble of nbBits; 15 after an escape). Positions are 32-bit
 // offsets into the window. One LDS round trip and a few dozen scalar operations per symbol.
 __device__ __forceinline__ uint64_t seq_decode(SeqState& S, const FZ_LDS uint8_t* win, int64_t wlo, int64_t until,
-                                               uint64_t max_n, const FZ_LDS uint32_t* hot, const uint32_t* dtg,
-                                               const FZ_LDS int16_t* vbuf, uint64_t ve0, uint64_t ve1,
-                                               const int16_t* ov, uint64_t novf, FZ_LDS int16_t* obuf) {
+                                               uint64_t max_n, const FZ_LDS uint32_t* hot, uint32_t hot0,
+                                               const uint32_t* dtg, const FZ_LDS int16_t* vbuf, uint64_t ve0,
+                                               uint64_t ve1, const int16_t* ov, uint64_t novf, FZ_LDS int16_t* obuf) {
     const FZ_LDS uint32_t* w32 = reinterpret_cast<const FZ_LDS uint32_t*>(win);
     const int64_t base = (wlo - kWinPad) * 8;  // bit 0 of the window
     int32_t p = (int32_t)(S.B - base);
@@ -1250,7 +1252,7 @@ __device__ __forceinline__ uint64_t seq_decode(SeqState& S, const FZ_LDS uint8_t
     uint32_t k = 0;
     while (p > lim && k < kmax) {
         const uint32_t xi = X & ((1u << kO1Log) - 1);
-        const uint32_t hc = cx - kWalkCtx0;
+        const uint32_t hc = cx - hot0;
         const int32_t j = (p - 32) >> 5;  // words j, j+1 hold bits [32 j, 32 j + 64), which contain [p - 32, p)
         uint32_t raw = hot[(min(hc, (uint32_t)kWalkCtxN - 1) << kO1Log) + xi];
         const uint64_t w = (uint64_t)w32[j] | ((uint64_t)w32[j + 1] << 32);
@@ -1560,8 +1562,9 @@ __global__ void __launch_bounds__(256) k_o1d_glink(const O1DecChain* ch, int nch
 // One workgroup per chain: the true path through the graphs. Gives every subsequence its start
 // state, escape index and output offset (relative to the chain). A subsequence without a usable
 // graph (capacity exceeded, or a start state missing from its set) is decoded exactly here, by one
-// lane, with the tables of the contexts after large codes (11..14, most of the symbols where
-// escapes are dense) staged in LDS.
+// lane, with the tables of four consecutive contexts staged in LDS: the window the chain's segment
+// uses most (by the stationary distribution of its context chain; 11..14, the contexts after large
+// codes, where escapes are dense).
 constexpr int64_t kWalkStage = 16384;          // bitstream bytes staged per window
 
 __global__ void __launch_bounds__(kSeqLanes) k_o1d_walk(const uint8_t* in, const O1DecChain* ch, int nch,
@@ -1576,7 +1579,7 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_walk(const uint8_t* in, const
     const O1DecChain c = ch[q];
     const uint32_t* dt = reinterpret_cast<const uint32_t*>(dt_all + c.dset * kO1Ctx * (1u << kO1Log));
     for (uint32_t i = threadIdx.x; i < (kWalkCtxN << kO1Log); i += blockDim.x)
-        hot[i] = dt[((size_t)kWalkCtx0 << kO1Log) + i];
+        hot[i] = dt[((size_t)c.hot0 << kO1Log) + i];
     __syncthreads();
     const uint8_t* cb = in + c.off;
     const int16_t* ov = ovf + c.ovf_base;
@@ -1636,7 +1639,7 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_walk(const uint8_t* in, const
             // window reads reach 64 bits below the read position: stay above the staged bytes unless
             // they start at the chain's first byte (bits below it read as zero)
             const int64_t floor_bits = lo == 0 ? INT64_MIN : lo * 8 + 64;
-            off += seq_decode(z, lds(stage), w0 + kWinPad, max<int64_t>(stop, floor_bits), m - off, lds(hot), dt, nullptr, 0, 0,
+            off += seq_decode(z, lds(stage), w0 + kWinPad, max<int64_t>(stop, floor_bits), m - off, lds(hot), c.hot0, dt, nullptr, 0, 0,
                               ov, c.novf, nullptr);
         }
         if (z.bad) {
@@ -2550,11 +2553,10 @@ constexpr int64_t kWarmSubs = 4;
 // Chains below this many bits a symbol are decoded from their start.
 constexpr double kSeqBitsPerSymbol = 2.5;
 
-// The renumbering of the contexts of a dense chain in segment g: contexts by their share under the
-// stationary distribution of the segment's context chain (a code's context follows from the
-// previous code; an escape leads to 13 or 14, counted half each), the four largest first; with
-// esc set, 13 and 14 are among the four.
-static DenseMap dense_map(const uint8_t* host, uint64_t nf_off, bool esc) {
+// Stationary distribution of segment g's context chain (a code's context follows from the previous
+// code; an escape leads to 13 or 14, counted half each), from the normalized counts of its 15
+// tables at nf_off.
+static void context_shares(const uint8_t* host, uint64_t nf_off, double pi[kO1Ctx]) {
     double T[kO1Ctx][kO1Ctx] = {};
     for (int a = 0; a < kO1Ctx; a++) {
         double tot = 0;
@@ -2573,7 +2575,6 @@ static DenseMap dense_map(const uint8_t* host, uint64_t nf_off, bool esc) {
         }
         for (int b = 0; b < kO1Ctx; b++) T[a][b] = tot > 0 ? T[a][b] / tot : 0.0;
     }
-    double pi[kO1Ctx];
     for (int a = 0; a < kO1Ctx; a++) pi[a] = 1.0 / kO1Ctx;
     for (int it = 0; it < 64; it++) {
         double nx[kO1Ctx] = {};
@@ -2581,6 +2582,28 @@ static DenseMap dense_map(const uint8_t* host, uint64_t nf_off, bool esc) {
             for (int b = 0; b < kO1Ctx; b++) nx[b] += pi[a] * T[a][b];
         for (int b = 0; b < kO1Ctx; b++) pi[b] = nx[b];
     }
+}
+
+// The four consecutive contexts the walk keeps in LDS for a chain of the segment at nf_off: the
+// window with the largest share (ties: the higher window, so 11..14 where escapes dominate).
+static uint32_t hot_window(const uint8_t* host, uint64_t nf_off) {
+    double pi[kO1Ctx];
+    context_shares(host, nf_off, pi);
+    uint32_t best = kWalkCtx0;
+    double bw = -1;
+    for (int h = kO1Ctx - kWalkCtxN; h >= 0; h--) {
+        const double w = pi[h] + pi[h + 1] + pi[h + 2] + pi[h + 3];
+        if (w > bw + 1e-12) { bw = w; best = (uint32_t)h; }
+    }
+    return best;
+}
+
+// The renumbering of the contexts of a dense chain in segment g: contexts by their share under the
+// stationary distribution of the segment's context chain, the four largest first; with esc set,
+// 13 and 14 are among the four.
+static DenseMap dense_map(const uint8_t* host, uint64_t nf_off, bool esc) {
+    double pi[kO1Ctx];
+    context_shares(host, nf_off, pi);
     int order[kO1Ctx];
     for (int a = 0; a < kO1Ctx; a++) order[a] = a;
     std::stable_sort(order, order + kO1Ctx, [&](int x, int y) {
@@ -2633,7 +2656,7 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
         }
         const int64_t pos0 = top - (int64_t)kO1Log;
         if (all_seq || c[6] * 4096 > kDenseEscapes * m || (in_dense_seg(c[5]) && c[6] * 16384 > m)) {
-            hd.push_back(O1DecChain{c[0], c[1], c[2], c[3], c[4], c[6], c[5], pos0, x0, 1, 0, 0});
+            hd.push_back(O1DecChain{c[0], c[1], c[2], c[3], c[4], c[6], c[5], pos0, x0, 1, 0, 0, kWalkCtx0, 0});
             dense_bytes += c[1];
             dense_codes += m;
             continue;
@@ -2642,7 +2665,8 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
         const double bits_per_sym = std::max(1.0, (double)std::max<int64_t>(pos0, 1) / (double)m);
         const int64_t sub = std::max<int64_t>(2048, (int64_t)(bits_per_sym * kSubSymbols));
         const uint64_t ns = std::max<int64_t>(1, (pos0 + sub - 1) / sub);
-        hc.push_back(O1DecChain{c[0], c[1], c[2], c[3], c[4], c[6], c[5], pos0, x0, (uint32_t)ns, 0, sub});
+        hc.push_back(O1DecChain{c[0], c[1], c[2], c[3], c[4], c[6], c[5], pos0, x0, (uint32_t)ns, 0, sub,
+                                hot_window(host, nf_off[c[5]]), 0});
     }
     // Chains of fewer than 2.5 bits a symbol go to the sequential decoder as well: there a decoder
     // started from a guessed state seldom falls into step with the true one (few bits are read per
