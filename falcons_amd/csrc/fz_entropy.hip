@@ -1862,7 +1862,10 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
 
 __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, const O1DecChain* ch, const DEntry* dt_all,
                                                          const DenseMap* maps, const uint32_t* dtd_all,
-                                                         const int16_t* ovf, int16_t* out, int* err) {
+                                                         const int16_t* ovf, int16_t* out, int* err,
+                                                         uint64_t* times) {
+    // times (statistics only, else null): the 100 MHz clock at the chain's start and end
+    if (times && threadIdx.x == 0) times[2 * blockIdx.x] = __builtin_amdgcn_s_memrealtime();
     __shared__ uint32_t hot[kWalkCtxN << kO1Log];
     __shared__ __align__(16) uint32_t stage32[(kDenseStage + kWinPad + 32) / 4 + 4];
     uint8_t* stage = reinterpret_cast<uint8_t*>(stage32);
@@ -1907,6 +1910,7 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, cons
         i += nk;
     }
     if (threadIdx.x == 0 && (z.bad || i != m || z.e != c.novf)) *err = 1;
+    if (times && threadIdx.x == 0) times[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
 }
 
 // Final pass: every subsequence decodes from its settled start into its place in the code array.
@@ -2555,25 +2559,35 @@ constexpr double kSeqBitsPerSymbol = 2.5;
 
 // Stationary distribution of segment g's context chain (a code's context follows from the previous
 // code; an escape leads to 13 or 14, counted half each), from the normalized counts of its 15
-// tables at nf_off.
+// tables at nf_off. Counts of 1 and below (and the low-probability mark -1) are left out: the
+// encoder's sampled histograms add one to every symbol, and that floor, spread over the 232
+// symbols whose context is 13 or 14, would rank those two contexts far above their use (8192^2
+// field, segments 2-7: {0, 1, 13, 14} kept in LDS for 54% of the symbols instead of {0, 1, 2, 4}
+// for 86%). A table whose counts all sit at the floor keeps them.
 static void context_shares(const uint8_t* host, uint64_t nf_off, double pi[kO1Ctx]) {
     double T[kO1Ctx][kO1Ctx] = {};
     for (int a = 0; a < kO1Ctx; a++) {
-        double tot = 0;
-        for (int sy = 0; sy < kO1Sym; sy++) {
-            int16_t v;
-            memcpy(&v, host + nf_off + 2 * ((size_t)a * kO1Sym + sy), 2);
-            const double w = v == -1 ? 1.0 : (v > 0 ? (double)v : 0.0);
-            if (w == 0) continue;
-            tot += w;
-            if (sy == 255) {
-                T[a][13] += w / 2;
-                T[a][14] += w / 2;
-            } else {
-                T[a][o1_ctx((int)((sy >> 1) ^ (~(sy & 1) + 1)))] += w;
+        for (int floor_out = 1; floor_out >= 0; floor_out--) {
+            double tot = 0;
+            for (int b = 0; b < kO1Ctx; b++) T[a][b] = 0;
+            for (int sy = 0; sy < kO1Sym; sy++) {
+                int16_t v;
+                memcpy(&v, host + nf_off + 2 * ((size_t)a * kO1Sym + sy), 2);
+                const double w = floor_out ? (v > 1 ? (double)v : 0.0) : (v == -1 ? 1.0 : (v > 0 ? (double)v : 0.0));
+                if (w == 0) continue;
+                tot += w;
+                if (sy == 255) {
+                    T[a][13] += w / 2;
+                    T[a][14] += w / 2;
+                } else {
+                    T[a][o1_ctx((int)((sy >> 1) ^ (~(sy & 1) + 1)))] += w;
+                }
+            }
+            if (tot > 0) {
+                for (int b = 0; b < kO1Ctx; b++) T[a][b] /= tot;
+                break;
             }
         }
-        for (int b = 0; b < kO1Ctx; b++) T[a][b] = tot > 0 ? T[a][b] / tot : 0.0;
     }
     for (int a = 0; a < kO1Ctx; a++) pi[a] = 1.0 / kO1Ctx;
     for (int it = 0; it < 64; it++) {
@@ -2694,6 +2708,8 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
     }
     // dense chains run beside the rest, on the auxiliary stream when there is one
     hipStream_t dst = aux.stream ? aux.stream : st;
+    const bool stats_on = getenv("FALCONS_DECODE_STATS") != nullptr;
+    uint64_t* dense_times = nullptr;  // per sequential chain: clock at start and end (statistics only)
     std::vector<DenseMap> maps;  // read by a copy in flight until the function returns
     if (!hd.empty()) {
         O1DecChain* ddc = ws.take<O1DecChain>(hd.size());
@@ -2712,8 +2728,9 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
         uint32_t* dtd = ws.take<uint32_t>(hd.size() * kWalkCtx0 * (1u << kO1Log));
         FZ_LAUNCH("o1_decode_dense_tables", 0, k_o1d_dense_tables, dim3((unsigned)hd.size(), (unsigned)kWalkCtx0),
                   dim3(256), 0, dst, ddc, dt, dmaps, dtd);
+        if (stats_on) dense_times = ws.take<uint64_t>(2 * hd.size());
         FZ_LAUNCH("o1_decode_dense", dense_bytes + dense_codes * 2, k_o1d_dense, dim3((unsigned)hd.size()), dim3(kSeqLanes), 0,
-                  dst, dev, ddc, dt, dmaps, dtd, ovf, out, err);
+                  dst, dev, ddc, dt, dmaps, dtd, ovf, out, err, dense_times);
         FZ_HIP(hipGetLastError());
     }
     struct Join {  // the caller's stream waits for the dense chains however this function exits
@@ -2779,6 +2796,21 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
             fprintf(stderr, " b%.2f/s%llu/esc%llu", (double)c.pos0 / (double)std::max<uint64_t>(1, c.hi - c.lo),
                     (unsigned long long)c.dset, (unsigned long long)c.novf);
         fprintf(stderr, "\n");
+        if (dense_times) {
+            FZ_HIP(hipStreamSynchronize(dst));
+            std::vector<uint64_t> tt(2 * hd.size());
+            FZ_HIP(hipMemcpy(tt.data(), dense_times, tt.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+            fprintf(stderr, "[o1 decode] sequential chains (ms / ns a symbol / share of symbols in LDS contexts):");
+            for (size_t q = 0; q < hd.size(); q++) {
+                const double ms = (double)(tt[2 * q + 1] - tt[2 * q]) / 1e5;  // 100 MHz clock
+                double pi[kO1Ctx];
+                context_shares(host, nf_off[hd[q].dset], pi);
+                double hs = 0;
+                for (int r = 0; r < kWalkCtxN; r++) hs += pi[maps[q].perm[r]];
+                fprintf(stderr, " %.1f/%.0f/%.3f", ms, ms * 1e6 / (double)std::max<uint64_t>(1, hd[q].hi - hd[q].lo), hs);
+            }
+            fprintf(stderr, "\n");
+        }
         fprintf(stderr, "[o1 decode] %d chains, %llu subsequences, %llu rounds, walk decoded %u subsequences "
                         "exactly (%u graphs over capacity [exits %u, nodes %u, branches %u, open edges %u], %u start "
                         "states outside the set)\n",
