@@ -106,6 +106,8 @@ _L.fz_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int
 _L.fz_memcpy.restype = C.c_int
 _L.fz_synchronize.argtypes = [C.c_void_p]
 _L.fz_synchronize.restype = C.c_int
+_L.fz_ctx_stream.argtypes = [C.c_void_p]
+_L.fz_ctx_stream.restype = C.c_void_p
 _L.fz_zstd_version.argtypes = []
 _L.fz_zstd_version.restype = C.c_uint
 _L.fz_profile_enable.argtypes = [C.c_void_p, C.c_int]
@@ -171,6 +173,7 @@ EXPORTED_SYMBOLS = ("fz_ctx_create", "fz_ctx_destroy", "fz_last_error", "fz_opts
                     "fz_config_load", "fz_compress_size_bound",
                     "fz_compress", "fz_decompress", "fz_prezstd_bound", "fz_compress_device",
                     "fz_decompress_device", "fz_device_alloc", "fz_device_free", "fz_memcpy", "fz_synchronize",
+                    "fz_ctx_stream",
                     "fz_zstd_version", "fz_profile_enable", "fz_profile_reset", "fz_profile_count",
                     "fz_profile_entry", "fz_set_workers", "fz_compress_batch", "fz_decompress_batch",
                     "fz_server_create", "fz_server_submit", "fz_server_poll", "fz_server_get_stats",
@@ -350,8 +353,10 @@ class DLDeviceArray:
     no other GPU runtime involved. Owns the capsule (consumed, as the protocol asks) and calls the
     producer's deleter on release()."""
 
-    def __init__(self, producer):
-        cap = producer.__dlpack__()
+    def __init__(self, producer, stream=None):
+        # stream: the consuming stream (an integer handle); the producer orders its pending writes
+        # before it. None leaves the producer's default (the legacy default stream).
+        cap = producer.__dlpack__() if stream is None else producer.__dlpack__(stream=stream)
         if not _capi.PyCapsule_IsValid(cap, b"dltensor"):
             raise ValueError("not an unconsumed DLPack capsule")
         mt = C.cast(_capi.PyCapsule_GetPointer(cap, b"dltensor"), C.POINTER(_DLManagedTensor))
@@ -388,17 +393,18 @@ class DLDeviceArray:
             pass
 
 
-def from_dlpack(x):
+def from_dlpack(x, stream=None):
     """Any array that speaks DLPack (__dlpack__ / __dlpack_device__), without a copy: a GPU array
     becomes a DLDeviceArray (device pointer, shape, dtype), a host array a numpy array. numpy
-    arrays and torch tensors pass through unchanged."""
+    arrays and torch tensors pass through unchanged. `stream` is the stream that will read the GPU
+    array (the library's calls pass their context's stream)."""
     if isinstance(x, (np.ndarray, DLDeviceArray)) or type(x).__module__.startswith("torch"):
         return x
     if not (hasattr(x, "__dlpack__") and hasattr(x, "__dlpack_device__")):
         return x
     dev_type, _ = x.__dlpack_device__()
     if int(dev_type) in (_DL_CUDA, _DL_ROCM):
-        return DLDeviceArray(x)
+        return DLDeviceArray(x, stream)
     if int(dev_type) == _DL_CPU:
         return np.from_dlpack(x)
     raise TypeError(f"unsupported DLPack device type {int(dev_type)}")
@@ -447,6 +453,10 @@ class DeviceArray:
         return (_DL_ROCM, int(self.buf.ctx.device))
 
     def __dlpack__(self, stream=None, **kw):
+        # the library's work on this buffer is finished before any consumer stream reads it
+        rc = _L.fz_synchronize(self.buf.ctx.h) if getattr(self.buf.ctx, "h", None) else 0
+        if rc:
+            _raise(rc, self.buf.ctx._err())
         shp = (C.c_int64 * max(1, len(self.shape)))(*self.shape)
         mt = _DLManagedTensor()
         t = mt.dl_tensor
@@ -520,10 +530,14 @@ class Context:
     def _err(self):
         return _L.fz_last_error(self.h).decode()
 
+    def stream_handle(self):
+        """The context's HIP stream as an integer (the `stream` a DLPack producer is given)."""
+        return int(_L.fz_ctx_stream(self.h) or 0)
+
     def compress_ex(self, data, config, simd=1, want_prezstd=False, copy_back=True):
         """Compress a numpy array (host) or a torch GPU tensor (device, overwritten in place).
         Returns (stream bytes as uint8 ndarray, fz_opts with timings, pre-zstd bytes or None)."""
-        data = from_dlpack(data)
+        data = from_dlpack(data, self.stream_handle())
         dv = _device_view(data)
         on_dev = dv is not None
         if on_dev:
@@ -570,7 +584,7 @@ class Context:
         if out is None:
             hdr = read_config(b)
             out = np.empty(hdr.num_elements, dtype=dtype)
-        out = from_dlpack(out)
+        out = from_dlpack(out, self.stream_handle())
         dv = _device_view(out)
         on_dev = dv is not None
         ptr, cap = (dv[0], dv[1]) if on_dev else (out.ctypes.data, out.size)
@@ -782,7 +796,7 @@ class falcons:
 
     @staticmethod
     def compress(data, config, copy=False):
-        data = from_dlpack(data)
+        data = from_dlpack(data, default_context().stream_handle())
         dv = _device_view(data)
         if dv is not None:
             out, _, _ = default_context().compress_ex(data, config)
@@ -828,7 +842,7 @@ class falcons:
         compressed = np.asarray(compressed)
         if compressed.dtype != np.uint8:
             raise TypeError(f"Compressed data must be uint8, got {compressed.dtype}")
-        out = from_dlpack(out)
+        out = from_dlpack(out, default_context().stream_handle())
         dv = _device_view(out)
         if dv is not None:
             dt, n = (np.float32 if dv[2] == 0 else np.float64), dv[1]

@@ -983,6 +983,8 @@ extern "C" int fz_memcpy(fz_ctx* ctx, void* dst, const void* src, size_t bytes, 
     })
 }
 
+extern "C" void* fz_ctx_stream(const fz_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
 extern "C" int fz_synchronize(fz_ctx* ctx) {
     FZ_GUARD({
         FZ_HIP(hipSetDevice(ctx->device));

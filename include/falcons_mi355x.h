@@ -130,6 +130,11 @@ int fz_device_alloc(fz_ctx *ctx, size_t bytes, void **out);
 int fz_device_free(fz_ctx *ctx, void *p);
 int fz_memcpy(fz_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
 int fz_synchronize(fz_ctx *ctx);
+/* The context's HIP stream (a hipStream_t), for array exchange (no counterpart in the reference):
+ * a DLPack consumer passes it as __dlpack__(stream=...) so that the producer orders its pending
+ * writes before the library's kernels. The library's streams do not synchronize with the null
+ * stream. */
+void *fz_ctx_stream(const fz_ctx *ctx);
 
 /* Batched form (no counterpart in the reference, which compresses one field per call): many
  * independent device-resident fields in one call. Each item is exactly one fz_compress_device (or

@@ -74,6 +74,22 @@ def test_device_array_exports_a_rocm_capsule():
         fa.DeviceArray(FakeBuffer(0x1000, 64), (16, 8), np.float32)
 
 
+def test_consumer_stream_reaches_the_producer():
+    seen = {}
+
+    class Recorder(Producer):
+        def __dlpack__(self, **kw):
+            seen.update(kw)
+            return self.inner.__dlpack__(**kw)
+
+    arr = fa.DeviceArray(FakeBuffer(0x7F0000003000, 256), (64,), np.float32)
+    v = fa.from_dlpack(Recorder(arr), stream=0x1234)
+    assert seen == {"stream": 0x1234} and v.ptr == 0x7F0000003000
+    seen.clear()
+    fa.from_dlpack(Recorder(arr))
+    assert seen == {}  # no stream: the producer's default
+
+
 def test_exported_capsule_outlives_its_producer():
     import gc
     before = len(fa._DL_EXPORTS)
