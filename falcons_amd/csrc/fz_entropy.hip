@@ -1296,26 +1296,39 @@ __device__ __forceinline__ int64_t stage_window(FZ_LDS uint8_t* win, const uint8
     const int64_t s0 = lo - kWinPad;
     const int64_t w0 = s0 - (int64_t)(((uintptr_t)cb + (uintptr_t)s0) & 15u);
     const int64_t nblk = (hi - w0 + 15) / 16;
-    for (int64_t q = threadIdx.x; q < nblk; q += blockDim.x) {
-        const int64_t c0 = w0 + 16 * q;
-        u32x4 v;
-        if (c0 >= 0 && c0 + 16 <= len) {
-            v = *reinterpret_cast<const u32x4*>(cb + c0);
-        } else {
-            uint32_t w[4];
+    // kStageBatch blocks per lane are loaded before any is stored, so their loads are in flight
+    // together (a window is a dozen or so blocks per lane)
+    constexpr int kStageBatch = 8;
+    for (int64_t q0 = 0; q0 < nblk; q0 += (int64_t)blockDim.x * kStageBatch) {
+        u32x4 v[kStageBatch];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                uint32_t x = 0;
+        for (int b = 0; b < kStageBatch; b++) {
+            const int64_t q = q0 + (int64_t)b * blockDim.x + threadIdx.x;
+            const int64_t c0 = w0 + 16 * q;
+            if (q < nblk && c0 >= 0 && c0 + 16 <= len) {
+                v[b] = *reinterpret_cast<const u32x4*>(cb + c0);
+            } else if (q < nblk) {
+                uint32_t w[4];
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const int64_t i = c0 + 4 * j + t;
-                    x |= (i >= 0 && i < len ? (uint32_t)cb[i] : 0u) << (8 * t);
+                for (int j = 0; j < 4; j++) {
+                    uint32_t x = 0;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        const int64_t i = c0 + 4 * j + t;
+                        x |= (i >= 0 && i < len ? (uint32_t)cb[i] : 0u) << (8 * t);
+                    }
+                    w[j] = x;
                 }
-                w[j] = x;
+                v[b] = u32x4{w[0], w[1], w[2], w[3]};
+            } else {
+                v[b] = u32x4{0u, 0u, 0u, 0u};
             }
-            v = u32x4{w[0], w[1], w[2], w[3]};
         }
-        *reinterpret_cast<FZ_LDS u32x4*>(win + 16 * q) = v;
+#pragma unroll
+        for (int b = 0; b < kStageBatch; b++) {
+            const int64_t q = q0 + (int64_t)b * blockDim.x + threadIdx.x;
+            if (q < nblk) *reinterpret_cast<FZ_LDS u32x4*>(win + 16 * q) = v[b];
+        }
     }
     return w0;
 }
@@ -1895,7 +1908,23 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, cons
         const uint64_t e0 = z.e, ve = min<uint64_t>(c.novf, e0 + kDenseOvf);
         __syncthreads();
         const int64_t w0 = stage_window(lds(stage), cb, (int64_t)c.len, lo, hi);
-        for (uint64_t k = e0 + threadIdx.x; k < ve; k += blockDim.x) vbuf[k - e0] = ov[k];
+        {
+            // the round's escaped values, eight loads per lane in flight at a time
+            const uint32_t nv = (uint32_t)(ve - e0);  // at most kDenseOvf
+            for (uint32_t k0 = 0; k0 < nv; k0 += blockDim.x * 8u) {
+                int16_t t[8];
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    const uint32_t k = k0 + (uint32_t)b * blockDim.x + threadIdx.x;
+                    t[b] = k < nv ? ov[e0 + k] : (int16_t)0;
+                }
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    const uint32_t k = k0 + (uint32_t)b * blockDim.x + threadIdx.x;
+                    if (k < nv) vbuf[k] = t[b];
+                }
+            }
+        }
         __syncthreads();
         const int64_t floor_bits = lo == 0 ? INT64_MIN : lo * 8 + 64;
         const uint64_t nk =
