@@ -2814,11 +2814,12 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
         std::vector<unsigned int> hsv(6 + nc);
         d2h(hsv.data(), stats, 6 + nc, st);
         const unsigned int* hs = hsv.data();
-        fprintf(stderr, "[o1 decode] chains (chain:exact subsequences/subsequences/escapes/bits a symbol/segment):");
+        fprintf(stderr, "[o1 decode] chains (chain:exact subsequences/subsequences/escapes/bits a symbol/segment/"
+                        "first LDS context of the walk):");
         for (int q = 0; q < nc; q++)
-            fprintf(stderr, " %d:%u/%u/esc%llu/b%.2f/s%llu", q, hs[6 + q], hc[q].nsub, (unsigned long long)hc[q].novf,
+            fprintf(stderr, " %d:%u/%u/esc%llu/b%.2f/s%llu/h%u", q, hs[6 + q], hc[q].nsub, (unsigned long long)hc[q].novf,
                     (double)hc[q].pos0 / (double)std::max<uint64_t>(1, hc[q].hi - hc[q].lo),
-                    (unsigned long long)hc[q].dset);
+                    (unsigned long long)hc[q].dset, hc[q].hot0);
         fprintf(stderr, "\n");
         fprintf(stderr, "[o1 decode] %zu dense chains decoded from their start:", hd.size());
         for (const O1DecChain& c : hd)

@@ -192,6 +192,19 @@ def test_truncated_stream_raises(ctx):
             ctx.decompress_ex(stream[:cut], np.float32)
 
 
+def test_walk_with_low_context_window(ctx, oracle, monkeypatch, capfd):
+    """A smooth 2-D field whose coarse-level chains do not settle: the walk decodes subsequences
+    exactly with the tables of a low context window (not 11..14) in LDS. Same bytes as the oracle;
+    the decoder's statistics show the path was taken."""
+    import re
+    monkeypatch.setenv("FALCONS_DECODE_STATS", "1")
+    check(ctx, oracle, synth.power_law((4096, 4096), -3.0, seed=3), method=1, eb_mode=1, rel_eb=1e-3)
+    err = capfd.readouterr().err
+    chains = re.findall(r" (\d+):(\d+)/(\d+)/esc\d+/b[\d.]+/s\d+/h(\d+)", err)
+    assert chains, err[-2000:]
+    assert any(int(ex) > 0 and int(h) != 11 for _, ex, _, h in chains), chains
+
+
 SEQ_CASES = [
     ("no-delta", lambda: synth.power_law((96, 96, 96), -3.0, seed=5), dict(eb_mode=1, rel_eb=1e-4)),
     ("delta", lambda: quad3(), dict(eb_mode=0, abs_eb=2e-3, interp_algo=0)),
