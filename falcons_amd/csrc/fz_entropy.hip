@@ -338,9 +338,10 @@ __device__ void dtable_parallel(const TL& L, unsigned tl, DEntry* dt) {
 // FSE_compress bounds the table log by FSE_DEFAULT_TABLELOG (11).
 constexpr unsigned kFblkLog = kFseDefaultTableLog;
 
+// The encoder's LDS: the block's bytes, its histogram and its tables (about 72 KB, two workgroups
+// to a CU); the FSE bytes go straight to the block's output slot in device memory.
 struct FseLds {
     alignas(16) uint8_t in[kFblk];
-    uint8_t out[kFblk + 1024];
     uint32_t count[256];
     TableLdsT<kFblkLog> tab;
     CTableT<kFblkLog> ct;
@@ -348,9 +349,10 @@ struct FseLds {
     uint32_t tl, maxSym;
 };
 
-// Leaves the FSE bytes (when the result is > 1) in L.out. Returns 0 (not compressible), 1 (one
-// symbol), kFseErr, or the compressed size.
-__device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, FseLds& L) {
+// Writes the FSE bytes (when the result is > 1) to fo, which has room for n + 1024 bytes (a block
+// stored raw may leave up to n + 16 of them written). Returns 0 (not compressible), 1 (one symbol),
+// kFseErr, or the compressed size.
+__device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_t* fo, FseLds& L) {
     constexpr uint64_t kPending = ~1ull;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) L.in[i] = src[i];
     for (int s = threadIdx.x; s < 256; s += blockDim.x) L.count[s] = 0;
@@ -374,7 +376,7 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, FseLds
                 if (tl > kFblkLog || normalize_count(L.tab.norm, tl, L.count, n, maxSym) == kFseErr) {
                     r = kFseErr;
                 } else {
-                    L.nc = write_ncount(L.out, L.tab.norm, maxSym, tl);
+                    L.nc = write_ncount(fo, L.tab.norm, maxSym, tl);
                     L.tl = tl;
                     L.maxSym = maxSym;
                 }
@@ -395,7 +397,7 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, FseLds
     if (threadIdx.x == 0) {
         const uint64_t nc = L.nc;
         const uint8_t* in = L.in;
-        BitW w{L.out + nc, 0, 0, 0};
+        BitW w{fo + nc, 0, 0, 0};
         uint32_t ip = n, s1, s2;
         if (n & 1) {
             s1 = init_state2(&L.ct, in[--ip]);
@@ -410,8 +412,8 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, FseLds
             fse_encode(w, s1, L.ct, in[--ip]);
         }
         // A block whose output reaches n - 1 bytes is stored raw: stop once the flushed bytes reach
-        // n (the LDS buffer holds n + 1024 bytes, the check lags the bits by under 4 bytes and one
-        // round flushes 8). The two states are independent chains; the bits go out in the
+        // n (fo has room for n + 1024 bytes, the check lags the bits by under 4 bytes and one round
+        // flushes 8). The two states are independent chains; the bits go out in the
         // reference's order (s2 then s1). Four symbols per round (ip is a multiple of four here): one word of input, their table
         // values read a round ahead (they do not depend on the states), the two state chains
         // through LDS address-space pointers. The bits go to a 64-bit accumulator flushed 32 bits at
@@ -421,7 +423,7 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, FseLds
         const FZ_LDS uint32_t* dnb = lds(L.ct.deltaNbBits);
         const FZ_LDS int32_t* dfs = lds(L.ct.deltaFindState);
         const FZ_LDS uint16_t* stt = lds(L.ct.stateTable);
-        FZ_LDS uint8_t* ob = lds(w.buf);
+        uint8_t* ob = w.buf;
         uint64_t acc = w.acc;
         uint32_t an = w.n;
         uint32_t pos = (uint32_t)w.pos;
@@ -497,7 +499,7 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, FseLds
 // Frames one class-byte block as in RleFse:150-176: [0][raw] | [2][sym] | [1][u32 len][FSE bytes].
 // Returns the framed length written to out.
 __device__ uint64_t frame_block_wg(const uint8_t* src, uint32_t len, uint8_t* out, FseLds& L) {
-    const uint64_t r = fse_block_compress_wg(src, len, L);
+    const uint64_t r = fse_block_compress_wg(src, len, out + 5, L);
     uint64_t sz;
     if (r == kFseErr || r == 0 || r >= len) {
         if (threadIdx.x == 0) out[0] = 0;
@@ -512,8 +514,7 @@ __device__ uint64_t frame_block_wg(const uint8_t* src, uint32_t len, uint8_t* ou
             const uint32_t rr = (uint32_t)r;
             out[1] = (uint8_t)rr; out[2] = (uint8_t)(rr >> 8); out[3] = (uint8_t)(rr >> 16); out[4] = (uint8_t)(rr >> 24);
         }
-        for (uint32_t i = threadIdx.x; i < r; i += blockDim.x) out[5 + i] = L.out[i];
-        sz = 5 + r;
+        sz = 5 + r;  // the FSE bytes are in place
     }
     __syncthreads();
     return sz;
