@@ -550,11 +550,10 @@ __global__ void k_pack_slots(const uint8_t* slots, uint64_t slot_stride, const u
 // the decoding table; lane 0 decodes. Tables up to log 11 (all that FSE_compress writes) run with
 // the compressed bytes and the decoded bytes in LDS, copied in and out by the whole workgroup;
 // logs 12 and 13 read and write device memory directly.
-struct FseDecSmall {
+struct FseDecSmall {  // about 75 KB: two workgroups to a CU
     TableLdsT<kFblkLog> tab;
     DEntry dt[1u << kFblkLog];
     alignas(16) uint8_t in[kFblk + 64];
-    alignas(16) uint8_t out[kFblk];
 };
 struct FseDecBig {
     TableLdsT<kFseMaxTableLog> tab;
@@ -588,12 +587,14 @@ __device__ __forceinline__ void fse_decode_run(const DEntry* dt, const uint8_t* 
     if (i < len) dst[i] = dt[s0].symbol;
 }
 
-// The same loop with the table, the bitstream and the output in LDS (in: the bitstream, bsz
-// bytes, 4-byte aligned with 12 readable bytes past its end; out: 4-byte aligned). The bits below
-// the read position sit in a 64-bit buffer refilled from aligned LDS words once per two symbol
-// pairs (two pairs read at most 4 * 11 bits); the state values stay in vector registers.
+// The same loop with the table and the bitstream in LDS (in: the bitstream, bsz bytes, 4-byte
+// aligned with 12 readable bytes past its end); the symbols go to out in device memory, four to a
+// store when out is 4-byte aligned (kAligned4), else byte by byte. The bits below the read position
+// sit in a 64-bit buffer refilled from aligned LDS words once per two symbol pairs (two pairs read
+// at most 4 * 11 bits); the state values stay in vector registers.
+template <bool kAligned4>
 __device__ __forceinline__ void fse_decode_run_lds(const FZ_LDS DEntry* dt, const FZ_LDS uint8_t* in, uint64_t bsz,
-                                                   unsigned tl, FZ_LDS uint8_t* out, uint32_t len) {
+                                                   unsigned tl, uint8_t* out, uint32_t len) {
     const FZ_LDS uint32_t* w32 = reinterpret_cast<const FZ_LDS uint32_t*>(in);
     // entries as words: [15:0] next state base, [23:16] symbol, [31:24] bit count
     const FZ_LDS uint32_t* dtw = reinterpret_cast<const FZ_LDS uint32_t*>(dt);
@@ -629,8 +630,16 @@ __device__ __forceinline__ void fse_decode_run_lds(const FZ_LDS DEntry* dt, cons
         const uint32_t f0 = dtw[s0], f1 = dtw[s1];
         s0 = (f0 & 0xFFFFu) + read(f0 >> 24);
         s1 = (f1 & 0xFFFFu) + read(f1 >> 24);
-        *reinterpret_cast<FZ_LDS uint32_t*>(out + i) = ((e0 >> 16) & 0xFFu) | (((e1 >> 16) & 0xFFu) << 8) |
-                                                       (((f0 >> 16) & 0xFFu) << 16) | (((f1 >> 16) & 0xFFu) << 24);
+        const uint32_t v = ((e0 >> 16) & 0xFFu) | (((e1 >> 16) & 0xFFu) << 8) | (((f0 >> 16) & 0xFFu) << 16) |
+                           (((f1 >> 16) & 0xFFu) << 24);
+        if constexpr (kAligned4) {
+            *reinterpret_cast<uint32_t*>(out + i) = v;
+        } else {
+            out[i] = (uint8_t)v;
+            out[i + 1] = (uint8_t)(v >> 8);
+            out[i + 2] = (uint8_t)(v >> 16);
+            out[i + 3] = (uint8_t)(v >> 24);
+        }
     }
     for (; i + 2 <= len; i += 2) {
         refill();
@@ -690,10 +699,9 @@ __global__ void __launch_bounds__(256) k_fse_dec_blocks(const FseBlk* blks, int*
         __syncthreads();
         if (threadIdx.x == 0) {
             if (S.in[bsz - 1] == 0) *err = 1;
-            else fse_decode_run_lds(lds(S.dt), lds(S.in), bsz, tl, lds(S.out), len);
+            else if (((uintptr_t)dst & 3u) == 0) fse_decode_run_lds<true>(lds(S.dt), lds(S.in), bsz, tl, dst, len);
+            else fse_decode_run_lds<false>(lds(S.dt), lds(S.in), bsz, tl, dst, len);
         }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) dst[i] = S.out[i];
         return;
     }
     FseDecBig& G = L.g;
