@@ -338,10 +338,10 @@ __device__ void dtable_parallel(const TL& L, unsigned tl, DEntry* dt) {
 // FSE_compress bounds the table log by FSE_DEFAULT_TABLELOG (11).
 constexpr unsigned kFblkLog = kFseDefaultTableLog;
 
-// The encoder's LDS: the block's bytes, its histogram and its tables (about 72 KB, two workgroups
-// to a CU); the FSE bytes go straight to the block's output slot in device memory.
+// The encoder's LDS: the block's histogram and its tables (a few KB, so that many blocks run on a
+// CU at once); the block's bytes are read from device memory and the FSE bytes go straight to the
+// block's output slot.
 struct FseLds {
-    alignas(16) uint8_t in[kFblk];
     uint32_t count[256];
     TableLdsT<kFblkLog> tab;
     CTableT<kFblkLog> ct;
@@ -354,10 +354,9 @@ struct FseLds {
 // kFseErr, or the compressed size.
 __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_t* fo, FseLds& L) {
     constexpr uint64_t kPending = ~1ull;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) L.in[i] = src[i];
     for (int s = threadIdx.x; s < 256; s += blockDim.x) L.count[s] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&L.count[L.in[i]], 1u);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&L.count[src[i]], 1u);
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t r = kPending;
@@ -396,7 +395,7 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint64_t nc = L.nc;
-        const uint8_t* in = L.in;
+        const uint8_t* in = src;
         BitW w{fo + nc, 0, 0, 0};
         uint32_t ip = n, s1, s2;
         if (n & 1) {
@@ -419,7 +418,16 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_
         // through LDS address-space pointers. The bits go to a 64-bit accumulator flushed 32 bits at
         // a time (the byte stream is that of a byte-wise flush), after each pair of symbols.
         bool full = false;
-        const FZ_LDS uint32_t* in32 = reinterpret_cast<const FZ_LDS uint32_t*>(lds(L.in));
+        // the word of bytes [x - 4, x) of the block (x a multiple of four; 0 below byte 4), read
+        // three rounds before its round so that its load is in flight meanwhile
+        const bool al4 = ((uintptr_t)src & 3u) == 0;
+        auto word = [&](int64_t x) -> uint32_t {
+            if (x < 4) return 0u;
+            if (al4) return *reinterpret_cast<const uint32_t*>(src + x - 4);
+            return (uint32_t)src[x - 4] | ((uint32_t)src[x - 3] << 8) | ((uint32_t)src[x - 2] << 16) |
+                   ((uint32_t)src[x - 1] << 24);
+        };
+        uint32_t qa = word(ip), qb = word((int64_t)ip - 4), qc = word((int64_t)ip - 8);
         const FZ_LDS uint32_t* dnb = lds(L.ct.deltaNbBits);
         const FZ_LDS int32_t* dfs = lds(L.ct.deltaFindState);
         const FZ_LDS uint16_t* stt = lds(L.ct.stateTable);
@@ -428,8 +436,11 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_
         uint32_t an = w.n;
         uint32_t pos = (uint32_t)w.pos;
         uint32_t da[4], fa[4];  // [0]: in[ip-1] (s2), [1]: in[ip-2] (s1), [2]: in[ip-3] (s2), [3]: in[ip-4] (s1)
-        auto load = [&](uint32_t ipx) {
-            const uint32_t v = in32[(ipx - 4) >> 2];
+        auto load = [&](uint32_t ipx) {  // called for ipx = ip, ip - 4, ... in turn
+            const uint32_t v = qa;
+            qa = qb;
+            qb = qc;
+            qc = word((int64_t)ipx - 12);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const uint32_t sym = (v >> (8 * (3 - q))) & 0xFFu;
@@ -503,10 +514,10 @@ __device__ uint64_t frame_block_wg(const uint8_t* src, uint32_t len, uint8_t* ou
     uint64_t sz;
     if (r == kFseErr || r == 0 || r >= len) {
         if (threadIdx.x == 0) out[0] = 0;
-        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) out[1 + i] = L.in[i];
+        for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) out[1 + i] = src[i];
         sz = 1 + len;
     } else if (r == 1) {
-        if (threadIdx.x == 0) { out[0] = 2; out[1] = L.in[0]; }
+        if (threadIdx.x == 0) { out[0] = 2; out[1] = src[0]; }
         sz = 2;
     } else {
         if (threadIdx.x == 0) {
