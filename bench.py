@@ -293,15 +293,12 @@ class _View:
 
 
 def gen_chunks(synth, count, seed=4):
-    """The 4096-chunk mix: per-axis sides floor(exp(U(ln 32, ln 128))), chunk i a k^-3 field of seed
-    seed + i, REL bound drawn uniformly from {1e-2, 1e-3, 1e-4}."""
+    """The chunk mix of synth.chunk_mix, generated on host threads."""
     from concurrent.futures import ThreadPoolExecutor
-    sides = synth.chunk_sides(count, seed=seed)
-    rels = np.random.default_rng(seed + 7919).choice([1e-2, 1e-3, 1e-4], size=count)
+    sides, rels = synth.chunk_mix(count, seed=seed)
     workers = min(16, os.cpu_count() or 1)
     with ThreadPoolExecutor(workers) as ex:
-        fields = list(ex.map(lambda i: synth.power_law(tuple(int(v) for v in sides[i]), -3.0, seed=seed + i),
-                             range(count)))
+        fields = list(ex.map(lambda i: synth.chunk_field(sides, i, seed=seed), range(count)))
     return fields, rels
 
 
@@ -404,36 +401,15 @@ def bench_batched(a, fa, synth):
     ctx.close()
 
 
-def serving_mix(synth, count, seed=5):
-    """Request mix: 60% compress / 40% decompress; 70% 3D float32 side in [64, 512], 20% 2D float32 side
-    in [1024, 8192], 10% 3D float64 side in [128, 256] (sides log-uniform); ABS or REL bound
-    log-uniform in [1e-5, 1e-2]. Fields are crops of one k^-3 field per class at random offsets."""
-    rng = np.random.default_rng(seed)
-    return [{
-        "op": 0 if rng.random() < 0.6 else 1,
-        "cls": int(rng.choice(3, p=[0.7, 0.2, 0.1])),
-        "u": rng.random(3),
-        "mode": int(rng.integers(0, 2)),
-        "eb": float(10 ** rng.uniform(-5, -2)),
-    } for _ in range(count)]
-
-
 def bench_serving(a, fa, synth):
     """Poisson arrivals at --rate through fz_server; latency = completion - arrival per request."""
     t = time.time()
-    base = {0: synth.power_law((512, 512, 512), -3.0, seed=5, dtype=np.float32),
-            1: synth.power_law((8192, 8192), -3.0, seed=6, dtype=np.float32),
-            2: synth.power_law((256, 256, 256), -3.0, seed=7, dtype=np.float64)}
+    base = synth.serving_base()
     log(f"base fields generated in {time.time() - t:.1f} s")
-    lims = {0: (64, 512, 3), 1: (1024, 8192, 2), 2: (128, 256, 3)}
-    reqs = serving_mix(synth, a.warm_requests + a.requests)
+    reqs = synth.serving_mix(a.warm_requests + a.requests)
 
     def field_of(r):
-        lo, hi, nd = lims[r["cls"]]
-        side = int(np.floor(np.exp(np.log(lo) + r["u"][0] * (np.log(hi) - np.log(lo)))))
-        b = base[r["cls"]]
-        off = [int(r["u"][1 + (k % 2)] * (b.shape[k] - side)) for k in range(nd)]
-        return np.ascontiguousarray(b[tuple(slice(o, o + side) for o in off)])
+        return synth.serving_field(base, r)
 
     def config_of(r, f):
         c = fa.falconsConfig(*f.shape)

@@ -78,6 +78,8 @@ _L.fz_config_save.argtypes = [C.POINTER(fz_config), _u8p]
 _L.fz_config_save.restype = C.c_size_t
 _L.fz_read_config.argtypes = [_u8p, C.c_size_t, C.POINTER(fz_config), C.c_char_p, C.c_size_t]
 _L.fz_read_config.restype = C.c_int
+_L.fz_check_prezstd.argtypes = [C.POINTER(fz_config), C.c_int, _u8p, C.c_size_t, C.c_char_p, C.c_size_t]
+_L.fz_check_prezstd.restype = C.c_int
 _L.fz_config_set_dims.argtypes = [C.POINTER(fz_config), C.c_int, C.POINTER(C.c_uint64)]
 _L.fz_config_set_dims.restype = C.c_int
 _L.fz_config_load.argtypes = [_u8p, C.c_size_t, C.POINTER(fz_config), C.c_char_p, C.c_size_t]
@@ -177,7 +179,7 @@ EXPORTED_SYMBOLS = ("fz_ctx_create", "fz_ctx_destroy", "fz_last_error", "fz_opts
                     "fz_zstd_version", "fz_profile_enable", "fz_profile_reset", "fz_profile_count",
                     "fz_profile_entry", "fz_set_workers", "fz_compress_batch", "fz_decompress_batch",
                     "fz_server_create", "fz_server_submit", "fz_server_poll", "fz_server_get_stats",
-                    "fz_server_destroy", "fz_tune_report")
+                    "fz_server_destroy", "fz_tune_report", "fz_check_prezstd")
 
 
 def zstd_version():
@@ -301,6 +303,17 @@ class falconsConfig:
         buf = (C.c_uint8 * 256)()
         n = _L.fz_config_save(C.byref(self.conf), buf)
         return bytes(buf[:n])
+
+
+def check_prezstd(pre, config, dtype):
+    """fz_check_prezstd: raises ValueError unless the pre-zstd buffer is well formed for config (the
+    stream's configuration, e.g. from read_config) and dtype. Host only; needs no GPU."""
+    b = np.ascontiguousarray(np.frombuffer(bytes(pre), dtype=np.uint8) if not isinstance(pre, np.ndarray) else pre,
+                             dtype=np.uint8)
+    err = C.create_string_buffer(256)
+    rc = _L.fz_check_prezstd(C.byref(config.conf), _DT[np.dtype(dtype)], b.ctypes.data_as(_u8p), b.size, err, 256)
+    if rc:
+        _raise(rc, err.value.decode())
 
 
 def read_config(stream):
@@ -883,4 +896,4 @@ class falcons:
 
 
 __all__ = ["falcons", "from_dlpack", "DLDeviceArray", "DeviceArray", "falconsConfig", "falconsErrorBoundMode", "falconsAlgorithm", "falconsInterpAlgorithm",
-           "Context", "DeviceBuffer", "zstd_version", "read_config", "FalconsError", "FalconsUnsupported", "LIB_PATH", "EXPORTED_SYMBOLS"]
+           "Context", "DeviceBuffer", "zstd_version", "read_config", "check_prezstd", "FalconsError", "FalconsUnsupported", "LIB_PATH", "EXPORTED_SYMBOLS"]

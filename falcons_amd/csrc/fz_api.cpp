@@ -510,7 +510,7 @@ size_t gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, int simd
     if (c.eb_mode == 0) {
         FZ_HIP(hipMemcpyAsync(eb_dev, &c.abs_eb, 8, hipMemcpyHostToDevice, st));
     } else {
-        abs_error_bound(dtype, d_data, n, c.eb_mode, c.abs_eb, c.rel_eb, psnr_factor, mm_scratch, eb_dev, st);
+        abs_error_bound(dtype, d_data, n, simd, c.eb_mode, c.abs_eb, c.rel_eb, psnr_factor, mm_scratch, eb_dev, st);
     }
     double eb_host = c.abs_eb;
     const bool need_host_eb = c.eb_mode != 0 || c.cmpr_algo == 1;
@@ -605,41 +605,68 @@ size_t gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, int simd
 // copy (headers and offsets are read from it), d_pre its device copy when the caller already holds
 // one (otherwise the needed parts are uploaded first, outside the timed region). *d_out_io is the
 // device field (allocated in the workspace when null).
-void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h_pre, const uint8_t* d_pre,
-                    size_t pre_len, void** d_out_io, fz_opts* o) {
+// The headers of a pre-zstd buffer (decomposition.load, IDH:256-267; LinearQuantizer.hpp:154-176; the
+// code count, FGC:116-118), checked against the stream's configuration.
+struct PreLayout {
+    uint64_t dims[4];
+    int32_t algo, dir;
+    uint64_t anchor;
+    double alpha, beta, eb;
+    int32_t nun;
+    size_t un_off, enc_off, enc_len;
+    uint64_t ncodes;
+    Plan P;
+};
+
+PreLayout parse_pre(const fz_config& c, int dtype, const uint8_t* h_pre, size_t pre_len) {
     if (c.N > 3) throw Unsupported("4-D fields are not on this path");
-    hipStream_t st = ctx->stream;
     const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
     const uint64_t n = c.num;
-    // decomposition.load (IDH:256-267, LinearQuantizer.hpp:154-176)
+    PreLayout L;
     const uint8_t* p = h_pre;
     const uint8_t* bend = h_pre + pre_len;
     auto need = [&](size_t k) {
         if ((size_t)(bend - p) < k) throw ArgError("Truncated Falcons payload");
     };
-    uint64_t dims[4];
     need(decomp_head_bytes(c.N));
-    for (int i = 0; i < c.N; i++) dims[i] = get<uint64_t>(p);
+    for (int i = 0; i < c.N; i++) L.dims[i] = get<uint64_t>(p);
     (void)get<uint32_t>(p);
-    const int32_t algo = get<int32_t>(p);
-    const int32_t dir = get<int32_t>(p);
-    const uint64_t anchor = get<uint64_t>(p);
-    const double alpha = get<double>(p), beta = get<double>(p);
+    L.algo = get<int32_t>(p);
+    L.dir = get<int32_t>(p);
+    L.anchor = get<uint64_t>(p);
+    L.alpha = get<double>(p);
+    L.beta = get<double>(p);
     if (get<uint8_t>(p) != 2) throw ArgError("LinearQuantizer uid mismatch");
-    const double eb = get<double>(p);
+    L.eb = get<double>(p);
     (void)get<int32_t>(p);
-    const int32_t nun = get<int32_t>(p);
+    L.nun = get<int32_t>(p);
     uint64_t prod = 1;
-    for (int i = 0; i < c.N; i++) prod *= dims[i];
+    for (int i = 0; i < c.N; i++) prod *= L.dims[i];
     if (prod != n) throw ArgError("field shape does not match the header");
-    need((size_t)nun * es + 8);
-    const size_t un_off = (size_t)(p - h_pre);
-    p += (size_t)nun * es;
-    const uint64_t ncodes = get<uint64_t>(p);
-    const size_t enc_off = (size_t)(p - h_pre);
-    const size_t enc_len = pre_len - enc_off;
-    const Plan P = make_plan(c.N, dims, (long)anchor, algo, dir);
-    if (P.ncodes != ncodes) throw ArgError("code count does not match the field shape");
+    if (L.nun < 0 || (uint64_t)L.nun > n) throw ArgError("unpredictable count does not match the field");
+    need((size_t)L.nun * es + 8);
+    L.un_off = (size_t)(p - h_pre);
+    p += (size_t)L.nun * es;
+    L.ncodes = get<uint64_t>(p);
+    L.enc_off = (size_t)(p - h_pre);
+    L.enc_len = pre_len - L.enc_off;
+    L.P = make_plan(c.N, L.dims, (long)L.anchor, L.algo, L.dir);
+    if (L.P.ncodes != L.ncodes) throw ArgError("code count does not match the field shape");
+    entropy_check(h_pre + L.enc_off, L.enc_len, L.ncodes);
+    return L;
+}
+
+void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h_pre, const uint8_t* d_pre,
+                    size_t pre_len, void** d_out_io, fz_opts* o) {
+    hipStream_t st = ctx->stream;
+    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    const uint64_t n = c.num;
+    const PreLayout PL = parse_pre(c, dtype, h_pre, pre_len);
+    const int32_t nun = PL.nun;
+    const size_t un_off = PL.un_off, enc_off = PL.enc_off, enc_len = PL.enc_len;
+    const uint64_t ncodes = PL.ncodes;
+    const double eb = PL.eb, alpha = PL.alpha, beta = PL.beta;
+    const Plan& P = PL.P;
     DevArena& ws = ctx->arena;
     ws.reset();
     ws.reserve(decompress_workspace_bytes(n, dtype, (uint64_t)nun, pre_len));
@@ -694,6 +721,22 @@ void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h
 }
 
 }  // namespace
+
+extern "C" int fz_check_prezstd(const fz_config* conf, int dtype, const uint8_t* pre, size_t len, char* err,
+                                size_t errlen) {
+    try {
+        if (!conf || !pre) throw ArgError("null argument");
+        if (dtype != FZ_FLOAT && dtype != FZ_DOUBLE) throw Unsupported("only float and double fields are on this path");
+        parse_pre(*conf, dtype, pre, len);
+        return FZ_OK;
+    } catch (const Unsupported& e) {
+        if (err && errlen) snprintf(err, errlen, "%s", e.what());
+        return FZ_ERR_UNSUPPORTED;
+    } catch (const std::exception& e) {
+        if (err && errlen) snprintf(err, errlen, "%s", e.what());
+        return FZ_ERR_INVALID_ARGUMENT;
+    }
+}
 
 extern "C" void fz_opts_init(fz_opts* o) {
     memset(o, 0, sizeof *o);
@@ -757,6 +800,7 @@ size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_
     uint8_t* payload = out + est;
     size_t plen = 0;
     bool lossless = c.cmpr_algo == 4;
+    *mutated = false;
     std::vector<uint8_t> framed;
     double tz = 0;
     if (!lossless) {
@@ -781,14 +825,23 @@ size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_
                     o->prezstd_len = pre.size();
                     if (pre.size() <= o->prezstd_cap) memcpy(o->prezstd, pre.data(), pre.size());
                 }
-                const double tz0 = now_ms();
-                zstd_frame(pre.data(), pre.size(), framed);
-                tz += now_ms() - tz0;
-                plen = framed.size();
+                *mutated = true;
+                // Lossless_zstd::compress refuses a destination below ZSTD_compressBound of its input
+                // (Lossless_zstd.hpp:39-41); the dispatcher then stores the (already reconstructed)
+                // field losslessly instead (FalconsDispatcher.hpp:100-118)
+                const size_t room = cap - est;
+                const size_t need = getenv("FALCONS_NOZSTD") ? pre.size() : zstd().compressBound(pre.size());
+                if (room < 8 || room - 8 < need) {
+                    lossless = true;
+                } else {
+                    const double tz0 = now_ms();
+                    zstd_frame(pre.data(), pre.size(), framed);
+                    tz += now_ms() - tz0;
+                    plen = framed.size();
+                }
             }
         }
     }
-    *mutated = !lossless;
     if (lossless) {
         c.cmpr_algo = 4;
         std::vector<uint8_t> src(n * es);

@@ -2039,18 +2039,29 @@ __global__ void __launch_bounds__(256) k_ovf_values(const uint8_t* vcl, const ui
     const uint32_t zz = (uint32_t)(vv - 1);
     ovf[i] = (int16_t)(int32_t)((zz >> 1) ^ (~(zz & 1) + 1));
 }
-__global__ void k_runs_decode(const uint8_t* rcl, const uint64_t* roff, const uint8_t* rman, uint64_t nr,
-                              uint64_t* runlen) {
+// Mantissa fields are read only inside the stream's mantissa bytes (rman_bits, vman_bits); a class
+// that points past them marks the stream corrupt.
+__global__ void k_runs_decode(const uint8_t* rcl, const uint64_t* roff, const uint8_t* rman, uint64_t rman_bits,
+                              uint64_t nr, uint64_t* runlen, int* err) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nr) return;
     const int k = rcl[j];
+    if (k > 40 || roff[j] + (uint64_t)k > rman_bits) {
+        *err = 1;
+        runlen[j] = 0;
+        return;
+    }
     runlen[j] = ((1ull << k) | get_msb_field(rman, roff[j], k)) - 1;
 }
-__global__ void k_values_place(const uint8_t* vcl, const uint64_t* voff, const uint8_t* vman, uint64_t nv,
-                               const uint64_t* runsum, uint64_t n, int16_t* out) {
+__global__ void k_values_place(const uint8_t* vcl, const uint64_t* voff, const uint8_t* vman, uint64_t vman_bits,
+                               uint64_t nv, const uint64_t* runsum, uint64_t n, int16_t* out, int* err) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nv) return;
     const int k = vcl[j];
+    if (k > 17 || voff[j] + (uint64_t)k > vman_bits) {
+        *err = 1;
+        return;
+    }
     const uint64_t vv = (1ull << k) | get_msb_field(vman, voff[j], k);
     const uint32_t zz = (uint32_t)(vv - 1);
     const int32_t f = (int32_t)((zz >> 1) ^ (~(zz & 1) + 1));
@@ -2548,17 +2559,35 @@ static uint64_t get64(const uint8_t*& p) {
     return v;
 }
 
-// Block offsets of a framed class-byte stream, found by walking the block headers.
-static uint64_t frame_offsets(const uint8_t* h, uint64_t n, std::vector<uint64_t>& offs) {
+// Block offsets of a framed class-byte stream of n bytes (RleFse:150-176: per 64 K block [0][raw],
+// [2][symbol] or [1][u32 len][FSE_compress output]), found by walking the block headers within the
+// stream's avail bytes. Returns the stream's length; a header or block past avail, or an unknown
+// flag, is a corrupt stream.
+static uint64_t frame_offsets(const uint8_t* h, uint64_t n, uint64_t avail, std::vector<uint64_t>& offs) {
     uint64_t rp = 0, dp = 0;
     offs.clear();
+    auto corrupt = [] { throw std::invalid_argument("corrupt entropy stream: FSE block framing"); };
     while (dp < n) {
         const uint64_t len = std::min<uint64_t>(kFblk, n - dp);
         offs.push_back(rp);
+        if (rp >= avail) corrupt();
         const uint8_t flag = h[rp];
-        if (flag == 0) rp += 1 + len;
-        else if (flag == 2) rp += 2;
-        else { uint32_t rr; memcpy(&rr, h + rp + 1, 4); rp += 5 + rr; }
+        uint64_t blk;
+        if (flag == 0) {
+            blk = 1 + len;
+        } else if (flag == 2) {
+            blk = 2;
+        } else if (flag == 1) {
+            if (avail - rp < 5) corrupt();
+            uint32_t rr;
+            memcpy(&rr, h + rp + 1, 4);
+            if (rr < 2 || rr >= len) corrupt();  // FSE_compress output of a block is shorter than the block
+            blk = 5 + (uint64_t)rr;
+        } else {
+            corrupt();
+        }
+        if (avail - rp < blk) corrupt();
+        rp += blk;
         dp += len;
     }
     return rp;
@@ -2570,6 +2599,7 @@ struct FseDecItem {
     const uint8_t* hsrc;
     const uint8_t* dsrc;
     uint64_t n;
+    uint64_t avail;  // bytes of the framed stream
     uint8_t* out;
 };
 static void fse_dec_multi(const std::vector<FseDecItem>& items, int* err, DevArena& ws, hipStream_t st) {
@@ -2577,7 +2607,7 @@ static void fse_dec_multi(const std::vector<FseDecItem>& items, int* err, DevAre
     std::vector<uint64_t> offs;
     for (const FseDecItem& it : items) {
         if (it.n == 0) continue;
-        frame_offsets(it.hsrc, it.n, offs);
+        frame_offsets(it.hsrc, it.n, it.avail, offs);
         for (size_t k = 0; k < offs.size(); k++)
             blks.push_back(FseBlk{it.dsrc + offs[k], it.out + k * kFblk,
                                   (uint32_t)std::min<uint64_t>(kFblk, it.n - k * kFblk), 0});
@@ -2870,29 +2900,46 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
     FZ_HIP(hipGetLastError());
 }
 
-void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
-                    hipStream_t st, const AuxStream& aux) {
+// Where the parts of an entropy stream sit (RleFseEncoder::save / encode, RleFse:897-982, 1072), read
+// and checked on the host before anything runs on the device: every header, table, chain, escape
+// stream and FSE block lies inside the stream, or the stream is refused as corrupt.
+struct OvfPiece { uint64_t n, vcl_off, vman_off, nraw_off, mode, vbp, svcl; };
+struct EntropyLayout {
+    int method = 0, use_delta = 0;
+    // method 1 (order-1 tANS)
+    uint64_t S = 0, total_ovf = 0;
+    std::vector<uint64_t> chains;  // 7 values a chain, see o1_decode_parallel
+    std::vector<uint64_t> nf_off, ovf_base;
+    std::vector<OvfPiece> pieces;
+    // method 0 (RLE + FSE)
+    uint64_t nr = 0, nv = 0, s_rcl = 0, s_vcl = 0, rbp = 0, vbp = 0, o_rcl = 0, o_vcl = 0, o_rman = 0, o_vman = 0;
+};
+
+static void parse_entropy(const uint8_t* host, size_t len, uint64_t n, EntropyLayout& Lay) {
     if (len < 5) throw std::invalid_argument("Truncated entropy stream");
     const uint8_t* const hend = host + len;
     auto need_in = [&](const uint8_t* at, uint64_t k) {
         if (at > hend || (uint64_t)(hend - at) < k) throw std::invalid_argument("Truncated entropy stream");
     };
-    const int method = host[0] & 0x7F, use_delta = (host[0] & 0x80) != 0;
-    int* err = ws.take<int>(1);
-    FZ_HIP(hipMemsetAsync(err, 0, 4, st));
+    Lay.method = host[0] & 0x7F;
+    Lay.use_delta = (host[0] & 0x80) != 0;
+    if (Lay.method > 1) throw std::invalid_argument("corrupt entropy stream: unknown method");
     const uint8_t* hp = host + 5;
-    int16_t* out = use_delta ? ws.take<int16_t>(n + 1) : codes;
-    if (method == 1) {
+    if (Lay.method == 1) {
         need_in(hp, 8);
         uint64_t S = get64(hp);
         if (S > 16) throw std::invalid_argument("corrupt o1 tANS stream");
         if (!S) S = 1;
         const uint64_t seg = (n + S - 1) / S;
-        std::vector<uint64_t> chains;
-        std::vector<uint64_t> nf_off(S), ovf_sizes(S), ovf_base(S);
+        Lay.S = S;
+        std::vector<uint64_t>& chains = Lay.chains;
+        std::vector<uint64_t>& nf_off = Lay.nf_off;
+        std::vector<uint64_t>& ovf_base = Lay.ovf_base;
+        std::vector<OvfPiece>& pieces = Lay.pieces;
+        nf_off.assign(S, 0);
+        ovf_base.assign(S, 0);
+        pieces.assign(S, OvfPiece{});
         uint64_t total_ovf = 0;
-        struct OvfPiece { uint64_t n, vcl_off, vman_off, nraw_off, mode, vbp; };
-        std::vector<OvfPiece> pieces(S);
         for (uint64_t g = 0; g < S; g++) {
             const uint64_t lo = g * seg, hi = std::min(lo + seg, n), sn = hi - lo;
             need_in(hp, 16);
@@ -2918,7 +2965,7 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
                 hp += blen[k];
                 ovfo += nvk[k];
             }
-            OvfPiece pc{ovfo, 0, 0, 0, 0, 0};
+            OvfPiece pc{ovfo, 0, 0, 0, 0, 0, 0};
             need_in(hp, 1);
             pc.mode = *hp++;
             if (ovfo) {
@@ -2934,6 +2981,9 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
                     hp += svcl;
                     pc.vman_off = (uint64_t)(hp - host);
                     pc.vbp = vbp;
+                    pc.svcl = svcl;
+                    std::vector<uint64_t> offs;
+                    frame_offsets(host + pc.vcl_off, ovfo, svcl, offs);
                     need_in(hp, vbp);
                     if (vbp * 8 < ovfo) throw std::invalid_argument("corrupt o1 tANS escape stream");
                     hp += vbp;
@@ -2943,6 +2993,47 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
             ovf_base[g] = total_ovf;
             total_ovf += ovfo;
         }
+        Lay.total_ovf = total_ovf;
+        return;
+    }
+    // [u64 nr, nv, s_rcl, s_vcl, rbp, vbp][fse_enc(rcl)][fse_enc(vcl)][rman][vman] (RleFse:972-979)
+    need_in(hp, 48);
+    const uint64_t nr = get64(hp), nv = get64(hp), s_rcl = get64(hp), s_vcl = get64(hp), rbp = get64(hp),
+                   vbp = get64(hp);
+    const uint64_t room = (uint64_t)(hend - hp);
+    if (nr > n + 1 || nv > n || s_rcl > room || s_vcl > room - s_rcl || rbp > room - s_rcl - s_vcl ||
+        vbp > room - s_rcl - s_vcl - rbp)
+        throw std::invalid_argument("corrupt entropy stream: RLE header");
+    const uint64_t o_rcl = (uint64_t)(hp - host), o_vcl = o_rcl + s_rcl, o_rman = o_vcl + s_vcl,
+                   o_vman = o_rman + rbp;
+    Lay.nr = nr; Lay.nv = nv; Lay.s_rcl = s_rcl; Lay.s_vcl = s_vcl; Lay.rbp = rbp; Lay.vbp = vbp;
+    Lay.o_rcl = o_rcl; Lay.o_vcl = o_vcl; Lay.o_rman = o_rman; Lay.o_vman = o_vman;
+    {
+        std::vector<uint64_t> offs;
+        frame_offsets(host + o_rcl, nr, s_rcl, offs);
+        frame_offsets(host + o_vcl, nv, s_vcl, offs);
+    }
+}
+
+void entropy_check(const uint8_t* host, size_t len, uint64_t n) {
+    EntropyLayout Lay;
+    parse_entropy(host, len, n, Lay);
+}
+
+void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
+                    hipStream_t st, const AuxStream& aux) {
+    EntropyLayout Lay;
+    parse_entropy(host, len, n, Lay);
+    const int method = Lay.method, use_delta = Lay.use_delta;
+    int* err = ws.take<int>(1);
+    FZ_HIP(hipMemsetAsync(err, 0, 4, st));
+    int16_t* out = use_delta ? ws.take<int16_t>(n + 1) : codes;
+    if (method == 1) {
+        const uint64_t S = Lay.S, total_ovf = Lay.total_ovf;
+        const std::vector<uint64_t>& chains = Lay.chains;
+        const std::vector<uint64_t>& nf_off = Lay.nf_off;
+        const std::vector<uint64_t>& ovf_base = Lay.ovf_base;
+        const std::vector<OvfPiece>& pieces = Lay.pieces;
         int16_t* ovf = ws.take<int16_t>(total_ovf + 1);
         // the class bytes of every segment in one batch
         std::vector<uint8_t*> vcls(S, nullptr);
@@ -2951,7 +3042,7 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
             const OvfPiece& pc = pieces[g];
             if (!pc.n || pc.mode == 0) continue;
             vcls[g] = ws.take<uint8_t>(pc.n);
-            fd.push_back(FseDecItem{host + pc.vcl_off, dev + pc.vcl_off, pc.n, vcls[g]});
+            fd.push_back(FseDecItem{host + pc.vcl_off, dev + pc.vcl_off, pc.n, pc.svcl, vcls[g]});
         }
         fse_dec_multi(fd, err, ws, st);
         for (uint64_t g = 0; g < S; g++) {
@@ -2986,14 +3077,12 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
         }
         o1_decode_parallel(host, dev, chains, nf_off, dt, ovf, out, err, ws, st, aux);
     } else {
-        const uint64_t nr = get64(hp), nv = get64(hp), s_rcl = get64(hp), s_vcl = get64(hp), rbp = get64(hp),
-                       vbp = get64(hp);
-        const uint64_t o_rcl = (uint64_t)(hp - host), o_vcl = o_rcl + s_rcl, o_rman = o_vcl + s_vcl,
-                       o_vman = o_rman + rbp;
-        (void)vbp;
+        const uint64_t nr = Lay.nr, nv = Lay.nv, s_rcl = Lay.s_rcl, s_vcl = Lay.s_vcl, rbp = Lay.rbp, vbp = Lay.vbp;
+        const uint64_t o_rcl = Lay.o_rcl, o_vcl = Lay.o_vcl, o_rman = Lay.o_rman, o_vman = Lay.o_vman;
         uint8_t* rcl = ws.take<uint8_t>(nr + 1);
         uint8_t* vcl = ws.take<uint8_t>(nv + 1);
-        fse_dec_multi({{host + o_rcl, dev + o_rcl, nr, rcl}, {host + o_vcl, dev + o_vcl, nv, vcl}}, err, ws, st);
+        fse_dec_multi({{host + o_rcl, dev + o_rcl, nr, s_rcl, rcl}, {host + o_vcl, dev + o_vcl, nv, s_vcl, vcl}}, err, ws,
+                      st);
         // bit offsets of the mantissas
         uint64_t* rk = ws.take<uint64_t>(nr + 1);
         uint64_t* vk = ws.take<uint64_t>(nv + 1);
@@ -3015,7 +3104,8 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
         (void)rk; (void)vk;
         uint64_t* runlen = ws.take<uint64_t>(nr + 1);
         uint64_t* runsum = ws.take<uint64_t>(nr + 1);
-        FZ_LAUNCH("rle_runs_decode", 0, k_runs_decode, dim3(nblk(nr)), dim3(256), 0, st, rcl, roff, dev + o_rman, nr, runlen);
+        FZ_LAUNCH("rle_runs_decode", 0, k_runs_decode, dim3(nblk(nr)), dim3(256), 0, st, rcl, roff, dev + o_rman, rbp * 8,
+                  nr, runlen, err);
         FZ_HIP(hipGetLastError());
         {
             size_t b = 0;
@@ -3024,8 +3114,8 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
             FZ_HIP(hipcub::DeviceScan::InclusiveSum(t, b, runlen, runsum, (int64_t)nr, st));
         }
         FZ_HIP(hipMemsetAsync(out, 0, n * 2, st));
-        if (nv) FZ_LAUNCH("rle_values_place", 0, k_values_place, dim3(nblk(nv)), dim3(256), 0, st, vcl, voff, dev + o_vman, nv,
-                                   runsum, n, out);
+        if (nv) FZ_LAUNCH("rle_values_place", 0, k_values_place, dim3(nblk(nv)), dim3(256), 0, st, vcl, voff, dev + o_vman,
+                          vbp * 8, nv, runsum, n, out, err);
         FZ_HIP(hipGetLastError());
     }
     if (use_delta && n > 0) {

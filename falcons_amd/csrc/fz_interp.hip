@@ -203,18 +203,21 @@ __global__ void k_level_eb(const double* eb_dev, const double* ratio, int mode, 
     out[l] = L;
 }
 
-// min/max of the field (Statistic.hpp:98-204) for value-range based bounds.
+// min/max of the field (Statistic.hpp:98-204) for value-range based bounds. Each workgroup leaves the
+// min and max of its elements with NaNs passed over, and whether it met a NaN; without NaNs the
+// range is order-independent. Partials: [2 * nb] values, then nb NaN flags.
 template <typename T>
 __global__ void __launch_bounds__(256) k_minmax(const T* __restrict__ d, uint64_t n, T* part) {
     __shared__ T smin[256], smax[256];
-    // every partial starts from d[0] and takes an element only when it compares below / above, so the
-    // result does not depend on the order (a NaN enters only through d[0], as in the serial scan)
-    T mn = d[0], mx = d[0];
+    __shared__ int snan;
+    T mn = INFINITY, mx = -INFINITY;
+    bool nan = false;
     constexpr int V = 16 / sizeof(T);
     using Vec = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
     const uint64_t nv = ((uintptr_t)d & 15) == 0 ? n / V : 0;  // vector loads only on aligned fields
     const Vec* dv = reinterpret_cast<const Vec*>(d);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    if (threadIdx.x == 0) snan = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
         const Vec w = dv[i];
         const T* e = reinterpret_cast<const T*>(&w);
@@ -222,16 +225,19 @@ __global__ void __launch_bounds__(256) k_minmax(const T* __restrict__ d, uint64_
         for (int k = 0; k < V; k++) {
             mn = e[k] < mn ? e[k] : mn;
             mx = e[k] > mx ? e[k] : mx;
+            nan |= e[k] != e[k];
         }
     }
     for (uint64_t i = nv * V + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const T v = d[i];
         mn = v < mn ? v : mn;
         mx = v > mx ? v : mx;
+        nan |= v != v;
     }
     smin[threadIdx.x] = mn;
     smax[threadIdx.x] = mx;
     __syncthreads();
+    if (nan) snan = 1;
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             smin[threadIdx.x] = smin[threadIdx.x + w] < smin[threadIdx.x] ? smin[threadIdx.x + w] : smin[threadIdx.x];
@@ -242,22 +248,100 @@ __global__ void __launch_bounds__(256) k_minmax(const T* __restrict__ d, uint64_
     if (threadIdx.x == 0) {
         part[2 * blockIdx.x] = smin[0];
         part[2 * blockIdx.x + 1] = smax[0];
+        reinterpret_cast<int*>(part + 2 * gridDim.x)[blockIdx.x] = snan;
     }
 }
 
-// calAbsErrorBound (Statistic.hpp:216-240) once the partial min/max are in.
+// avx2_minmax (Statistic.hpp:97-157) on a field that holds NaNs, by one workgroup. The reference's
+// L = 32 / sizeof(T) lanes start at d[0] and take each full vector by MAXPS / MINPS, which return
+// the element unless the lane compares greater (less): a NaN element enters the lane, and the next
+// element of the lane replaces it together with all the lane held before. So a lane ends as NaN when
+// its last element is NaN, else as the max (min) of its elements after its last NaN, or of d[0] and
+// all its elements when it holds none (d[0] itself drops out when it is NaN). The lanes then fold into
+// lane 0 and the tail follows by std::max / std::min, which keep a NaN they hold and pass over a NaN
+// they are given.
 template <typename T>
-__global__ void __launch_bounds__(256) k_abs_eb(const T* part, int nparts, int mode, double abs_in, double rel,
-                                                double psnr_factor, double* eb_out) {
+__device__ void avx2_range_with_nans(const T* __restrict__ d, uint64_t n, T& out_min, T& out_max) {
+    constexpr int L = 32 / sizeof(T);
+    __shared__ unsigned long long last_nan[L];  // index + 1 of the lane's last NaN, 0 for none
     __shared__ T smin[256], smax[256];
-    T mn = part[0], mx = part[1];
+    const int t = threadIdx.x;
+    const uint64_t nfull = n / L * L;
+    if (t < L) last_nan[t] = 0;
+    __syncthreads();
+    for (uint64_t i = t; i < nfull; i += 256) {  // the stride is a multiple of L: lane t % L throughout
+        const T v = d[i];
+        if (v != v) atomicMax(&last_nan[i % L], (unsigned long long)(i + 1));
+    }
+    __syncthreads();
+    T mn = INFINITY, mx = -INFINITY;
+    const unsigned long long from = last_nan[t % L];
+    for (uint64_t i = t; i < nfull; i += 256) {
+        if (i + 1 > from) {
+            const T v = d[i];
+            mn = v < mn ? v : mn;
+            mx = v > mx ? v : mx;
+        }
+    }
+    smin[t] = mn;
+    smax[t] = mx;
+    __syncthreads();
+    for (int w = 128; w >= L; w >>= 1) {  // partners t and t + w share a lane
+        if (t < w) {
+            smin[t] = smin[t + w] < smin[t] ? smin[t + w] : smin[t];
+            smax[t] = smax[t + w] > smax[t] ? smax[t + w] : smax[t];
+        }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    T lmax[L], lmin[L];
+    const T d0 = d[0];
+    for (int k = 0; k < L; k++) {
+        const unsigned long long ln = last_nan[k];
+        if (ln == nfull - L + k + 1) {
+            lmax[k] = lmin[k] = NAN;
+        } else if (ln > 0 || d0 != d0) {
+            lmax[k] = smax[k];
+            lmin[k] = smin[k];
+        } else {
+            lmax[k] = smax[k] > d0 ? smax[k] : d0;
+            lmin[k] = smin[k] < d0 ? smin[k] : d0;
+        }
+    }
+    out_max = lmax[0];
+    out_min = lmin[0];
+    for (int k = 1; k < L; k++) {
+        out_max = out_max < lmax[k] ? lmax[k] : out_max;
+        out_min = lmin[k] < out_min ? lmin[k] : out_min;
+    }
+    for (uint64_t i = nfull; i < n; i++) {
+        out_max = out_max < d[i] ? d[i] : out_max;
+        out_min = d[i] < out_min ? d[i] : out_min;
+    }
+}
+
+// calAbsErrorBound (Statistic.hpp:216-240) once the partial min/max are in (data_range with openmp
+// off, :160-178). Without NaNs every build gives max - min. With NaNs: scalar_minmax (the scalar
+// build, and fields of 16 values or fewer) starts from d[0] and passes over every other NaN, so the
+// range is NaN exactly when d[0] is; the AVX2 build follows the lane rule above.
+template <typename T>
+__global__ void __launch_bounds__(256) k_abs_eb(const T* d, uint64_t n, int simd, const T* part, int nparts, int mode,
+                                                double abs_in, double rel, double psnr_factor, double* eb_out) {
+    __shared__ T smin[256], smax[256];
+    __shared__ int snan;
+    if (threadIdx.x == 0) snan = 0;
+    T mn = INFINITY, mx = -INFINITY;
+    bool nan = false;
+    const int* flags = reinterpret_cast<const int*>(part + 2 * nparts);
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         mn = part[2 * b] < mn ? part[2 * b] : mn;
         mx = part[2 * b + 1] > mx ? part[2 * b + 1] : mx;
+        nan |= flags[b] != 0;
     }
     smin[threadIdx.x] = mn;
     smax[threadIdx.x] = mx;
     __syncthreads();
+    if (nan) snan = 1;
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             smin[threadIdx.x] = smin[threadIdx.x + w] < smin[threadIdx.x] ? smin[threadIdx.x + w] : smin[threadIdx.x];
@@ -265,9 +349,16 @@ __global__ void __launch_bounds__(256) k_abs_eb(const T* part, int nparts, int m
         }
         __syncthreads();
     }
-    if (threadIdx.x != 0) return;
     mn = smin[0];
     mx = smax[0];
+    if (snan) {
+        if (simd && n > 16) {
+            avx2_range_with_nans(d, n, mn, mx);
+        } else if (d[0] != d[0]) {
+            mn = mx = d[0];
+        }
+    }
+    if (threadIdx.x != 0) return;
     const double range = (double)(T)(mx - mn);
     double eb = abs_in;
     switch (mode) {
@@ -384,19 +475,19 @@ void level_constants(const double* eb_dev, const double* ratio_dev, int mode, in
     FZ_HIP(hipGetLastError());
 }
 
-size_t minmax_scratch_bytes() { return 2 * 4096 * sizeof(double); }
+size_t minmax_scratch_bytes() { return 2 * 4096 * sizeof(double) + 4096 * sizeof(int); }
 
-void abs_error_bound(int dtype, const void* data, uint64_t n, int mode, double abs_in, double rel, double psnr_factor,
-                     void* scratch, double* eb_dev, hipStream_t st) {
+void abs_error_bound(int dtype, const void* data, uint64_t n, int simd, int mode, double abs_in, double rel,
+                     double psnr_factor, void* scratch, double* eb_dev, hipStream_t st) {
     const int nb = (int)std::min<uint64_t>(4096, (n + 1023) / 1024);
     if (dtype == kF32) {
         FZ_LAUNCH("minmax", n * 4, k_minmax<float>, dim3(nb), dim3(256), 0, st, (const float*)data, n, (float*)scratch);
-        FZ_LAUNCH("abs_eb", 0, k_abs_eb<float>, dim3(1), dim3(256), 0, st, (const float*)scratch, nb, mode, abs_in, rel,
-                           psnr_factor, eb_dev);
+        FZ_LAUNCH("abs_eb", 0, k_abs_eb<float>, dim3(1), dim3(256), 0, st, (const float*)data, n, simd,
+                  (const float*)scratch, nb, mode, abs_in, rel, psnr_factor, eb_dev);
     } else {
         FZ_LAUNCH("minmax", n * 8, k_minmax<double>, dim3(nb), dim3(256), 0, st, (const double*)data, n, (double*)scratch);
-        FZ_LAUNCH("abs_eb", 0, k_abs_eb<double>, dim3(1), dim3(256), 0, st, (const double*)scratch, nb, mode, abs_in, rel,
-                           psnr_factor, eb_dev);
+        FZ_LAUNCH("abs_eb", 0, k_abs_eb<double>, dim3(1), dim3(256), 0, st, (const double*)data, n, simd,
+                  (const double*)scratch, nb, mode, abs_in, rel, psnr_factor, eb_dev);
     }
     FZ_HIP(hipGetLastError());
 }

@@ -99,8 +99,8 @@ void interp_anchors(int dtype, bool decomp, void* data, void* unpred, const uint
 void level_constants(const double* eb_dev, const double* ratio_dev, int mode, int levels, LevelEb* out,
                      hipStream_t st);
 size_t minmax_scratch_bytes();
-void abs_error_bound(int dtype, const void* data, uint64_t n, int mode, double abs_in, double rel, double psnr_factor,
-                     void* scratch, double* eb_dev, hipStream_t st);
+void abs_error_bound(int dtype, const void* data, uint64_t n, int simd, int mode, double abs_in, double rel,
+                     double psnr_factor, void* scratch, double* eb_dev, hipStream_t st);
 size_t select_escapes_scratch(uint64_t n);
 void select_escapes(const int16_t* codes, uint64_t n, uint64_t* out_idx, uint64_t* nsel_dev, void* scratch,
                     size_t scratch_bytes, hipStream_t st);
@@ -127,6 +127,8 @@ struct AuxStream {
     hipStream_t stream = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
 };
+// Host-side check of an entropy stream's headers and framing (throws std::invalid_argument).
+void entropy_check(const uint8_t* host, size_t len, uint64_t n);
 void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
                     hipStream_t st, const AuxStream& aux);
 

@@ -60,3 +60,52 @@ def front(shape, seed=3, dtype=np.float64):
 def chunk_sides(count, seed=4, lo=32, hi=128):
     rng = np.random.default_rng(seed)
     return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), size=(count, 3)))).astype(int)
+
+
+def chunk_mix(count, seed=4):
+    """The batched chunk mix: per-axis sides floor(exp(U(ln 32, ln 128))) (seed), chunk i a k^-3
+    float32 field of seed seed + i, REL bound drawn uniformly from {1e-2, 1e-3, 1e-4} (seed + 7919).
+    Returns (sides [count, 3], rels [count]); chunk_field(sides, i, seed) makes chunk i."""
+    sides = chunk_sides(count, seed=seed)
+    rels = np.random.default_rng(seed + 7919).choice([1e-2, 1e-3, 1e-4], size=count)
+    return sides, rels
+
+
+def chunk_field(sides, i, seed=4):
+    return power_law(tuple(int(v) for v in sides[i]), -3.0, seed=seed + i)
+
+
+# request classes of the serving mix: (side lo, side hi, ndim, dtype, base field shape, base seed)
+SERVE_CLASSES = {
+    0: (64, 512, 3, np.float32, (512, 512, 512), 5),
+    1: (1024, 8192, 2, np.float32, (8192, 8192), 6),
+    2: (128, 256, 3, np.float64, (256, 256, 256), 7),
+}
+
+
+def serving_mix(count, seed=5):
+    """Request mix: 60% compress / 40% decompress; 70% 3D float32 side in [64, 512], 20% 2D float32 side
+    in [1024, 8192], 10% 3D float64 side in [128, 256] (sides log-uniform); ABS or REL bound
+    log-uniform in [1e-5, 1e-2]. Fields are crops of one k^-3 field per class (serving_base) at
+    random offsets (serving_field)."""
+    rng = np.random.default_rng(seed)
+    return [{
+        "op": 0 if rng.random() < 0.6 else 1,
+        "cls": int(rng.choice(3, p=[0.7, 0.2, 0.1])),
+        "u": rng.random(3),
+        "mode": int(rng.integers(0, 2)),
+        "eb": float(10 ** rng.uniform(-5, -2)),
+    } for _ in range(count)]
+
+
+def serving_base(classes=(0, 1, 2)):
+    return {c: power_law(SERVE_CLASSES[c][4], -3.0, seed=SERVE_CLASSES[c][5], dtype=SERVE_CLASSES[c][3])
+            for c in classes}
+
+
+def serving_field(base, r):
+    lo, hi, nd = SERVE_CLASSES[r["cls"]][:3]
+    side = int(np.floor(np.exp(np.log(lo) + r["u"][0] * (np.log(hi) - np.log(lo)))))
+    b = base[r["cls"]]
+    off = [int(r["u"][1 + (k % 2)] * (b.shape[k] - side)) for k in range(nd)]
+    return np.ascontiguousarray(b[tuple(slice(o, o + side) for o in off)])

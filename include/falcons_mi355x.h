@@ -111,8 +111,9 @@ int fz_decompress(fz_ctx *ctx, const uint8_t *cmp, size_t len, int dtype, void *
  * capacity pre_cap, at least fz_prezstd_bound bytes); *pre_len receives their length. conf is
  * updated as Falcons_compress_Interp updates it (absolute bound, anchor stride;
  * include/Falcons/api/impl/FalconsAlgoInterp.hpp:36-40). d_data (device) is overwritten with the
- * reconstructed values. ALGO_INTERP only: ALGO_INTERP_LORENZO's tuner samples on the host
- * (use fz_compress). Lossless configurations return FZ_ERR_UNSUPPORTED. */
+ * reconstructed values. ALGO_INTERP_LORENZO runs its parameter search on the device
+ * (FalconsAlgoInterp.hpp:273-507) and conf comes back with cmpr_algo set to ALGO_INTERP and the
+ * chosen interpolation settings. Lossless configurations return FZ_ERR_UNSUPPORTED. */
 size_t fz_prezstd_bound(const fz_config *c, int dtype);
 int fz_compress_device(fz_ctx *ctx, fz_config *conf, int dtype, void *d_data, uint8_t *d_pre, size_t pre_cap,
                        size_t *pre_len, fz_opts *opts);
@@ -123,6 +124,15 @@ int fz_compress_device(fz_ctx *ctx, fz_config *conf, int dtype, void *d_data, ui
  * (fz_read_config). d_out (device) receives conf->num values. */
 int fz_decompress_device(fz_ctx *ctx, const fz_config *conf, int dtype, const uint8_t *h_pre, const uint8_t *d_pre,
                          size_t pre_len, void *d_out, size_t out_cap_elems, fz_opts *opts);
+
+/* Checks, on the host and without a GPU, that a pre-zstd buffer (what fz_decompress_device takes) is
+ * well formed for the stream configuration conf: the decomposition and quantizer headers
+ * (InterpolationDecomposition.hpp:256-267, LinearQuantizer.hpp:154-176), the code count, and every
+ * header, table, chain, escape stream and FSE block of the entropy stream
+ * (RleFseEncoder.hpp:897-1070) lying inside the buffer. The decoding entry points run the same
+ * checks before any kernel starts. No counterpart in the reference, whose decoders read such
+ * buffers unchecked. FZ_OK, or FZ_ERR_INVALID_ARGUMENT with the reason in err. */
+int fz_check_prezstd(const fz_config *conf, int dtype, const uint8_t *pre, size_t len, char *err, size_t errlen);
 
 /* Device memory helpers on the context's GPU, so that callers need no other GPU runtime binding.
  * fz_memcpy kind: 0 host->device, 1 device->host, 2 device->device; synchronous. */

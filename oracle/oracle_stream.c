@@ -161,19 +161,48 @@ int fo_config_load(fo_config *c, const uint8_t *in, size_t avail, size_t *hdr_le
     return 0;
 }
 
-/* data_range (Statistic.hpp:160-204): exact min and max, difference formed in the element type */
+/* data_range (Statistic.hpp:160-204) with openmp off. Fields of 16 values or fewer, and the scalar
+ * build, take scalar_minmax (:38-44): std::max / std::min from d[0], so a NaN in d[0] stays and any
+ * other NaN is passed over. The AVX2 build takes avx2_minmax (:97-157): L = 32 / sizeof(T) lanes
+ * start at d[0]; each full vector updates lane k by MAXPS / MINPS, which return their second
+ * operand (the element) unless the first compares greater / less, so a NaN element enters the lane
+ * and the next element of that lane replaces it, and with it everything the lane held before; the
+ * lanes are then folded into lane 0 with std::max / std::min (a NaN in lane 0 stays, NaNs in the
+ * other lanes are passed over) and the tail after the last full vector is scanned with
+ * std::max / std::min. The difference is formed in the element type. */
+#define FO_MAX(a, b) ((a) < (b) ? (b) : (a)) /* std::max(a, b) */
+#define FO_MIN(a, b) ((b) < (a) ? (b) : (a)) /* std::min(a, b) */
+#define FO_MAXPS(a, b) ((a) > (b) ? (a) : (b))
+#define FO_MINPS(a, b) ((a) < (b) ? (a) : (b))
+#define FO_RANGE_BODY(T, L)                                                          \
+    const T *d = (const T *)data;                                                    \
+    T mn = d[0], mx = d[0];                                                          \
+    size_t i = 0;                                                                    \
+    if (fo_get_simd() && n > 16) {                                                   \
+        T vmx[L], vmn[L];                                                            \
+        for (int k = 0; k < L; k++) vmx[k] = vmn[k] = d[0];                          \
+        for (; i + L <= n; i += L)                                                   \
+            for (int k = 0; k < L; k++) {                                            \
+                vmx[k] = FO_MAXPS(vmx[k], d[i + k]);                                 \
+                vmn[k] = FO_MINPS(vmn[k], d[i + k]);                                 \
+            }                                                                        \
+        mx = vmx[0];                                                                 \
+        mn = vmn[0];                                                                 \
+        for (int k = 1; k < L; k++) { mx = FO_MAX(mx, vmx[k]); mn = FO_MIN(mn, vmn[k]); } \
+    } else {                                                                         \
+        i = 1;                                                                       \
+    }                                                                                \
+    for (; i < n; i++) { mx = FO_MAX(mx, d[i]); mn = FO_MIN(mn, d[i]); }
+
 double fo_data_range(int dtype, const void *data, size_t n) {
     if (dtype == 0) {
-        const float *d = (const float *)data;
-        float mn = d[0], mx = d[0];
-        for (size_t i = 1; i < n; i++) { if (d[i] < mn) mn = d[i]; if (d[i] > mx) mx = d[i]; }
+        FO_RANGE_BODY(float, 8)
         return (double)(float)(mx - mn);
     }
-    const double *d = (const double *)data;
-    double mn = d[0], mx = d[0];
-    for (size_t i = 1; i < n; i++) { if (d[i] < mn) mn = d[i]; if (d[i] > mx) mx = d[i]; }
+    FO_RANGE_BODY(double, 4)
     return mx - mn;
 }
+#undef FO_RANGE_BODY
 
 /* calAbsErrorBound (Statistic.hpp:216-240) */
 static int fo_abs_eb(fo_config *c, int dtype, const void *data) {
@@ -196,18 +225,21 @@ static int fo_abs_eb(fo_config *c, int dtype, const void *data) {
     return 0;
 }
 
-/* Lossless_zstd::compress (Lossless_zstd.hpp:29-46) */
+/* Lossless_zstd::compress (Lossless_zstd.hpp:29-46). A destination below 8 + ZSTD_compressBound(n)
+ * (or below 8 + n for the raw form) is the reference's std::length_error: returns FO_LENGTH_ERROR. */
+#define FO_LENGTH_ERROR ((size_t)-1)
 static size_t fo_lossless(const uint8_t *src, size_t n, uint8_t *dst, size_t cap) {
-    if (cap < 8) return 0;
-    memcpy(dst, &n, 8);
+    if (cap < 8) return FO_LENGTH_ERROR;
     if (getenv("FALCONS_NOZSTD")) {
-        if (cap - 8 < n) return 0;
+        if (cap - 8 < n) return FO_LENGTH_ERROR;
+        memcpy(dst, &n, 8);
         memcpy(dst + 8, src, n);
         return n + 8;
     }
     const fo_zstd_api *Z = fo_zstd();
     if (!Z) { set_err("cannot load libzstd"); return 0; }
-    if (cap - 8 < Z->compressBound(n)) return 0;
+    if (cap - 8 < Z->compressBound(n)) return FO_LENGTH_ERROR;
+    memcpy(dst, &n, 8);
     size_t r = Z->compress(dst + 8, cap - 8, src, n, 3);
     if (Z->isError(r)) { set_err("Zstd compression failed"); return 0; }
     return r + 8;
@@ -322,14 +354,18 @@ size_t fo_compress(const fo_config *conf_in, int dtype, void *data, uint8_t *out
         if (c.cmpr_algo != 2) { set_err("only the interpolation algorithms are on this path"); return 0; }
         plen = fo_interp_payload(&c, dtype, data, payload, pcap, pre, pre_cap, pre_len);
         if (plen == 0) return 0;
+        /* the lossless stage did not fit: the dispatcher stores the field, already overwritten with
+         * the reconstruction, losslessly instead (FalconsDispatcher.hpp:100-118) */
+        if (plen == FO_LENGTH_ERROR) c.cmpr_algo = 4;
     }
     if (c.cmpr_algo == 4) {
         plen = fo_lossless((const uint8_t *)data, c.num * es, payload, pcap);
+        if (plen == FO_LENGTH_ERROR) { set_err("The buffer for compressed data is not large enough."); return 0; }
     } else if ((double)(c.num * es) / 1.0 / (double)plen < 3) {
         size_t zc = fo_zstd()->compressBound(c.num * es) + 8;
         uint8_t *z = (uint8_t *)malloc(zc);
         size_t zl = fo_lossless((const uint8_t *)data, c.num * es, z, zc);
-        if (zl && zl < plen && zl <= pcap) {
+        if (zl && zl != FO_LENGTH_ERROR && zl < plen && zl <= pcap) {
             c.cmpr_algo = 4;
             memcpy(payload, z, zl);
             plen = zl;

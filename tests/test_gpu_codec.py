@@ -223,3 +223,50 @@ def test_o1_sequential_decoder(ctx, oracle, case, monkeypatch):
     data = make()
     monkeypatch.setenv("FALCONS_O1_SEQUENTIAL", "1")
     check(ctx, oracle, data, **kw)
+
+
+def nan_placed(dtype, where):
+    """A (41, 50, 61) field (tails of 2 after the last full AVX2 vector) with NaNs placed where the
+    AVX2 and scalar data_range differ (Statistic.hpp:38-157)."""
+    d = synth.sinusoids((41, 50, 61), seed=4, dtype=dtype)
+    flat = d.reshape(-1)
+    L = 8 if dtype == np.float32 else 4
+    nfull = flat.size // L * L
+    if where == "first":
+        flat[0] = np.nan
+    elif where == "last_lane0":
+        flat[nfull - L] = np.nan
+    elif where == "lane_history":
+        flat[3] = 40.0
+        flat[9 * L + 3] = np.nan
+    elif where == "last_other_lane":
+        flat[L + 2] = 40.0
+        flat[nfull - L + 2] = np.nan
+    elif where == "tail":
+        flat[-1] = np.nan
+    return d
+
+
+@pytest.mark.parametrize("where", ["first", "last_lane0", "lane_history", "last_other_lane", "tail"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("simd", [1, 0])
+def test_nan_rel_bound(ctx, oracle, where, dtype, simd):
+    """REL bound on fields with NaNs: the value range, and so every code, follows the build's
+    data_range (AVX2 lanes or scalar scan)."""
+    import falcons_amd
+    data = nan_placed(dtype, where)
+    shape = data.shape
+    c = falcons_amd.falconsConfig(*shape)
+    oc = oracle.config(shape, cmpr_algo=2, eb_mode=1, rel_eb=1e-3)
+    c.conf.cmpr_algo, c.conf.eb_mode, c.conf.rel_eb = 2, 1, 1e-3
+    print(f"case nan {where} {np.dtype(dtype).name} simd {simd}", flush=True)
+    oracle.L.fo_set_simd(simd)
+    try:
+        o_stream, o_mut, o_pre = oracle.compress(data.copy(), oc, want_pre=True)
+    finally:
+        oracle.L.fo_set_simd(1)
+    g = data.copy()
+    g_stream, _, g_pre = ctx.compress_ex(g, c, simd=simd, want_prezstd=True)
+    assert g_pre == o_pre
+    assert bytes(g_stream) == o_stream
+    assert np.array_equal(g.view(np.uint8), o_mut.view(np.uint8))
