@@ -87,6 +87,11 @@ const char *fo_last_error(void);
 int fo_tune_interp(fo_config *conf, int dtype, const void *data, int omp_threads);
 /* Ratios of the candidates the last fo_tune_interp evaluated, in evaluation order; returns their number. */
 int fo_tune_report(double *out, int cap);
+/* The parameter search's sampler alone: sample block side, profiling decision, the starts of the
+ * blocks profiling_block keeps (three slots each) and the starts of the selected blocks. Returns the
+ * number of selected blocks, or -1 when the field is not tuned. */
+long fo_tune_sample(const fo_config *conf, int dtype, const void *data, size_t *sbs, int *profiling, size_t *starts,
+                    size_t starts_cap, size_t *nstarts, size_t *sel, size_t sel_cap);
 
 #ifdef __cplusplus
 }

@@ -233,13 +233,14 @@ int fo_tune_report(double *out, int cap) {
 
 #define VAL(dtype, data, idx) ((dtype) ? ((const double *)(data))[idx] : (double)((const float *)(data))[idx])
 
-int fo_tune_interp(fo_config *c, int dtype, const void *data, int omp_threads) {
+/* Sample block side, profiling_block (Sample.hpp:5-159) and sampleBlocks (Sample.hpp:233-344) as
+ * Falcons_compress_Interp_lorenzo calls them (FalconsAlgoInterp.hpp:286-324). Fills *sbs_out,
+ * *per_out, *prof_out, the profiled starts (*starts_out, *nstart_out, three slots per start, the
+ * field's own axis order first) and the selected block starts (*sel_out); returns the number of
+ * selected blocks, or (size_t)-1 when the field is not tuned. The caller frees both arrays. */
+static size_t tn_sample(const fo_config *c, int dtype, const void *data, size_t *sbs_out, size_t *per_out,
+                        int *prof_out, size_t (**starts_out)[3], size_t *nstart_out, size_t (**sel_out)[3]) {
     int N = c->N;
-    g_tune_n = 0;
-    if (c->interp_anchor_stride < 0) {
-        static const int def[4] = {4096, 128, 32, 16};
-        c->interp_anchor_stride = def[N - 1];
-    }
     double rate = 0.005;
     static const size_t sbs0[4] = {4096, 128, 32, 16};
     size_t sbs = sbs0[N - 1];
@@ -250,7 +251,7 @@ int fo_tune_interp(fo_config *c, int dtype, const void *data, int omp_threads) {
     if (sbs < 8) sbs = 8;
     int to_tune = pow((double)(sbs + 1), N) <= 0.05 * (double)c->num;
     for (int i = 0; i < N; i++) if (c->dims[i] < sbs) to_tune = 0;
-    if (!to_tune) return 0;
+    if (!to_tune) return (size_t)-1;
     size_t per = (size_t)pow((double)(sbs + 1), N);
     size_t es = dtype ? 8 : 4;
     size_t D[3] = {1, 1, 1};
@@ -326,7 +327,46 @@ int fo_tune_interp(fo_config *c, int dtype, const void *data, int omp_threads) {
                     idx++;
                 }
     }
+    *sbs_out = sbs;
+    *per_out = per;
+    *prof_out = profiling;
+    *starts_out = starts;
+    *nstart_out = nstart;
+    *sel_out = sel;
+    (void)es;
+    return nsel;
+}
+
+long fo_tune_sample(const fo_config *c, int dtype, const void *data, size_t *sbs, int *profiling, size_t *starts,
+                    size_t starts_cap, size_t *nstarts, size_t *sel, size_t sel_cap) {
+    size_t per = 0, nst = 0;
+    size_t(*st)[3] = NULL, (*sl)[3] = NULL;
+    size_t n = tn_sample(c, dtype, data, sbs, &per, profiling, &st, &nst, &sl);
+    if (n == (size_t)-1) return -1;
+    *nstarts = nst;
+    for (size_t i = 0; i < nst && i < starts_cap; i++) memcpy(starts + 3 * i, st[i], 3 * sizeof(size_t));
+    for (size_t i = 0; i < n && i < sel_cap; i++) memcpy(sel + 3 * i, sl[i], 3 * sizeof(size_t));
+    free(st);
+    free(sl);
+    return (long)n;
+}
+
+int fo_tune_interp(fo_config *c, int dtype, const void *data, int omp_threads) {
+    int N = c->N;
+    g_tune_n = 0;
+    if (c->interp_anchor_stride < 0) {
+        static const int def[4] = {4096, 128, 32, 16};
+        c->interp_anchor_stride = def[N - 1];
+    }
+    size_t sbs, per, nstart;
+    int profiling;
+    size_t(*starts)[3], (*sel)[3];
+    size_t nsel = tn_sample(c, dtype, data, &sbs, &per, &profiling, &starts, &nstart, &sel);
+    if (nsel == (size_t)-1) return 0;
     free(starts);
+    size_t es = dtype ? 8 : 4;
+    size_t D[3] = {1, 1, 1};
+    for (int i = 0; i < N; i++) D[3 - N + i] = c->dims[i];
     if (nsel * per == 0 || (double)(nsel * per) >= (double)c->num * 0.2) { free(sel); return 0; }
     void **blocks = (void **)malloc(nsel * sizeof(void *));
     size_t side = sbs + 1;

@@ -1,8 +1,11 @@
 // Checker-side driver for the parts of the reference that compile from their own sources:
 // the RLE/FSE + order-1 tANS entropy coder (include/Falcons/encoder/RleFseEncoder.hpp with the
-// vendored zfse/*.c), the scalar LinearQuantizer and the scalar interpolators.
+// vendored zfse/*.c), the scalar LinearQuantizer, the scalar interpolators and the parameter
+// search's block sampler (include/Falcons/utils/Sample.hpp, which needs only Falcons/def.hpp).
 // It only instantiates the reference templates and exports them with C linkage so the tests can
 // call them through ctypes. It is test infrastructure: nothing in the product links it.
+#include <cassert>
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -10,6 +13,40 @@
 #include "Falcons/encoder/RleFseEncoder.hpp"
 #include "Falcons/quantizer/LinearQuantizer.hpp"
 #include "Falcons/utils/Interpolators.hpp"
+#include "Falcons/def.hpp"
+#include "Falcons/utils/Sample.hpp"
+
+namespace {
+// profiling_block then sampleBlocks, as Falcons_compress_Interp_lorenzo calls them
+// (FalconsAlgoInterp.hpp:318-324), for one element type and rank.
+template <class T, Falcons::uint N>
+long ref_sample_t(const size_t *dims_in, const void *data_in, size_t sbs, double abseb, double rate, int profiling,
+                  size_t *starts_out, size_t starts_cap, size_t *nstarts, void *blocks_out, size_t blocks_cap) {
+    std::vector<size_t> dims(dims_in, dims_in + N);
+    size_t num = 1;
+    for (size_t d : dims) num *= d;
+    std::vector<T> data(static_cast<const T *>(data_in), static_cast<const T *>(data_in) + num);
+    std::vector<std::vector<size_t>> starts;
+    Falcons::profiling_block<T, N>(data.data(), dims, starts, sbs, abseb, sbs / 4, false);
+    *nstarts = starts.size();
+    for (size_t i = 0; i < starts.size() && i < starts_cap; i++)
+        for (size_t a = 0; a < 3; a++) starts_out[3 * i + a] = a < starts[i].size() ? starts[i][a] : 0;
+    if (profiling < 0) {  // the decision of FalconsAlgoInterp.hpp:321
+        size_t per = 1;
+        for (Falcons::uint i = 0; i < N; i++) per *= sbs + 1;
+        profiling = starts.size() * per >= 0.5 * rate * num;
+    }
+    std::vector<std::vector<T>> blocks;
+    Falcons::sampleBlocks<T, N>(data.data(), dims, sbs, blocks, rate, profiling, starts, 0, false);
+    size_t o = 0;
+    for (auto &b : blocks) {
+        if (o + b.size() > blocks_cap) return -2;
+        memcpy(static_cast<T *>(blocks_out) + o, b.data(), b.size() * sizeof(T));
+        o += b.size();
+    }
+    return static_cast<long>(blocks.size()) * 2 + profiling;  // blocks, and the profiling branch in bit 0
+}
+}  // namespace
 
 extern "C" {
 
@@ -86,6 +123,25 @@ double ref_interp_f64(int kind, double a, double b, double c, double d) {
         case 3: return Falcons::interp_quad_2(a, b, c);
         default: return Falcons::interp_cubic(a, b, c, d);
     }
+}
+
+// Block sampler of the parameter search (Sample.hpp:5-344). dtype 0 float, 1 double; N 1..3.
+// profiling < 0 takes the reference's own decision. Returns 2 * blocks + profiling branch, or < 0.
+long ref_sample(int dtype, int N, const size_t *dims, const void *data, size_t sbs, double abseb, double rate,
+                int profiling, size_t *starts_out, size_t starts_cap, size_t *nstarts, void *blocks_out,
+                size_t blocks_cap) {
+#define RS_CASE(T, R) return ref_sample_t<T, R>(dims, data, sbs, abseb, rate, profiling, starts_out, starts_cap, nstarts, blocks_out, blocks_cap)
+    if (dtype == 0) {
+        if (N == 1) RS_CASE(float, 1);
+        if (N == 2) RS_CASE(float, 2);
+        if (N == 3) RS_CASE(float, 3);
+    } else {
+        if (N == 1) RS_CASE(double, 1);
+        if (N == 2) RS_CASE(double, 2);
+        if (N == 3) RS_CASE(double, 3);
+    }
+#undef RS_CASE
+    return -1;
 }
 
 // Vendored FSE entry points, re-exported under stable names.

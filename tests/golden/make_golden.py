@@ -12,8 +12,12 @@ that build on seeded inputs and stores inputs and outputs:
                  sha256 of the encoded bytes (regenerated bit-exactly by codes_from_recipe)
   quantizer.npz  LinearQuantizer::quantize_and_overwrite and recover (f32 and f64)
   interp.npz     interp_linear / linear1 / quad_1 / quad_2 / cubic (f32 and f64)
+  sampler.json   profiling_block + sampleBlocks (utils/Sample.hpp:5-344) of the ALGO_INTERP_LORENZO
+                 parameter search on integer-generated fields (recipes in golden_inputs.sampler_field):
+                 the profiling branch, the profiled block starts and the sha256 of the sampled
+                 blocks, at the sample block side the search picks for each field
 
-Run from the repository root: python tests/golden/make_golden.py
+Run from the repository root: python tests/golden/make_golden.py [--only sampler]
 """
 import hashlib
 import json
@@ -24,7 +28,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
-from golden_inputs import codes_from_recipe  # noqa: E402
+from golden_inputs import codes_from_recipe, sampler_field  # noqa: E402
 from oracle_lib import Ref  # noqa: E402
 
 BIG_RECIPES = [
@@ -52,6 +56,40 @@ SMALL_RECIPES = [
     {"kind": "escapes", "n": 8192, "seed": 13, "scale": 9},
     {"kind": "wide", "n": 90_000, "seed": 14, "scale": 0},
 ]
+
+
+# Sampler recipes: sbs is the sample block side Falcons_compress_Interp_lorenzo picks for the shape
+# (FalconsAlgoInterp.hpp:297-305), eb the absolute bound profiling_block compares ranges with.
+SAMPLER_RECIPES = [
+    {"kind": "walk", "shape": [160, 192, 160], "dtype": "f32", "seed": 61, "sbs": 32, "eb": 0.5},
+    {"kind": "patch", "shape": [136, 160, 240], "dtype": "f64", "seed": 62, "sbs": 32, "eb": 1e-6,
+     "box": [[20, 90], [0, 70], [100, 240]]},
+    {"kind": "flat", "shape": [96, 100, 110], "dtype": "f32", "seed": 63, "sbs": 16, "eb": 1e-4},
+    {"kind": "patch", "shape": [96, 100, 110], "dtype": "f32", "seed": 64, "sbs": 16, "eb": 1e-3,
+     "box": [[40, 50], [40, 50], [40, 50]]},
+    {"kind": "walk", "shape": [40, 44, 48], "dtype": "f64", "seed": 65, "sbs": 8, "eb": 0.25},
+    {"kind": "walk", "shape": [1000, 1200], "dtype": "f32", "seed": 66, "sbs": 64, "eb": 2.0},
+    {"kind": "patch", "shape": [700, 900], "dtype": "f64", "seed": 67, "sbs": 64, "eb": 1e-3,
+     "box": [[100, 400], [300, 500]]},
+    {"kind": "flat", "shape": [900, 800], "dtype": "f64", "seed": 68, "sbs": 64, "eb": 1e-3},
+    {"kind": "walk", "shape": [2000000], "dtype": "f32", "seed": 69, "sbs": 4096, "eb": 0.5},
+    {"kind": "patch", "shape": [4000000], "dtype": "f64", "seed": 70, "sbs": 4096, "eb": 1e-3,
+     "box": [[500000, 502000]]},
+    # the 256 x 384 x 384 workload's shape: 847 candidate blocks of 33^3
+    {"kind": "patch", "shape": [256, 384, 384], "dtype": "f32", "seed": 71, "sbs": 32, "eb": 1e-3,
+     "box": [[0, 256], [0, 384], [100, 384]]},
+]
+
+
+def sampler_vectors(ref):
+    res = []
+    for r in SAMPLER_RECIPES:
+        f = sampler_field(r)
+        prof, starts, blocks = ref.sample(f, r["sbs"], r["eb"])
+        res.append(dict(r, profiling=int(prof), starts=starts[:, : f.ndim].tolist(),
+                        nblocks=int(blocks.size // (r["sbs"] + 1) ** f.ndim),
+                        blocks_sha256=hashlib.sha256(blocks.tobytes()).hexdigest()))
+    return res
 
 
 def fse_blocks():
@@ -106,6 +144,13 @@ def interp_args(dtype, count=600):
 def main():
     ref = Ref()
     out = {}
+    samp = sampler_vectors(ref)
+    with open(os.path.join(HERE, "sampler.json"), "w") as f:
+        json.dump(samp, f, indent=1)
+    out["sampler"] = [(s["nblocks"], s["profiling"], len(s["starts"])) for s in samp]
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "sampler":
+        print(json.dumps(out))
+        return
     # FSE
     blocks = fse_blocks()
     src = np.concatenate(blocks)

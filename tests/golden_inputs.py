@@ -55,3 +55,29 @@ def codes_from_recipe(r):
     else:
         raise ValueError(kind)
     return np.clip(v, -32768, 32767).astype(np.int16)
+
+
+def sampler_field(r):
+    """Fields for the parameter search's block sampler, from integer arithmetic scaled by powers of
+    two (exact in float32 and float64):
+      walk   an integer random walk along the flattened field: every block varies
+      patch  zero except a box where a walk runs: profiling keeps only the blocks that touch it
+      flat   a constant plus steps of 2^-20 (below any bound used): no block is kept
+    """
+    kind, shape, seed = r["kind"], tuple(r["shape"]), int(r["seed"])
+    dt = np.float32 if r["dtype"] == "f32" else np.float64
+    n = int(np.prod(shape))
+    h = splitmix64(seed, n)
+    if kind == "flat":
+        v = 3.0 + (h % np.uint64(3)).astype(np.float64) * 2.0 ** -20
+        return v.astype(dt).reshape(shape)
+    step = (h % np.uint64(9)).astype(np.int64) - 4
+    walk = (np.cumsum(step) % 4096).astype(np.float64) * 2.0 ** -6
+    if kind == "walk":
+        return walk.astype(dt).reshape(shape)
+    if kind == "patch":
+        f = np.zeros(shape, np.float64)
+        box = tuple(slice(int(a), int(b)) for a, b in r["box"])
+        f[box] = walk.reshape(shape)[box]
+        return f.astype(dt)
+    raise ValueError(kind)

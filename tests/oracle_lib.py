@@ -87,6 +87,10 @@ class Oracle:
         L.fo_tune_interp.argtypes = [C.POINTER(FoConfig), C.c_int, C.c_void_p, C.c_int]
         L.fo_tune_report.restype = C.c_int
         L.fo_tune_report.argtypes = [C.POINTER(C.c_double), C.c_int]
+        L.fo_tune_sample.restype = C.c_long
+        L.fo_tune_sample.argtypes = [C.POINTER(FoConfig), C.c_int, C.c_void_p, C.POINTER(C.c_size_t),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_size_t), C.c_size_t,
+                                     C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.c_size_t]
         L.fo_o1_histogram.argtypes = [_i16p, C.c_size_t, C.POINTER(C.c_uint32), _i16p, _szp]
         L.fo_o1_ctx.restype = C.c_int
         L.fo_o1_sym.restype = C.c_int
@@ -177,6 +181,23 @@ class Oracle:
             setattr(c, k, v)
         return c
 
+    def tune_sample(self, data, conf):
+        """The parameter search's sampler: (block side, profiling branch, profiled starts, selected
+        starts), starts as (count, 3) arrays; None when the field is not tuned."""
+        data = np.ascontiguousarray(data)
+        dtype = 0 if data.dtype == np.float32 else 1
+        cap = data.size // 8 + 16
+        st = np.zeros((cap, 3), dtype=np.uint64)
+        sel = np.zeros((cap, 3), dtype=np.uint64)
+        sbs, prof, nst = C.c_size_t(0), C.c_int(0), C.c_size_t(0)
+        sp = C.POINTER(C.c_size_t)
+        n = self.L.fo_tune_sample(C.byref(conf), dtype, data.ctypes.data, C.byref(sbs), C.byref(prof),
+                                  st.ctypes.data_as(sp), cap, C.byref(nst), sel.ctypes.data_as(sp), cap)
+        if n < 0:
+            return None
+        assert nst.value <= cap and n <= cap
+        return sbs.value, prof.value, st[: nst.value].copy(), sel[:n].copy()
+
     def compress(self, data, conf, want_pre=False):
         data = np.ascontiguousarray(data).copy()
         dtype = 0 if data.dtype == np.float32 else 1
@@ -242,6 +263,27 @@ class Ref:
         L.ref_interp_f32.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float]
         L.ref_interp_f64.restype = C.c_double
         L.ref_interp_f64.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.ref_sample.restype = C.c_long
+        L.ref_sample.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_size_t), C.c_void_p, C.c_size_t, C.c_double,
+                                 C.c_double, C.c_int, C.POINTER(C.c_size_t), C.c_size_t, C.POINTER(C.c_size_t),
+                                 C.c_void_p, C.c_size_t]
+
+    def sample(self, data, sbs, abseb, rate=0.005, profiling=-1):
+        """profiling_block + sampleBlocks of the reference (Sample.hpp): (profiling branch, profiled
+        starts as a (count, 3) array, the sampled blocks concatenated)."""
+        data = np.ascontiguousarray(data)
+        dtype = 0 if data.dtype == np.float32 else 1
+        dims = (C.c_size_t * 3)(*data.shape)
+        cap = data.size // 8 + 16
+        st = np.zeros((cap, 3), dtype=np.uint64)
+        blocks = np.zeros(data.size // 4 + (sbs + 1) ** data.ndim, dtype=data.dtype)
+        nst = C.c_size_t(0)
+        sp = C.POINTER(C.c_size_t)
+        r = self.L.ref_sample(dtype, data.ndim, dims, data.ctypes.data, sbs, abseb, rate, profiling,
+                              st.ctypes.data_as(sp), cap, C.byref(nst), blocks.ctypes.data, blocks.size)
+        assert r >= 0 and nst.value <= cap, r
+        nb = r >> 1
+        return r & 1, st[: nst.value].copy(), blocks[: nb * (sbs + 1) ** data.ndim].copy()
 
     def rle_encode(self, codes):
         codes = np.ascontiguousarray(codes, dtype=np.int16)
