@@ -1,6 +1,14 @@
 """The falcons CLI (tools/falcons/falcons.cpp, unchanged) built against the API front header and
 linked to libfalcons_mi355x.so: its compressed files and decompressed fields on the GPU against the
-oracle, byte for byte. The binary is built by `make -C oracle shim` where the reference tree is."""
+oracle. The binary is built by `make -C oracle shim` where the reference tree is.
+
+Compared byte for byte: the stream's Config header, the buffer inside its zstd frame (the pre-zstd
+buffer), the CLI's decompressed field, and the CLI's decompression of the oracle's stream. The zstd
+frame itself is not: like the reference's own CLI (its CMake links the vendored zfse objects into
+the executable beside a shared libzstd), this process binds libzstd's calls of its exported FSE_*
+functions to the vendored ones, so libzstd 1.4.9 frames the same buffer a few bytes differently
+from a process with libzstd alone (the oracle's and the library's)."""
+import ctypes as C
 import os
 import subprocess
 
@@ -38,8 +46,30 @@ def test_cli_matches_oracle(case, tmp_path):
     orc = Oracle()
     orc.L.fo_set_simd(1)
     oc = orc.config(shape, cmpr_algo=1, eb_mode=0 if mode == "ABS" else 1, abs_eb=eb, rel_eb=eb)
-    o_stream, _ = orc.compress(data, oc)
-    assert z.read_bytes() == o_stream, "the CLI's compressed file differs from the oracle's stream"
+    o_stream, _, o_pre = orc.compress(data, oc, want_pre=True)
+    cli_stream = z.read_bytes()
+    h = o_stream[0]
+    assert cli_stream[:h] == o_stream[:h], "Config header differs from the oracle's"
+    assert _unzstd(cli_stream[h:]) == o_pre, "the buffer inside the CLI's zstd frame differs from the oracle's"
     o_dec, _ = orc.decompress(o_stream, dt)
     dec = np.fromfile(o, dtype=dt)
     assert np.array_equal(dec.view(np.uint8), o_dec.reshape(-1).view(np.uint8))
+    # the CLI decompresses the oracle's stream to the oracle's field
+    zo, oo = tmp_path / "oracle.falcons", tmp_path / "oracle.out"
+    zo.write_bytes(o_stream)
+    r = subprocess.run([CLI, tflag, "-z", str(zo), "-o", str(oo), f"-{len(shape)}", *dims], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert np.array_equal(np.fromfile(oo, dtype=dt).view(np.uint8), o_dec.reshape(-1).view(np.uint8))
+
+
+def _unzstd(payload):
+    """Lossless_zstd's framing: [size_t n][zstd frame of n bytes]."""
+    z = C.CDLL("/opt/conda/lib/libzstd.so.1")
+    z.ZSTD_decompress.restype = C.c_size_t
+    z.ZSTD_isError.restype = C.c_uint
+    n = int.from_bytes(payload[:8], "little")
+    out = C.create_string_buffer(max(n, 1))
+    r = z.ZSTD_decompress(out, n, payload[8:], len(payload) - 8)
+    assert not z.ZSTD_isError(r) and r == n
+    return out.raw[:n]
