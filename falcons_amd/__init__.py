@@ -97,6 +97,8 @@ _L.fz_prezstd_bound.restype = C.c_size_t
 _L.fz_compress_device.argtypes = [C.c_void_p, C.POINTER(fz_config), C.c_int, C.c_void_p, C.c_void_p, C.c_size_t,
                                   C.POINTER(C.c_size_t), C.POINTER(fz_opts)]
 _L.fz_compress_device.restype = C.c_int
+_L.fz_compress_device_noalloc.argtypes = _L.fz_compress_device.argtypes
+_L.fz_compress_device_noalloc.restype = C.c_int
 _L.fz_decompress_device.argtypes = [C.c_void_p, C.POINTER(fz_config), C.c_int, _u8p, C.c_void_p, C.c_size_t,
                                     C.c_void_p, C.c_size_t, C.POINTER(fz_opts)]
 _L.fz_decompress_device.restype = C.c_int
@@ -174,6 +176,7 @@ EXPORTED_SYMBOLS = ("fz_ctx_create", "fz_ctx_destroy", "fz_last_error", "fz_opts
                     "fz_config_size_est", "fz_config_save", "fz_read_config", "fz_config_set_dims",
                     "fz_config_load", "fz_compress_size_bound",
                     "fz_compress", "fz_decompress", "fz_prezstd_bound", "fz_compress_device",
+                    "fz_compress_device_noalloc",
                     "fz_decompress_device", "fz_device_alloc", "fz_device_free", "fz_memcpy", "fz_synchronize",
                     "fz_ctx_stream",
                     "fz_zstd_version", "fz_profile_enable", "fz_profile_reset", "fz_profile_count",
@@ -621,6 +624,20 @@ class Context:
         n = C.c_size_t(0)
         rc = _L.fz_compress_device(self.h, C.byref(config.conf), dt, d_data.ptr, d_pre.ptr, d_pre.nbytes, C.byref(n),
                                    C.byref(opts))
+        if rc:
+            _raise(rc, self._err())
+        return n.value, opts
+
+    def compress_device_noalloc(self, config, dtype, d_data, d_pre, simd=1):
+        """fz_compress_device_noalloc: as compress_device, with the workspace in the library's own
+        device array instead of allocated memory (fields up to 256 MiB of workspace)."""
+        dt = _DT[np.dtype(dtype)]
+        opts = fz_opts()
+        _L.fz_opts_init(C.byref(opts))
+        opts.simd = int(simd)
+        n = C.c_size_t(0)
+        rc = _L.fz_compress_device_noalloc(self.h, C.byref(config.conf), dt, d_data.ptr, d_pre.ptr, d_pre.nbytes,
+                                           C.byref(n), C.byref(opts))
         if rc:
             _raise(rc, self._err())
         return n.value, opts

@@ -486,12 +486,11 @@ struct ArenaReport {
 // The compression pipeline on the GPU. d_data (device) holds the field and is overwritten with the
 // reconstructed values. The buffer handed to the zstd stage is assembled in device memory, at d_pre
 // when given (capacity pre_cap) or in the context workspace; *pre_out receives its address.
-size_t gpu_compress(fz_ctx* ctx, fz_config& c, int dtype, void* d_data, int simd,
+size_t gpu_compress(fz_ctx* ctx, DevArena& ws, fz_config& c, int dtype, void* d_data, int simd,
                     uint8_t* d_pre, size_t pre_cap, uint8_t** pre_out, fz_opts* o) {
     hipStream_t st = ctx->stream;
     const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
     const uint64_t n = c.num;
-    DevArena& ws = ctx->arena;
     ws.reset();
     ws.reserve(compress_workspace_bytes(n, dtype));
     ws.clear_peak();
@@ -755,6 +754,7 @@ extern "C" int fz_ctx_create(int device, fz_ctx** out) {
         FZ_HIP(hipStreamCreateWithFlags(&c->aux.stream, hipStreamNonBlocking));
         FZ_HIP(hipEventCreateWithFlags(&c->aux.fork, hipEventDisableTiming));
         FZ_HIP(hipEventCreateWithFlags(&c->aux.join, hipEventDisableTiming));
+        noalloc_setup(device);
         *out = c;
         return FZ_OK;
     } catch (const std::exception& e) {
@@ -782,6 +782,17 @@ extern "C" void fz_ctx_destroy(fz_ctx* c) {
 extern "C" const char* fz_last_error(const fz_ctx* c) { return c ? c->err.c_str() : ""; }
 
 namespace fz {
+
+void check_device_compress(const fz_config& c, int dtype) {
+    check_supported(c, dtype);
+    if (c.cmpr_algo == 4 || (c.eb_mode == 0 && c.abs_eb == 0))
+        throw Unsupported("the lossless mode has no device-resident form");
+}
+
+size_t compress_to_device(fz_ctx* ctx, DevArena& ws, fz_config& c, int dtype, void* d_data, uint8_t* d_pre,
+                          size_t pre_cap, fz_opts* o) {
+    return gpu_compress(ctx, ws, c, dtype, d_data, o->simd, d_pre, pre_cap, nullptr, o);
+}
 
 // Falcons_compress (falcons.hpp:54-97, FalconsDispatcher.hpp:25-133) on a field that is already in
 // device memory. d_data is overwritten with the reconstructed values. *mutated tells whether the
@@ -812,7 +823,7 @@ size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_
             std::vector<uint8_t> pre;
             try {
                 uint8_t* d_pre = nullptr;
-                const size_t plen_pre = gpu_compress(ctx, c, dtype, d_data, o->simd, nullptr, 0, &d_pre, o);
+                const size_t plen_pre = gpu_compress(ctx, ctx->arena, c, dtype, d_data, o->simd, nullptr, 0, &d_pre, o);
                 pre.resize(plen_pre);
                 FZ_HIP(hipMemcpyAsync(pre.data(), d_pre, plen_pre, hipMemcpyDeviceToHost, ctx->stream));
                 FZ_HIP(hipStreamSynchronize(ctx->stream));
@@ -977,10 +988,8 @@ extern "C" int fz_compress_device(fz_ctx* ctx, fz_config* conf, int dtype, void*
         fz_opts* o = opts ? opts : &local;
         FZ_HIP(hipSetDevice(ctx->device));
         fz_config c = *conf;
-        check_supported(c, dtype);
-        if (c.cmpr_algo == 4 || (c.eb_mode == 0 && c.abs_eb == 0))
-            throw Unsupported("the lossless mode has no device-resident form");
-        *pre_len = gpu_compress(ctx, c, dtype, d_data, o->simd, d_pre, pre_cap, nullptr, o);
+        check_device_compress(c, dtype);
+        *pre_len = gpu_compress(ctx, ctx->arena, c, dtype, d_data, o->simd, d_pre, pre_cap, nullptr, o);
         *conf = c;
         o->t_total_ms = now_ms() - t0;
         return FZ_OK;

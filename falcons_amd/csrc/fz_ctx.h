@@ -54,6 +54,15 @@ size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_
 void decompress_to_device(fz_ctx* ctx, const uint8_t* cmp, size_t len, int dtype, void** d_out_io,
                           size_t out_cap_elems, fz_config* conf_out, fz_opts* o);
 
+// The configuration checks of fz_compress_device (types, rank, stream format, lossless modes).
+void check_device_compress(const fz_config& c, int dtype);
+// fz_compress_device's pipeline drawing its workspace from ws.
+size_t compress_to_device(fz_ctx* ctx, DevArena& ws, fz_config& c, int dtype, void* d_data, uint8_t* d_pre,
+                          size_t pre_cap, fz_opts* o);
+// Looks up, once per device, the address of the allocation-free entry point's scratch array on the
+// current device (fz_noalloc.hip); fz_ctx_create calls it with its device current.
+void noalloc_setup(int device);
+
 inline void set_err(fz_ctx* ctx, const char* m) {
     if (ctx) ctx->err = m;
 }

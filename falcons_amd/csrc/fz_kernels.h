@@ -4,6 +4,8 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "fz_common.h"
@@ -15,9 +17,18 @@ namespace fz {
 // full, so an estimate below the true need costs an allocation but never a failure, and a grown
 // arena is not freed (hipFree synchronizes the whole device). Positions (used(), rewind(), peak())
 // count through the blocks in order.
+//
+// A bound arena (bind()) draws from one region of device memory the caller owns and never
+// allocates: a take() beyond the region throws instead of adding a block.
 class DevArena {
    public:
     ~DevArena() { release(); }
+    void bind(void* p, size_t bytes) {
+        release();
+        blocks_.push_back(Block{p, bytes & ~size_t(255)});
+        cap_ = bytes & ~size_t(255);
+        bound_ = true;
+    }
     void reserve(size_t bytes) {
         bytes = (bytes + 255) & ~size_t(255);
         if (bytes <= cap_) return;
@@ -66,10 +77,12 @@ class DevArena {
     }
     size_t capacity() const { return cap_; }
     void release() {
-        for (Block& b : blocks_) (void)hipFree(b.p);
+        if (!bound_)
+            for (Block& b : blocks_) (void)hipFree(b.p);
         blocks_.clear();
         cap_ = used_ = off_ = 0;
         cur_ = 0;
+        bound_ = false;
     }
 
    private:
@@ -79,6 +92,9 @@ class DevArena {
     };
     void add_block(size_t bytes) {
         bytes = (bytes + 255) & ~size_t(255);
+        if (bound_)
+            throw std::length_error("the call needs more device workspace than the bound region holds (" +
+                                    std::to_string(cap_) + " bytes)");
         void* p = nullptr;
         FZ_HIP(hipMalloc(&p, bytes));
         blocks_.push_back(Block{p, bytes});
@@ -86,6 +102,7 @@ class DevArena {
     }
     std::vector<Block> blocks_;
     size_t cap_ = 0, used_ = 0, off_ = 0, peak_ = 0, cur_ = 0;
+    bool bound_ = false;
 };
 
 // interpolation kernels (fz_interp.hip)

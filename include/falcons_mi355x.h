@@ -118,6 +118,19 @@ size_t fz_prezstd_bound(const fz_config *c, int dtype);
 int fz_compress_device(fz_ctx *ctx, fz_config *conf, int dtype, void *d_data, uint8_t *d_pre, size_t pre_cap,
                        size_t *pre_len, fz_opts *opts);
 
+/* fz_compress_device without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under stream capture, tight serving loops, hosts
+ * that forbid hipMalloc). Same arguments, checks and output as fz_compress_device, but the
+ * pipeline's workspace is a 256 MiB array in the library's device image (one copy per GPU, held
+ * until the process ends) instead of the context's arena, and nothing calls hipMalloc, hipFree or
+ * an allocating library routine. Limits: fields up to 5,592,405 float or 4,194,304 double elements
+ * (FZ_ERR_INVALID_ARGUMENT beyond, before any GPU work); ALGO_INTERP only (ALGO_INTERP_LORENZO,
+ * whose candidate scoring calls libzstd, returns FZ_ERR_UNSUPPORTED). Every call on one GPU shares
+ * that GPU's array, which keeps the previous call's contents: calls in one process serialize on a
+ * lock and return with their work finished; calls that overlap on one GPU otherwise would race. */
+int fz_compress_device_noalloc(fz_ctx *ctx, fz_config *conf, int dtype, void *d_data, uint8_t *d_pre, size_t pre_cap,
+                               size_t *pre_len, fz_opts *opts);
+
 /* FalconsGenericCompressor::decompress after the lossless stage (FalconsGenericCompressor.hpp:
  * 109-124): h_pre is the host copy of the pre-zstd buffer (headers are parsed from it), d_pre its
  * device copy (may be NULL: then the payload is uploaded first). conf is the stream's configuration
