@@ -331,7 +331,11 @@ def read_config(stream):
     return cfg
 
 
-_DT = {np.dtype(np.float32): 0, np.dtype(np.float64): 1}
+# element types, numbered as Falcons_FLOAT .. Falcons_INT64 (Config.hpp:29-38)
+_DT = {np.dtype(np.float32): 0, np.dtype(np.float64): 1, np.dtype(np.uint8): 2, np.dtype(np.int8): 3,
+       np.dtype(np.uint16): 4, np.dtype(np.int16): 5, np.dtype(np.uint32): 6, np.dtype(np.int32): 7,
+       np.dtype(np.uint64): 8, np.dtype(np.int64): 9}
+_DT_OF = {v: k for k, v in _DT.items()}
 
 
 def _is_device_tensor(x):
@@ -339,7 +343,7 @@ def _is_device_tensor(x):
 
 
 _DL_CPU, _DL_CUDA, _DL_ROCM = 1, 2, 10  # DLDeviceType (dlpack.h)
-_DL_FLOAT = 2  # DLDataTypeCode kDLFloat
+_DL_INT, _DL_UINT, _DL_FLOAT = 0, 1, 2  # DLDataTypeCode kDLInt, kDLUInt, kDLFloat
 
 
 class _DLTensor(C.Structure):  # DLTensor (dlpack.h)
@@ -382,10 +386,11 @@ class DLDeviceArray:
         self.device_type, self.device = int(t.device_type), int(t.device_id)
         self.shape = tuple(int(t.shape[i]) for i in range(t.ndim))
         self.size = int(np.prod(self.shape)) if self.shape else 1
-        if t.code != _DL_FLOAT or t.bits not in (32, 64) or t.lanes != 1:
+        kinds = {_DL_FLOAT: "f", _DL_INT: "i", _DL_UINT: "u"}
+        if t.code not in kinds or t.lanes != 1 or np.dtype(f"{kinds[t.code]}{t.bits // 8}") not in _DT:
             self.release()
-            raise TypeError("Unsupported dtype: only float32 and float64 arrays are compressed")
-        self.dtype = np.dtype(np.float32 if t.bits == 32 else np.float64)
+            raise TypeError("Unsupported dtype: float32/64 and 8- to 64-bit integer arrays are compressed")
+        self.dtype = np.dtype(f"{kinds[t.code]}{t.bits // 8}")
         if t.strides:
             want, st = 1, []
             for d in reversed(self.shape):
@@ -478,7 +483,8 @@ class DeviceArray:
         t = mt.dl_tensor
         t.data = self.buf.ptr.value if isinstance(self.buf.ptr, C.c_void_p) else self.buf.ptr
         t.device_type, t.device_id, t.ndim = _DL_ROCM, int(self.buf.ctx.device), len(self.shape)
-        t.code, t.bits, t.lanes = _DL_FLOAT, 8 * self.dtype.itemsize, 1
+        t.code = {"f": _DL_FLOAT, "i": _DL_INT, "u": _DL_UINT}[self.dtype.kind]
+        t.bits, t.lanes = 8 * self.dtype.itemsize, 1
         t.shape, t.strides, t.byte_offset = C.cast(shp, C.POINTER(C.c_int64)), None, 0
         mt.deleter = C.cast(_DL_EXPORT_DELETER, C.c_void_p)
         _DL_EXPORTS[C.addressof(mt)] = (mt, shp, self.buf)
@@ -558,10 +564,10 @@ class Context:
         on_dev = dv is not None
         if on_dev:
             ptr, size, dt = dv
-            nbytes = size * (4 if dt == 0 else 8)
+            nbytes = size * _DT_OF[dt].itemsize
         else:
             if data.dtype not in _DT:
-                raise TypeError(f"Unsupported dtype: {data.dtype}. Supported: float32, float64")
+                raise TypeError(f"Unsupported dtype: {data.dtype}")
             if not data.flags["C_CONTIGUOUS"] or not data.flags["WRITEABLE"]:
                 raise ValueError("compression needs a C-contiguous writable array")
             dt, ptr, nbytes, size = _DT[data.dtype], data.ctypes.data, data.nbytes, data.size
@@ -822,7 +828,8 @@ def default_context():
 class falcons:
     """pyfalcons.falcons: compress / decompress / verify (falcons.pyx:31-333)."""
 
-    _SUPPORTED_DTYPES = (np.dtype(np.float32), np.dtype(np.float64))
+    # the element types pyfalcons compresses (falcons.pyx:22-28)
+    _SUPPORTED_DTYPES = (np.dtype(np.float32), np.dtype(np.float64), np.dtype(np.int32), np.dtype(np.int64))
 
     @staticmethod
     def compress(data, config, copy=False):
@@ -830,9 +837,9 @@ class falcons:
         dv = _device_view(data)
         if dv is not None:
             out, _, _ = default_context().compress_ex(data, config)
-            return out, dv[1] * (4 if dv[2] == 0 else 8) / float(out.size)
+            return out, dv[1] * _DT_OF[dv[2]].itemsize / float(out.size)
         if data.dtype not in falcons._SUPPORTED_DTYPES:
-            raise TypeError(f"Unsupported dtype: {data.dtype}. Supported: float32, float64")
+            raise TypeError(f"Unsupported dtype: {data.dtype}. Supported: float32, float64, int32, int64")
         if data.size == 0:
             raise ValueError("Cannot compress an empty array")
         if not isinstance(config, falconsConfig):
@@ -875,7 +882,7 @@ class falcons:
         out = from_dlpack(out, default_context().stream_handle())
         dv = _device_view(out)
         if dv is not None:
-            dt, n = (np.float32 if dv[2] == 0 else np.float64), dv[1]
+            dt, n = _DT_OF[dv[2]].type, dv[1]
         else:
             dt = out.dtype.type if out.dtype in falcons._SUPPORTED_DTYPES else None
             n = out.size

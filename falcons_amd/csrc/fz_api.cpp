@@ -258,7 +258,7 @@ extern "C" int fz_read_config(const uint8_t* cmp, size_t len, fz_config* out, ch
 }
 
 extern "C" size_t fz_compress_size_bound(const fz_config* c, int dtype) {
-    return 4096 + fz_config_size_est(c) + fz::zstd().compressBound(c->num * (dtype == FZ_DOUBLE ? 8 : 4));
+    return 4096 + fz_config_size_est(c) + fz::zstd().compressBound(c->num * (elem_size(dtype)));
 }
 
 // ------------------------------------------------------------------ level / pass plan
@@ -428,8 +428,8 @@ std::vector<uint8_t> zstd_unframe(const uint8_t* src, size_t n) {
 }
 
 void check_supported(const fz_config& c, int dtype) {
-    if (dtype != FZ_FLOAT && dtype != FZ_DOUBLE) throw Unsupported("only float and double fields are on this path");
-    if (c.N < 1 || c.N > 3) throw Unsupported("4-D fields are not on this path");
+    if (dtype < FZ_FLOAT || dtype > FZ_INT64) throw Unsupported("unknown element type");
+    if (c.N < 1 || c.N > 4) throw Unsupported("fields of more than 4 dimensions are not on this path");
     if (c.openmp) throw Unsupported("the OpenMP stream format is not produced by this path");
     if (c.quantbin_cnt != 65536) throw Unsupported("quantbinCnt other than 65536 is not on this path");
 }
@@ -454,12 +454,12 @@ struct Timer {
 // beyond it on demand (RLE+FSE streams and escape-heavy fields take more), so these size the first
 // allocation and the admission of served requests, not a hard limit.
 size_t fz::compress_workspace_bytes(uint64_t n, int dtype) {
-    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    const size_t es = elem_size(dtype);
     return n * (3 * es + 24) + (64u << 20);
 }
 
 size_t fz::decompress_workspace_bytes(uint64_t n, int dtype, uint64_t nunpred, size_t pre_len) {
-    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    const size_t es = elem_size(dtype);
     return n * (es + 14) + nunpred * es + 2 * pre_len + (64u << 20);
 }
 
@@ -489,7 +489,7 @@ struct ArenaReport {
 size_t gpu_compress(fz_ctx* ctx, DevArena& ws, fz_config& c, int dtype, void* d_data, int simd,
                     uint8_t* d_pre, size_t pre_cap, uint8_t** pre_out, fz_opts* o) {
     hipStream_t st = ctx->stream;
-    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    const size_t es = elem_size(dtype);
     const uint64_t n = c.num;
     ws.reset();
     ws.reserve(compress_workspace_bytes(n, dtype));
@@ -497,7 +497,7 @@ size_t gpu_compress(fz_ctx* ctx, DevArena& ws, fz_config& c, int dtype, void* d_
     ArenaReport arena_report{ws, "compress", n};
     Timer tm{st, ctx->ev0, ctx->ev1};
     double* eb_dev = ws.take<double>(1);
-    void* mm_scratch = ws.take<uint8_t>(minmax_scratch_bytes());
+    void* mm_scratch = ws.take<uint8_t>(std::max(minmax_scratch_bytes(), gen_minmax_scratch_bytes()));
     // error bound (calAbsErrorBound, Statistic.hpp:216-240)
     double psnr_factor = 0;
     if (c.eb_mode == 2) psnr_factor = std::pow(10, (c.psnr_eb + 10 * std::log10(1 - 2.0 / 3.0 * 0.99)) / -20);
@@ -505,14 +505,18 @@ size_t gpu_compress(fz_ctx* ctx, DevArena& ws, fz_config& c, int dtype, void* d_
         c.abs_eb = std::sqrt(3.0 / (double)n) * c.l2norm_eb;
         c.eb_mode = 0;
     }
+    // integer elements and 4-D fields take the generic traversal (IDH:2314-2519)
+    const bool generic = dtype > FZ_DOUBLE || c.N == 4;
     tm.start();
     if (c.eb_mode == 0) {
         FZ_HIP(hipMemcpyAsync(eb_dev, &c.abs_eb, 8, hipMemcpyHostToDevice, st));
+    } else if (dtype > FZ_DOUBLE) {
+        gen_abs_error_bound(dtype, d_data, n, c.eb_mode, c.abs_eb, c.rel_eb, psnr_factor, mm_scratch, eb_dev, st);
     } else {
         abs_error_bound(dtype, d_data, n, simd, c.eb_mode, c.abs_eb, c.rel_eb, psnr_factor, mm_scratch, eb_dev, st);
     }
     double eb_host = c.abs_eb;
-    const bool need_host_eb = c.eb_mode != 0 || c.cmpr_algo == 1;
+    const bool need_host_eb = c.eb_mode != 0 || c.cmpr_algo == 1 || generic;
     if (need_host_eb) {
         FZ_HIP(hipMemcpyAsync(&eb_host, eb_dev, 8, hipMemcpyDeviceToHost, st));
         FZ_HIP(hipStreamSynchronize(st));
@@ -532,34 +536,63 @@ size_t gpu_compress(fz_ctx* ctx, DevArena& ws, fz_config& c, int dtype, void* d_
         c.cmpr_algo = 2;
     }
     if (c.cmpr_algo != 2) throw Unsupported("only the interpolation algorithms are on this path");
-    const Plan P = make_plan(c.N, c.dims, c.interp_anchor_stride, c.interp_algo, c.interp_direction);
-    std::vector<double> ratio;
-    const int rmode = ratio_table(c.interp_alpha, c.interp_beta, P.levels, ratio);
-    double* ratio_dev = ws.take<double>(ratio.size());
-    LevelEb* lv = ws.take<LevelEb>(std::max(P.levels, 1));
-    FZ_HIP(hipMemcpyAsync(ratio_dev, ratio.data(), ratio.size() * 8, hipMemcpyHostToDevice, st));
-    if (P.levels > 0) level_constants(eb_dev, ratio_dev, rmode, P.levels, lv, st);
-    int16_t* codes = ws.take<int16_t>(P.ncodes + 16);
-    void* escval = ws.take<uint8_t>((P.ncodes + 16) * es);
-    void* unpred = ws.take<uint8_t>((n + 16) * es);
-    uint64_t* esc_idx = ws.take<uint64_t>(P.ncodes + 1);
-    uint64_t* nesc_dev = ws.take<uint64_t>(1);
-    const size_t sel_bytes = select_escapes_scratch(P.ncodes);
-    void* sel_tmp = ws.take<uint8_t>(sel_bytes + 1);
-    if (P.anchor) interp_anchors(dtype, false, d_data, unpred, P.acnt, P.astr, st);
-    else interp_first(dtype, false, d_data, codes, escval, eb_dev, nullptr, 0, nullptr, st);
-    const int avx2 = simd ? 1 : 0;
-    for (const auto& p : P.passes) interp_pass(dtype, false, p, d_data, codes, escval, lv, avx2, nullptr, 0, nullptr, st);
-    select_escapes(codes, P.ncodes, esc_idx, nesc_dev, sel_tmp, sel_bytes, st);
-    gather_escapes(dtype, esc_idx, nesc_dev, P.ncodes, escval, static_cast<uint8_t*>(unpred) + P.nanchors * es, st);
+    uint64_t ncodes, nanchors, anchor_used;
+    int16_t* codes;
+    void* unpred;
+    uint64_t* esc_idx;
+    uint64_t* nesc_dev;
+    if (generic) {
+        const GenPlan G = gen_plan(c.N, c.dims, c.interp_anchor_stride, c.interp_algo, c.interp_direction,
+                                   c.interp_alpha, c.interp_beta, c.abs_eb);
+        ncodes = G.ncodes;
+        nanchors = G.anchors.total;
+        anchor_used = G.anchor;
+        codes = ws.take<int16_t>(ncodes + 16);
+        void* escval = ws.take<uint8_t>((ncodes + 16) * es);
+        unpred = ws.take<uint8_t>((n + 16) * es);
+        esc_idx = ws.take<uint64_t>(ncodes + 1);
+        nesc_dev = ws.take<uint64_t>(1);
+        const size_t sel_bytes = select_escapes_scratch(ncodes);
+        void* sel_tmp = ws.take<uint8_t>(sel_bytes + 1);
+        gen_anchors(dtype, false, d_data, unpred, G.anchors, st);
+        gen_traverse(dtype, false, G.steps, 1, n, 0, d_data, codes, escval, GenEsc{}, st);
+        select_escapes(codes, ncodes, esc_idx, nesc_dev, sel_tmp, sel_bytes, st);
+        gen_gather_escapes(dtype, esc_idx, nesc_dev, ncodes, escval, static_cast<uint8_t*>(unpred) + nanchors * es, st);
+    } else {
+        const Plan P = make_plan(c.N, c.dims, c.interp_anchor_stride, c.interp_algo, c.interp_direction);
+        ncodes = P.ncodes;
+        nanchors = P.nanchors;
+        anchor_used = P.anchor;
+        std::vector<double> ratio;
+        const int rmode = ratio_table(c.interp_alpha, c.interp_beta, P.levels, ratio);
+        double* ratio_dev = ws.take<double>(ratio.size());
+        LevelEb* lv = ws.take<LevelEb>(std::max(P.levels, 1));
+        FZ_HIP(hipMemcpyAsync(ratio_dev, ratio.data(), ratio.size() * 8, hipMemcpyHostToDevice, st));
+        if (P.levels > 0) level_constants(eb_dev, ratio_dev, rmode, P.levels, lv, st);
+        codes = ws.take<int16_t>(P.ncodes + 16);
+        void* escval = ws.take<uint8_t>((P.ncodes + 16) * es);
+        unpred = ws.take<uint8_t>((n + 16) * es);
+        esc_idx = ws.take<uint64_t>(P.ncodes + 1);
+        nesc_dev = ws.take<uint64_t>(1);
+        const size_t sel_bytes = select_escapes_scratch(P.ncodes);
+        void* sel_tmp = ws.take<uint8_t>(sel_bytes + 1);
+        if (P.anchor) interp_anchors(dtype, false, d_data, unpred, P.acnt, P.astr, st);
+        else interp_first(dtype, false, d_data, codes, escval, eb_dev, nullptr, 0, nullptr, st);
+        const int avx2 = simd ? 1 : 0;
+        for (const auto& p : P.passes)
+            interp_pass(dtype, false, p, d_data, codes, escval, lv, avx2, nullptr, 0, nullptr, st);
+        select_escapes(codes, P.ncodes, esc_idx, nesc_dev, sel_tmp, sel_bytes, st);
+        gather_escapes(dtype, esc_idx, nesc_dev, P.ncodes, escval, static_cast<uint8_t*>(unpred) + P.nanchors * es,
+                       st);
+    }
     uint64_t nesc = 0;
     FZ_HIP(hipMemcpyAsync(&nesc, nesc_dev, 8, hipMemcpyDeviceToHost, st));
     FZ_HIP(hipStreamSynchronize(st));
-    const uint64_t nun = P.nanchors + nesc;
+    const uint64_t nun = nanchors + nesc;
     // pre-zstd buffer: decomposition.save, nCodes, encoder.save + payload (FGC:61-83, IDH:241-252,
     // LinearQuantizer.hpp:142-152); the encoder writes its bytes in place
     const size_t head = decomp_head_bytes(c.N);
-    const size_t enc_cap = entropy_encode_bound(P.ncodes);
+    const size_t enc_cap = entropy_encode_bound(ncodes);
     if (!d_pre) {
         pre_cap = head + nun * es + 8 + enc_cap;
         d_pre = ws.take<uint8_t>(pre_cap);
@@ -573,25 +606,25 @@ size_t gpu_compress(fz_ctx* ctx, DevArena& ws, fz_config& c, int dtype, void* d_
     put<uint32_t>(p, (uint32_t)max_dim);
     put<int32_t>(p, c.interp_algo);
     put<int32_t>(p, c.interp_direction);
-    put<uint64_t>(p, P.anchor);
+    put<uint64_t>(p, anchor_used);
     put<double>(p, c.interp_alpha);
     put<double>(p, c.interp_beta);
-    put<uint8_t>(p, 2);
+    put<uint8_t>(p, dtype > FZ_DOUBLE ? 4 : 2);  // integral_uid for integers (LinearQuantizer.hpp:142-152)
     put<double>(p, c.abs_eb);
     put<int32_t>(p, c.quantbin_cnt / 2);
     put<int32_t>(p, (int32_t)nun);
     FZ_HIP(hipMemcpyAsync(d_pre, hbuf.data(), head, hipMemcpyHostToDevice, st));
     if (nun) FZ_HIP(hipMemcpyAsync(d_pre + head, unpred, nun * es, hipMemcpyDeviceToDevice, st));
-    const uint64_t nq = P.ncodes;
+    const uint64_t nq = ncodes;
     FZ_HIP(hipMemcpyAsync(d_pre + head + nun * es, &nq, 8, hipMemcpyHostToDevice, st));
     EntropyStats es_stats;
-    const size_t enc_len = entropy_encode(codes, P.ncodes, d_pre + head + nun * es + 8, enc_cap, ws, st, &es_stats);
+    const size_t enc_len = entropy_encode(codes, ncodes, d_pre + head + nun * es + 8, enc_cap, ws, st, &es_stats);
     const double tdev = tm.stop();
     const size_t total = head + nun * es + 8 + enc_len;
     if (pre_out) *pre_out = d_pre;
     if (o) {
         o->t_device_ms = tdev;
-        o->ncodes = P.ncodes;
+        o->ncodes = ncodes;
         o->nunpred = nun;
         o->prezstd_bytes = total;
         o->method = es_stats.method;
@@ -614,12 +647,16 @@ struct PreLayout {
     int32_t nun;
     size_t un_off, enc_off, enc_len;
     uint64_t ncodes;
+    bool generic;  // integer elements or a 4-D field: the generic traversal
+    int legacy;    // an integer stream saved with the floating quantizer uid
     Plan P;
+    GenPlan G;
 };
 
 PreLayout parse_pre(const fz_config& c, int dtype, const uint8_t* h_pre, size_t pre_len) {
-    if (c.N > 3) throw Unsupported("4-D fields are not on this path");
-    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    if (dtype < FZ_FLOAT || dtype > FZ_INT64) throw Unsupported("unknown element type");
+    if (c.N < 1 || c.N > 4) throw Unsupported("fields of more than 4 dimensions are not on this path");
+    const size_t es = elem_size(dtype);
     const uint64_t n = c.num;
     PreLayout L;
     const uint8_t* p = h_pre;
@@ -635,7 +672,11 @@ PreLayout parse_pre(const fz_config& c, int dtype, const uint8_t* h_pre, size_t 
     L.anchor = get<uint64_t>(p);
     L.alpha = get<double>(p);
     L.beta = get<double>(p);
-    if (get<uint8_t>(p) != 2) throw ArgError("LinearQuantizer uid mismatch");
+    // LinearQuantizer::load (LinearQuantizer.hpp:154-170): integers take the integral uid, or the
+    // floating one for a legacy stream
+    const uint8_t uid = get<uint8_t>(p);
+    if (!(uid == 2 || (dtype > FZ_DOUBLE && uid == 4))) throw ArgError("LinearQuantizer uid mismatch");
+    L.legacy = dtype > FZ_DOUBLE && uid == 2;
     L.eb = get<double>(p);
     (void)get<int32_t>(p);
     L.nun = get<int32_t>(p);
@@ -649,8 +690,18 @@ PreLayout parse_pre(const fz_config& c, int dtype, const uint8_t* h_pre, size_t 
     L.ncodes = get<uint64_t>(p);
     L.enc_off = (size_t)(p - h_pre);
     L.enc_len = pre_len - L.enc_off;
-    L.P = make_plan(c.N, L.dims, (long)L.anchor, L.algo, L.dir);
-    if (L.P.ncodes != L.ncodes) throw ArgError("code count does not match the field shape");
+    L.generic = dtype > FZ_DOUBLE || c.N == 4;
+    if (L.generic) {
+        if (L.algo < 0 || L.algo > 1) throw ArgError("unknown interpolation algorithm");
+        int fact = 1;
+        for (int i = 2; i <= c.N; i++) fact *= i;
+        if (L.dir < 0 || L.dir >= fact) throw ArgError("interpolation direction out of range");
+        L.G = gen_plan(c.N, L.dims, (long)L.anchor, L.algo, L.dir, L.alpha, L.beta, L.eb);
+        if (L.G.ncodes != L.ncodes) throw ArgError("code count does not match the field shape");
+    } else {
+        L.P = make_plan(c.N, L.dims, (long)L.anchor, L.algo, L.dir);
+        if (L.P.ncodes != L.ncodes) throw ArgError("code count does not match the field shape");
+    }
     entropy_check(h_pre + L.enc_off, L.enc_len, L.ncodes);
     return L;
 }
@@ -658,7 +709,7 @@ PreLayout parse_pre(const fz_config& c, int dtype, const uint8_t* h_pre, size_t 
 void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h_pre, const uint8_t* d_pre,
                     size_t pre_len, void** d_out_io, fz_opts* o) {
     hipStream_t st = ctx->stream;
-    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    const size_t es = elem_size(dtype);
     const uint64_t n = c.num;
     const PreLayout PL = parse_pre(c, dtype, h_pre, pre_len);
     const int32_t nun = PL.nun;
@@ -704,13 +755,33 @@ void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h
     uint64_t nesc = 0;
     FZ_HIP(hipMemcpyAsync(&nesc, nesc_dev, 8, hipMemcpyDeviceToHost, st));
     FZ_HIP(hipStreamSynchronize(st));
-    if (P.nanchors + nesc != (uint64_t)nun) throw ArgError("unpredictable count does not match the code stream");
-    if (P.levels > 0) level_constants(eb_dev, ratio_dev, rmode, P.levels, lv, st);
-    const void* esc_vals = static_cast<const uint8_t*>(d_unpred) + P.nanchors * es;
-    if (P.anchor) interp_anchors(dtype, true, d_out, d_unpred, P.acnt, P.astr, st);
-    else interp_first(dtype, true, d_out, codes, nullptr, eb_dev, esc_idx, nesc, esc_vals, st);
-    for (const auto& ps : P.passes)
-        interp_pass(dtype, true, ps, d_out, codes, nullptr, lv, 1, esc_idx, nesc, esc_vals, st);
+    if (PL.generic) {
+        const GenPlan& G = PL.G;
+        if (G.anchors.total + nesc != (uint64_t)nun)
+            throw ArgError("unpredictable count does not match the code stream");
+        int* err_dev = ws.take<int>(1);
+        FZ_HIP(hipMemsetAsync(err_dev, 0, sizeof(int), st));
+        gen_anchors(dtype, true, d_out, d_unpred, G.anchors, st);
+        GenEsc e;
+        e.idx = esc_idx;
+        e.n = nesc;
+        e.vals = static_cast<const uint8_t*>(d_unpred) + G.anchors.total * es;
+        e.legacy = PL.legacy;
+        e.err = err_dev;
+        gen_traverse(dtype, true, G.steps, 1, n, 0, d_out, codes, nullptr, e, st);
+        int err = 0;
+        FZ_HIP(hipMemcpyAsync(&err, err_dev, sizeof(int), hipMemcpyDeviceToHost, st));
+        FZ_HIP(hipStreamSynchronize(st));
+        if (err) throw std::runtime_error("LinearQuantizer integral reconstruction overflow");
+    } else {
+        if (P.nanchors + nesc != (uint64_t)nun) throw ArgError("unpredictable count does not match the code stream");
+        if (P.levels > 0) level_constants(eb_dev, ratio_dev, rmode, P.levels, lv, st);
+        const void* esc_vals = static_cast<const uint8_t*>(d_unpred) + P.nanchors * es;
+        if (P.anchor) interp_anchors(dtype, true, d_out, d_unpred, P.acnt, P.astr, st);
+        else interp_first(dtype, true, d_out, codes, nullptr, eb_dev, esc_idx, nesc, esc_vals, st);
+        for (const auto& ps : P.passes)
+            interp_pass(dtype, true, ps, d_out, codes, nullptr, lv, 1, esc_idx, nesc, esc_vals, st);
+    }
     const double tdev = tm.stop();
     if (o) {
         o->t_device_ms = tdev;
@@ -725,7 +796,6 @@ extern "C" int fz_check_prezstd(const fz_config* conf, int dtype, const uint8_t*
                                 size_t errlen) {
     try {
         if (!conf || !pre) throw ArgError("null argument");
-        if (dtype != FZ_FLOAT && dtype != FZ_DOUBLE) throw Unsupported("only float and double fields are on this path");
         parse_pre(*conf, dtype, pre, len);
         return FZ_OK;
     } catch (const Unsupported& e) {
@@ -802,7 +872,7 @@ size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_
     fz_config c = *conf;
     if (cap < fz_compress_size_bound(&c, dtype)) throw ArgError("The buffer for compressed data is not large enough.");
     check_supported(c, dtype);
-    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    const size_t es = elem_size(dtype);
     const uint64_t n = c.num;
     // the payload starts after the header size estimated from the caller's configuration; the
     // final header is written at the front afterwards (falcons.hpp:59-96)
@@ -893,8 +963,8 @@ void decompress_to_device(fz_ctx* ctx, const uint8_t* cmp, size_t len, int dtype
     if (conf_out) *conf_out = c;
     if (c.num > out_cap_elems) throw ArgError("The decompression buffer is too small");
     if (c.openmp) throw Unsupported("the OpenMP stream format is not produced by this path");
-    if (dtype != FZ_FLOAT && dtype != FZ_DOUBLE) throw Unsupported("only float and double fields are on this path");
-    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    if (dtype < FZ_FLOAT || dtype > FZ_INT64) throw Unsupported("unknown element type");
+    const size_t es = elem_size(dtype);
     const uint64_t n = c.num;
     hipStream_t st = ctx->stream;
     const double tz0 = now_ms();
@@ -928,7 +998,7 @@ extern "C" int fz_compress(fz_ctx* ctx, const fz_config* conf, int dtype, void* 
         FZ_HIP(hipSetDevice(ctx->device));
         if (cap < fz_compress_size_bound(conf, dtype)) throw ArgError("The buffer for compressed data is not large enough.");
         check_supported(*conf, dtype);
-        const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+        const size_t es = elem_size(dtype);
         const uint64_t n = conf->num;
         // device copy of the field
         void* d_data = data;
@@ -969,7 +1039,7 @@ extern "C" int fz_decompress(fz_ctx* ctx, const uint8_t* cmp, size_t len, int dt
         void* d_out = out_on_device ? out : nullptr;
         decompress_to_device(ctx, cmp, len, dtype, &d_out, out_cap_elems, c, o);
         if (!out_on_device) {
-            const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+            const size_t es = elem_size(dtype);
             FZ_HIP(hipMemcpyAsync(out, d_out, c->num * es, hipMemcpyDeviceToHost, ctx->stream));
             FZ_HIP(hipStreamSynchronize(ctx->stream));
         }
@@ -997,7 +1067,7 @@ extern "C" int fz_compress_device(fz_ctx* ctx, fz_config* conf, int dtype, void*
 }
 
 extern "C" size_t fz_prezstd_bound(const fz_config* c, int dtype) {
-    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    const size_t es = elem_size(dtype);
     return decomp_head_bytes(c->N) + c->num * es + 8 + entropy_encode_bound(c->num);
 }
 

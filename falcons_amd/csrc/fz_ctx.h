@@ -11,6 +11,7 @@
 #include "fz_common.h"
 #include "fz_kernels.h"
 #include "fz_tuner.h"
+#include "fz_generic.h"
 
 namespace fz {
 

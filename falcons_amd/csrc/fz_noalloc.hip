@@ -31,7 +31,7 @@ std::mutex g_setup_mu;
 
 // Largest field the scratch array serves: compress_workspace_bytes(n) = n * (3 s + 24) + 64 MiB.
 size_t max_elems(int dtype) {
-    const size_t es = dtype == FZ_DOUBLE ? 8 : 4;
+    const size_t es = fz::elem_size(dtype);
     return (kScratchBytes - (size_t(64) << 20)) / (3 * es + 24);
 }
 

@@ -107,7 +107,7 @@ WorkerSet& workers_of(fz_ctx* ctx) {
     return *ctx->workers;
 }
 
-size_t es_of(int dtype) { return dtype == FZ_DOUBLE ? 8 : 4; }
+size_t es_of(int dtype) { return elem_size(dtype); }
 
 // Item order for the workers: largest first, so that the last items to finish are small.
 template <class Size>
@@ -407,7 +407,7 @@ extern "C" int fz_server_create(int device, int slots, size_t hbm_budget, fz_ser
 extern "C" int fz_server_submit(fz_server* S, const fz_request* r) {
     if (!S || !r) return FZ_ERR_INVALID_ARGUMENT;
     if (r->op != 0 && r->op != 1) return FZ_ERR_INVALID_ARGUMENT;
-    if (r->dtype != FZ_FLOAT && r->dtype != FZ_DOUBLE) return FZ_ERR_UNSUPPORTED;
+    if (r->dtype < FZ_FLOAT || r->dtype > FZ_INT64) return FZ_ERR_UNSUPPORTED;
     Job j;
     j.req = *r;
     j.t_submit = now_ms();

@@ -27,7 +27,19 @@ enum {
     FZ_ERR_RUNTIME = 4
 };
 
-enum { FZ_FLOAT = 0, FZ_DOUBLE = 1 };
+/* Element types, numbered as Falcons_FLOAT .. Falcons_INT64 (include/Falcons/utils/Config.hpp:29-38). */
+enum {
+    FZ_FLOAT = 0,
+    FZ_DOUBLE = 1,
+    FZ_UINT8 = 2,
+    FZ_INT8 = 3,
+    FZ_UINT16 = 4,
+    FZ_INT16 = 5,
+    FZ_UINT32 = 6,
+    FZ_INT32 = 7,
+    FZ_UINT64 = 8,
+    FZ_INT64 = 9
+};
 
 /* Serialized fields of Falcons::Config (include/Falcons/utils/Config.hpp:482-511) plus the
  * interpolation module settings that travel in the decomposition header. */

@@ -71,6 +71,16 @@ size_t fo_fse_compress_pub(uint8_t *dst, size_t cap, const uint8_t *src, size_t 
 size_t fo_fse_decompress_pub(uint8_t *dst, size_t n, const uint8_t *src, size_t srcSize);
 size_t fo_fse_normalize_pub(short *norm, unsigned tableLog, const unsigned *count, size_t total, unsigned maxSym);
 
+/* element types: 0 float, 1 double, 2 uint8, 3 int8, 4 uint16, 5 int16, 6 uint32, 7 int32,
+ * 8 uint64, 9 int64 (Falcons_FLOAT .. Falcons_INT64, Config.hpp:29-38) */
+size_t fo_elem_size(int dtype);
+/* The generic traversal (integer elements, 4-D fields): oracle_generic.c */
+int fo_gen_compress(int dtype, int N, const size_t *dims, int cubic, int dir, long anchor, double alpha,
+                    double beta, double eb, void *data, int16_t *codes, size_t *ncodes, void *unpred,
+                    size_t unpred_cap, size_t *nunpred, size_t *anchor_out);
+int fo_gen_decompress(int dtype, int N, const size_t *dims, int cubic, int dir, long anchor, double alpha,
+                      double beta, double eb, int legacy, const int16_t *codes, size_t ncodes, const void *unpred,
+                      size_t nunpred, void *out);
 double fo_data_range(int dtype, const void *data, size_t n);
 size_t fo_size_bound(const fo_config *c, int dtype);
 /* Falcons_compress: returns the stream length, 0 on error (error text in fo_last_error()).

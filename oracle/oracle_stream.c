@@ -194,14 +194,37 @@ int fo_config_load(fo_config *c, const uint8_t *in, size_t avail, size_t *hdr_le
     }                                                                                \
     for (; i < n; i++) { mx = FO_MAX(mx, d[i]); mn = FO_MIN(mn, d[i]); }
 
+/* Integer fields: scalar_minmax, the difference formed in the element type (wrapping). */
+#define FO_IRANGE(T, UT)                                                                          \
+    {                                                                                             \
+        const T *d = (const T *)data;                                                             \
+        T mn = d[0], mx = d[0];                                                                   \
+        for (size_t i = 1; i < n; i++) { mx = FO_MAX(mx, d[i]); mn = FO_MIN(mn, d[i]); }          \
+        return (double)(T)(UT)((UT)mx - (UT)mn);                                                  \
+    }
+
 double fo_data_range(int dtype, const void *data, size_t n) {
     if (dtype == 0) {
         FO_RANGE_BODY(float, 8)
         return (double)(float)(mx - mn);
     }
-    FO_RANGE_BODY(double, 4)
-    return mx - mn;
+    if (dtype == 1) {
+        FO_RANGE_BODY(double, 4)
+        return mx - mn;
+    }
+    switch (dtype) {
+        case 2: FO_IRANGE(uint8_t, uint8_t)
+        case 3: FO_IRANGE(int8_t, uint8_t)
+        case 4: FO_IRANGE(uint16_t, uint16_t)
+        case 5: FO_IRANGE(int16_t, uint16_t)
+        case 6: FO_IRANGE(uint32_t, uint32_t)
+        case 7: FO_IRANGE(int32_t, uint32_t)
+        case 8: FO_IRANGE(uint64_t, uint64_t)
+        case 9: FO_IRANGE(int64_t, uint64_t)
+        default: return 0;
+    }
 }
+#undef FO_IRANGE
 #undef FO_RANGE_BODY
 
 /* calAbsErrorBound (Statistic.hpp:216-240) */
@@ -276,14 +299,16 @@ static uint8_t *fo_unlossless(const uint8_t *src, size_t n, size_t *outlen) {
 size_t fo_size_bound(const fo_config *c, int dtype) {
     uint8_t tmp[256];
     size_t es = fo_config_save(c, tmp);
-    return 4096 + es + (fo_zstd() ? fo_zstd()->compressBound(c->num * (dtype ? 8 : 4)) : 0);
+    return 4096 + es + (fo_zstd() ? fo_zstd()->compressBound(c->num * fo_elem_size(dtype)) : 0);
 }
 
 /* Falcons_compress_Interp + FalconsGenericCompressor::compress: the buffer handed to the lossless
  * stage is [decomposition.save][u64 nCodes][encoder.save][encoder payload] (FGC:61-83). */
 static size_t fo_interp_payload(fo_config *c, int dtype, void *data, uint8_t *out, size_t cap, uint8_t *pre,
                                 size_t pre_cap, size_t *pre_len) {
-    size_t es = dtype ? 8 : 4;
+    size_t es = fo_elem_size(dtype);
+    /* integer elements and 4-D fields take the generic traversal (IDH:2314-2519) */
+    const int generic = dtype >= 2 || c->N == 4;
     if (c->interp_anchor_stride < 0) {
         static const int def[4] = {4096, 128, 32, 16};
         c->interp_anchor_stride = def[c->N - 1];
@@ -294,9 +319,13 @@ static size_t fo_interp_payload(fo_config *c, int dtype, void *data, uint8_t *ou
     int16_t *codes = (int16_t *)malloc((n + 16) * 2);
     void *unpred = malloc((n + 16) * es);
     size_t nq = 0, nun = 0, anchor = 0;
-    if (fo_interp_compress(dtype, c->N, dims, c->interp_algo, c->interp_direction, c->interp_anchor_stride,
-                           c->interp_alpha, c->interp_beta, c->abs_eb, data, codes, &nq, unpred, n + 16, &nun,
-                           &anchor)) {
+    int rc = generic ? fo_gen_compress(dtype, c->N, dims, c->interp_algo, c->interp_direction,
+                                       c->interp_anchor_stride, c->interp_alpha, c->interp_beta, c->abs_eb, data,
+                                       codes, &nq, unpred, n + 16, &nun, &anchor)
+                     : fo_interp_compress(dtype, c->N, dims, c->interp_algo, c->interp_direction,
+                                          c->interp_anchor_stride, c->interp_alpha, c->interp_beta, c->abs_eb, data,
+                                          codes, &nq, unpred, n + 16, &nun, &anchor);
+    if (rc) {
         set_err("unsupported interpolation direction");
         free(codes); free(unpred);
         return 0;
@@ -311,7 +340,7 @@ static size_t fo_interp_payload(fo_config *c, int dtype, void *data, uint8_t *ou
     w64(&p, anchor);
     wf64(&p, c->interp_alpha);
     wf64(&p, c->interp_beta);
-    w8(&p, 2); /* LinearQuantizer uid (LinearQuantizer.hpp:252) */
+    w8(&p, dtype >= 2 ? 4 : 2); /* LinearQuantizer uid, integral_uid for integers (LinearQuantizer.hpp:142-152, 252-253) */
     wf64(&p, c->abs_eb);
     w32(&p, (uint32_t)(c->quantbin_cnt / 2));
     w32(&p, (uint32_t)nun);
@@ -331,7 +360,7 @@ static size_t fo_interp_payload(fo_config *c, int dtype, void *data, uint8_t *ou
 size_t fo_compress(const fo_config *conf_in, int dtype, void *data, uint8_t *out, size_t cap, uint8_t *pre,
                    size_t pre_cap, size_t *pre_len) {
     fo_config c = *conf_in;
-    size_t es = dtype ? 8 : 4;
+    size_t es = fo_elem_size(dtype);
     if (pre_len) *pre_len = 0;
     if (cap < fo_size_bound(&c, dtype)) { set_err("The buffer for compressed data is not large enough."); return 0; }
     if (c.openmp) { set_err("the OpenMP stream format is not produced by this path"); return 0; }
@@ -384,7 +413,7 @@ int fo_decompress(const uint8_t *in, size_t len, int dtype, void *out, size_t ou
     if (conf_out) *conf_out = c;
     if (c.num > out_cap_elems) { set_err("The decompression buffer is too small"); return -1; }
     if (c.openmp) { set_err("the OpenMP stream format is not produced by this path"); return -1; }
-    size_t es = dtype ? 8 : 4;
+    size_t es = fo_elem_size(dtype);
     size_t bl;
     uint8_t *buf = fo_unlossless(in + hs, len - hs, &bl);
     if (!buf) return -1;
@@ -405,7 +434,10 @@ int fo_decompress(const uint8_t *in, size_t len, int dtype, void *out, size_t ou
     memcpy(&anchor, p, 8); p += 8;
     memcpy(&alpha, p, 8); p += 8;
     memcpy(&beta, p, 8); p += 8;
-    if (*p++ != 2) { free(buf); set_err("LinearQuantizer uid mismatch"); return -1; }
+    /* LinearQuantizer::load (LinearQuantizer.hpp:154-170): integers accept both uids, the floating
+     * one marking a legacy stream */
+    const uint8_t uid = *p++;
+    if (!(uid == 2 || (dtype >= 2 && uid == 4))) { free(buf); set_err("LinearQuantizer uid mismatch"); return -1; }
     memcpy(&eb, p, 8); p += 8;
     memcpy(&radius, p, 4); p += 4;
     memcpy(&nun, p, 4); p += 4;
@@ -417,7 +449,12 @@ int fo_decompress(const uint8_t *in, size_t len, int dtype, void *out, size_t ou
     if (fo_rle_decode(p, bl - (size_t)(p - buf), nq, codes, NULL)) { free(codes); free(buf); set_err("entropy decode failed"); return -1; }
     void *un = malloc((size_t)nun * es + 8);
     memcpy(un, unpred, (size_t)nun * es);
-    int rc = fo_interp_decompress(dtype, c.N, dims, algo, dir, (long)anchor, alpha, beta, eb, codes, nq, un, nun, out);
+    int rc = (dtype >= 2 || c.N == 4)
+                 ? fo_gen_decompress(dtype, c.N, dims, algo, dir, (long)anchor, alpha, beta, eb, uid == 2, codes, nq, un,
+                                     nun, out)
+                 : fo_interp_decompress(dtype, c.N, dims, algo, dir, (long)anchor, alpha, beta, eb, codes, nq, un, nun,
+                                        out);
+    if (rc == -4) set_err("LinearQuantizer integral reconstruction overflow");
     free(un); free(codes); free(buf);
     return rc;
 }

@@ -30,7 +30,7 @@ long ref_sample_t(const size_t *dims_in, const void *data_in, size_t sbs, double
     Falcons::profiling_block<T, N>(data.data(), dims, starts, sbs, abseb, sbs / 4, false);
     *nstarts = starts.size();
     for (size_t i = 0; i < starts.size() && i < starts_cap; i++)
-        for (size_t a = 0; a < 3; a++) starts_out[3 * i + a] = a < starts[i].size() ? starts[i][a] : 0;
+        for (size_t a = 0; a < 4; a++) starts_out[4 * i + a] = a < starts[i].size() ? starts[i][a] : 0;
     if (profiling < 0) {  // the decision of FalconsAlgoInterp.hpp:321
         size_t per = 1;
         for (Falcons::uint i = 0; i < N; i++) per *= sbs + 1;
@@ -125,21 +125,25 @@ double ref_interp_f64(int kind, double a, double b, double c, double d) {
     }
 }
 
-// Block sampler of the parameter search (Sample.hpp:5-344). dtype 0 float, 1 double; N 1..3.
+// Block sampler of the parameter search (Sample.hpp:5-344). dtype 0 float, 1 double, 7 int32,
+// 9 int64; N 1..4; starts four slots each, the field's axes first.
 // profiling < 0 takes the reference's own decision. Returns 2 * blocks + profiling branch, or < 0.
 long ref_sample(int dtype, int N, const size_t *dims, const void *data, size_t sbs, double abseb, double rate,
                 int profiling, size_t *starts_out, size_t starts_cap, size_t *nstarts, void *blocks_out,
                 size_t blocks_cap) {
 #define RS_CASE(T, R) return ref_sample_t<T, R>(dims, data, sbs, abseb, rate, profiling, starts_out, starts_cap, nstarts, blocks_out, blocks_cap)
-    if (dtype == 0) {
-        if (N == 1) RS_CASE(float, 1);
-        if (N == 2) RS_CASE(float, 2);
-        if (N == 3) RS_CASE(float, 3);
-    } else {
-        if (N == 1) RS_CASE(double, 1);
-        if (N == 2) RS_CASE(double, 2);
-        if (N == 3) RS_CASE(double, 3);
-    }
+#define RS_TYPE(T) \
+    do { \
+        if (N == 1) RS_CASE(T, 1); \
+        if (N == 2) RS_CASE(T, 2); \
+        if (N == 3) RS_CASE(T, 3); \
+        if (N == 4) RS_CASE(T, 4); \
+    } while (0)
+    if (dtype == 0) RS_TYPE(float);
+    if (dtype == 1) RS_TYPE(double);
+    if (dtype == 7) RS_TYPE(int32_t);
+    if (dtype == 9) RS_TYPE(int64_t);
+#undef RS_TYPE
 #undef RS_CASE
     return -1;
 }

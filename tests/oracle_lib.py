@@ -12,6 +12,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libfalcons_ref.so")
 
+# element type codes of Falcons_FLOAT .. Falcons_INT64 (Config.hpp:29-38)
+DTYPE_CODE = {np.dtype(np.float32): 0, np.dtype(np.float64): 1, np.dtype(np.uint8): 2, np.dtype(np.int8): 3,
+              np.dtype(np.uint16): 4, np.dtype(np.int16): 5, np.dtype(np.uint32): 6, np.dtype(np.int32): 7,
+              np.dtype(np.uint64): 8, np.dtype(np.int64): 9}
+
+
+def dtype_code(dt):
+    return DTYPE_CODE[np.dtype(dt)]
+
 _u8p = C.POINTER(C.c_uint8)
 _i16p = C.POINTER(C.c_int16)
 _szp = C.POINTER(C.c_size_t)
@@ -147,7 +156,7 @@ class Oracle:
     def interp_compress(self, data, cubic, direction, anchor, alpha, beta, eb):
         """Returns (codes in stream order, unpredictable list, mutated data, anchor used)."""
         data = np.ascontiguousarray(data).copy()
-        dtype = 0 if data.dtype == np.float32 else 1
+        dtype = dtype_code(data.dtype)
         dims = (C.c_size_t * 4)(*data.shape)
         n = data.size
         codes = np.zeros(n + 16, dtype=np.int16)
@@ -165,7 +174,7 @@ class Oracle:
         dims = (C.c_size_t * 4)(*shape)
         codes = np.ascontiguousarray(codes, dtype=np.int16)
         unpred = np.ascontiguousarray(unpred, dtype=dtype)
-        rc = self.L.fo_interp_decompress(0 if np.dtype(dtype) == np.float32 else 1, len(shape), dims, cubic,
+        rc = self.L.fo_interp_decompress(dtype_code(dtype), len(shape), dims, cubic,
                                          direction, anchor, alpha, beta, eb, _ptr(codes, _i16p), codes.size,
                                          unpred.ctypes.data, unpred.size, out.ctypes.data)
         if rc:
@@ -185,10 +194,10 @@ class Oracle:
         """The parameter search's sampler: (block side, profiling branch, profiled starts, selected
         starts), starts as (count, 3) arrays; None when the field is not tuned."""
         data = np.ascontiguousarray(data)
-        dtype = 0 if data.dtype == np.float32 else 1
+        dtype = dtype_code(data.dtype)
         cap = data.size // 8 + 16
-        st = np.zeros((cap, 3), dtype=np.uint64)
-        sel = np.zeros((cap, 3), dtype=np.uint64)
+        st = np.zeros((cap, 4), dtype=np.uint64)
+        sel = np.zeros((cap, 4), dtype=np.uint64)
         sbs, prof, nst = C.c_size_t(0), C.c_int(0), C.c_size_t(0)
         sp = C.POINTER(C.c_size_t)
         n = self.L.fo_tune_sample(C.byref(conf), dtype, data.ctypes.data, C.byref(sbs), C.byref(prof),
@@ -200,7 +209,7 @@ class Oracle:
 
     def compress(self, data, conf, want_pre=False):
         data = np.ascontiguousarray(data).copy()
-        dtype = 0 if data.dtype == np.float32 else 1
+        dtype = dtype_code(data.dtype)
         cap = self.L.fo_size_bound(C.byref(conf), dtype)
         out = np.zeros(cap, dtype=np.uint8)
         pre = np.zeros(cap + data.nbytes // 2 + (1 << 20), dtype=np.uint8) if want_pre else None
@@ -223,7 +232,7 @@ class Oracle:
                 raise ValueError(self.L.fo_last_error().decode())
             num = conf.num
         out = np.zeros(num, dtype=dtype)
-        rc = self.L.fo_decompress(_ptr(b, _u8p), b.size, 0 if np.dtype(dtype) == np.float32 else 1, out.ctypes.data,
+        rc = self.L.fo_decompress(_ptr(b, _u8p), b.size, dtype_code(dtype), out.ctypes.data,
                                   num, C.byref(conf))
         if rc:
             raise ValueError(self.L.fo_last_error().decode())
@@ -270,12 +279,12 @@ class Ref:
 
     def sample(self, data, sbs, abseb, rate=0.005, profiling=-1):
         """profiling_block + sampleBlocks of the reference (Sample.hpp): (profiling branch, profiled
-        starts as a (count, 3) array, the sampled blocks concatenated)."""
+        starts as a (count, 4) array, the sampled blocks concatenated)."""
         data = np.ascontiguousarray(data)
-        dtype = 0 if data.dtype == np.float32 else 1
-        dims = (C.c_size_t * 3)(*data.shape)
+        dtype = dtype_code(data.dtype)
+        dims = (C.c_size_t * 4)(*data.shape)
         cap = data.size // 8 + 16
-        st = np.zeros((cap, 3), dtype=np.uint64)
+        st = np.zeros((cap, 4), dtype=np.uint64)
         blocks = np.zeros(data.size // 4 + (sbs + 1) ** data.ndim, dtype=data.dtype)
         nst = C.c_size_t(0)
         sp = C.POINTER(C.c_size_t)

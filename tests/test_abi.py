@@ -153,8 +153,8 @@ def test_pyfalcons_config_validation():
 def test_pyfalcons_decompress_validation():
     with pytest.raises(TypeError):
         fa.falcons.decompress(np.zeros(10, np.int8), np.float32, (10,))
-    with pytest.raises(TypeError):
-        fa.falcons.decompress(np.zeros(10, np.uint8), np.int32, (10,))
+    with pytest.raises(TypeError):  # pyfalcons takes float32, float64, int32 and int64 (falcons.pyx:22-28)
+        fa.falcons.decompress(np.zeros(10, np.uint8), np.int16, (10,))
     with pytest.raises(ValueError):
         fa.falcons.decompress(np.zeros(10, np.uint8), np.float32, (0, 3))
     g = fa.falconsConfig(10, 10, 10)

@@ -60,8 +60,11 @@ def test_capsule_fields_read_without_copy():
     v.release()  # the deleter runs once
     with pytest.raises(ValueError):
         fa.DLDeviceArray(a[:, ::2])
-    with pytest.raises(TypeError):
-        fa.DLDeviceArray(np.arange(6, dtype=np.int32))
+    with pytest.raises(TypeError):  # element types outside Falcons_FLOAT .. Falcons_INT64
+        fa.DLDeviceArray(np.arange(6, dtype=np.float16))
+    v = fa.DLDeviceArray(np.arange(6, dtype=np.int32))  # integer fields are compressed too
+    assert v.dtype == np.int32
+    v.release()
 
 
 def test_device_array_exports_a_rocm_capsule():
