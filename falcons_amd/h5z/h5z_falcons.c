@@ -17,9 +17,8 @@
  *  - the compress buffer is 2 x the chunk bytes as in the reference, or the stream's size bound
  *    when that is larger (the reference throws through the C library there, for chunks under about
  *    2 KB);
- *  - integer data types, which this library does not compress, fail the filter (return 0, an HDF5
- *    error) where the reference compresses them on the CPU; an unknown type fails it too, where
- *    the reference calls exit;
+ *  - an unknown data type fails the filter (return 0, an HDF5 error), where the reference calls
+ *    exit;
  *  - one library context on GPU FALCONS_H5Z_DEVICE (default 0) serves every call of the process,
  *    under a lock.
  */
@@ -141,18 +140,14 @@ static size_t falcons_filter(unsigned int flags, size_t cd_nelmts, const unsigne
         return 0;
     }
     if (conf.num < 20) return nbytes;
-    int dtype;
-    size_t es;
-    if (conf.data_type == T_FLOAT) {
-        dtype = FZ_FLOAT;
-        es = 4;
-    } else if (conf.data_type == T_DOUBLE) {
-        dtype = FZ_DOUBLE;
-        es = 8;
-    } else {
-        push_error(__func__, __LINE__, H5E_CANTFILTER, "integer data types are not supported by this library");
+    /* the library numbers the element types as Falcons_FLOAT .. Falcons_INT64 */
+    static const size_t elem[10] = {4, 8, 1, 1, 2, 2, 4, 4, 8, 8};
+    if (conf.data_type < T_FLOAT || conf.data_type > T_INT64) {
+        push_error(__func__, __LINE__, H5E_CANTFILTER, "Unknown Datatype");
         return 0;
     }
+    const int dtype = conf.data_type;
+    const size_t es = elem[dtype];
 
     size_t ret = 0;
     pthread_mutex_lock(&g_lock);

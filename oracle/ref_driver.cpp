@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <stdexcept>
 #include <vector>
 
 #include "Falcons/encoder/RleFseEncoder.hpp"
@@ -147,6 +148,48 @@ long ref_sample(int dtype, int N, const size_t *dims, const void *data, size_t s
 #undef RS_CASE
     return -1;
 }
+
+// LinearQuantizer<T> for the integer element types (quantize_integral / recover_integral,
+// LinearQuantizer.hpp:190-248) and the integer interpolators (Interpolators.hpp:33-117).
+#define REF_INT(T, SFX)                                                                             \
+    int ref_quantize_##SFX(T data, T pred, double eb, T *out, int *unpred) {                       \
+        Falcons::LinearQuantizer<T> q(eb, 32768);                                                  \
+        q.set_eb(eb);                                                                              \
+        T d = data;                                                                                \
+        int code = q.quantize_and_overwrite(d, pred);                                              \
+        *out = d;                                                                                  \
+        *unpred = static_cast<int>(q.test_unpred().size());                                        \
+        return code;                                                                               \
+    }                                                                                              \
+    int ref_recover_##SFX(T pred, int code, double eb, T *out) {                                   \
+        Falcons::LinearQuantizer<T> q(eb, 32768);                                                  \
+        q.set_eb(eb);                                                                              \
+        try {                                                                                      \
+            *out = q.recover_pred(pred, code);                                                     \
+            return 1;                                                                              \
+        } catch (const std::runtime_error &) {                                                     \
+            return 0;                                                                              \
+        }                                                                                          \
+    }                                                                                              \
+    T ref_interp_##SFX(int kind, T a, T b, T c, T d) {                                             \
+        switch (kind) {                                                                            \
+            case 0: return Falcons::interp_linear(a, b);                                           \
+            case 1: return Falcons::interp_linear1(a, b);                                          \
+            case 2: return Falcons::interp_quad_1(a, b, c);                                        \
+            case 3: return Falcons::interp_quad_2(a, b, c);                                        \
+            case 5: return Falcons::interp_quad_3(a, b, c);                                        \
+            default: return Falcons::interp_cubic(a, b, c, d);                                     \
+        }                                                                                          \
+    }
+REF_INT(uint8_t, u8)
+REF_INT(int8_t, i8)
+REF_INT(uint16_t, u16)
+REF_INT(int16_t, i16)
+REF_INT(uint32_t, u32)
+REF_INT(int32_t, i32)
+REF_INT(uint64_t, u64)
+REF_INT(int64_t, i64)
+#undef REF_INT
 
 // Vendored FSE entry points, re-exported under stable names.
 size_t ref_fse_compress(void *dst, size_t cap, const void *src, size_t n) { return FSE_compress(dst, cap, src, n); }

@@ -12,12 +12,15 @@ that build on seeded inputs and stores inputs and outputs:
                  sha256 of the encoded bytes (regenerated bit-exactly by codes_from_recipe)
   quantizer.npz  LinearQuantizer::quantize_and_overwrite and recover (f32 and f64)
   interp.npz     interp_linear / linear1 / quad_1 / quad_2 / cubic (f32 and f64)
+  intquant.npz   LinearQuantizer<T>::quantize_and_overwrite / recover_pred and the interpolators for
+                 the eight integer element types (quantize_integral / recover_integral, wide-integer
+                 predictions; LinearQuantizer.hpp:190-248, Interpolators.hpp:33-117)
   sampler.json   profiling_block + sampleBlocks (utils/Sample.hpp:5-344) of the ALGO_INTERP_LORENZO
                  parameter search on integer-generated fields (recipes in golden_inputs.sampler_field):
                  the profiling branch, the profiled block starts and the sha256 of the sampled
                  blocks, at the sample block side the search picks for each field
 
-Run from the repository root: python tests/golden/make_golden.py [--only sampler]
+Run from the repository root: python tests/golden/make_golden.py [--only sampler|intquant]
 """
 import hashlib
 import json
@@ -92,6 +95,83 @@ def sampler_vectors(ref):
     return res
 
 
+INT_TYPES = ["u8", "i8", "u16", "i16", "u32", "i32", "u64", "i64"]
+
+
+def int_points(tag, count=1500):
+    """(data, pred, eb) triples for an integer type: near pairs at several bounds, far pairs, and the
+    type's limits, where distances overflow the narrower arithmetic and reconstructions leave the range."""
+    dt = np.dtype({"u": "uint", "i": "int"}[tag[0]] + tag[1:])
+    info = np.iinfo(dt)
+    rng = np.random.default_rng(100 + INT_TYPES.index(tag))
+    lo, hi = int(info.min), int(info.max)
+    def draw(k):
+        r = rng.integers(0, 2 ** 62, k, dtype=np.int64)
+        return np.array([lo + (int(x) % (hi - lo + 1)) for x in r], dtype=object)
+    base = draw(count)
+    delta = rng.integers(-20000, 20000, count)
+    pred = np.array([min(max(int(b) + int(d), lo), hi) for b, d in zip(base, delta)], dtype=object)
+    ebs = rng.choice([0.25, 0.5, 1.0, 1.5, 2.0, 3.7, 17.0, 1000.0, 1e6, 1e-3], count)
+    far_d, far_p = draw(300), draw(300)
+    lim = [lo, hi, lo + 1, hi - 1, 0, 0]
+    lim_d = np.array([lim[i % 6] for i in range(120)], dtype=object)
+    lim_p = np.array([lim[(i // 6) % 6] for i in range(120)], dtype=object)
+    data = np.concatenate([base, far_d, lim_d]).astype(dt)
+    pr = np.concatenate([pred, far_p, lim_p]).astype(dt)
+    eb = np.concatenate([ebs, rng.choice([0.5, 3.0, 1e9, 1e18], 300), rng.choice([0.5, 2.0, 1e15], 120)])
+    return data, pr, eb.astype(np.float64)
+
+
+def int_recover_args(tag, count=1200):
+    dt = np.dtype({"u": "uint", "i": "int"}[tag[0]] + tag[1:])
+    info = np.iinfo(dt)
+    rng = np.random.default_rng(200 + INT_TYPES.index(tag))
+    lo, hi = int(info.min), int(info.max)
+    r = rng.integers(0, 2 ** 62, count, dtype=np.int64)
+    pred = np.array([lo + (int(x) % (hi - lo + 1)) for x in r], dtype=object)
+    pred[:40] = [lo, hi] * 20
+    code = rng.integers(-32767, 32768, count)
+    code[:40] = rng.choice([-32767, -1, 1, 32767], 40)
+    eb = rng.choice([0.25, 0.5, 1.0, 3.7, 1e3, 1e12, 1e17], count)
+    return pred.astype(dt), code.astype(np.int32), eb.astype(np.float64)
+
+
+def int_interp_args(tag, count=800):
+    dt = np.dtype({"u": "uint", "i": "int"}[tag[0]] + tag[1:])
+    info = np.iinfo(dt)
+    rng = np.random.default_rng(300 + INT_TYPES.index(tag))
+    lo, hi = int(info.min), int(info.max)
+    r = rng.integers(0, 2 ** 62, (count, 4), dtype=np.int64)
+    a = np.array([[lo + (int(x) % (hi - lo + 1)) for x in row] for row in r], dtype=object)
+    a[:32] = np.array([[lo, hi, lo, hi], [hi, hi, hi, lo], [lo, lo, lo, hi], [hi, lo, hi, lo]] * 8, dtype=object)
+    return a.astype(dt)
+
+
+def intquant_vectors(ref):
+    out = {}
+    for tag in INT_TYPES:
+        d, p, e = int_points(tag)
+        codes = np.zeros(d.size, np.int32)
+        dec = np.zeros_like(d)
+        esc = np.zeros(d.size, np.int8)
+        for i in range(d.size):
+            codes[i], dec[i], esc[i] = ref.quantize_int(tag, d[i], p[i], e[i])
+        rp, rc, re = int_recover_args(tag)
+        rok = np.zeros(rp.size, np.int8)
+        rout = np.zeros_like(rp)
+        for i in range(rp.size):
+            rok[i], rout[i] = ref.recover_int(tag, rp[i], int(rc[i]), re[i])
+        ia = int_interp_args(tag)
+        iout = np.zeros((6, ia.shape[0]), ia.dtype)
+        for k in (0, 1, 2, 3, 4, 5):
+            for i in range(ia.shape[0]):
+                iout[k, i] = ref.interp_int(tag, k, *ia[i])
+        out.update({f"{tag}_ori": d, f"{tag}_pred": p, f"{tag}_eb": e, f"{tag}_code": codes, f"{tag}_dec": dec,
+                    f"{tag}_esc": esc, f"{tag}_rpred": rp, f"{tag}_rcode": rc, f"{tag}_reb": re, f"{tag}_rok": rok,
+                    f"{tag}_rout": rout, f"{tag}_iargs": ia, f"{tag}_iout": iout})
+    return out
+
+
 def fse_blocks():
     rng = np.random.default_rng(7)
     blocks = []
@@ -144,6 +224,10 @@ def interp_args(dtype, count=600):
 def main():
     ref = Ref()
     out = {}
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "intquant":
+        np.savez_compressed(os.path.join(HERE, "intquant.npz"), **intquant_vectors(ref))
+        print(json.dumps({"intquant": INT_TYPES}))
+        return
     samp = sampler_vectors(ref)
     with open(os.path.join(HERE, "sampler.json"), "w") as f:
         json.dump(samp, f, indent=1)

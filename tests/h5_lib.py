@@ -75,7 +75,9 @@ class H5:
         self.DCPL = g("H5P_CLS_DATASET_CREATE_ID_g")
         self.TYPES = {np.dtype(np.float32): g("H5T_NATIVE_FLOAT_g"), np.dtype(np.float64): g("H5T_NATIVE_DOUBLE_g"),
                       np.dtype(np.int16): g("H5T_NATIVE_SHORT_g"), np.dtype(np.uint8): g("H5T_NATIVE_UCHAR_g"),
-                      np.dtype(np.int32): g("H5T_NATIVE_INT_g")}
+                      np.dtype(np.int32): g("H5T_NATIVE_INT_g"), np.dtype(np.uint16): g("H5T_NATIVE_USHORT_g"),
+                      np.dtype(np.int64): g("H5T_NATIVE_LLONG_g"), np.dtype(np.uint64): g("H5T_NATIVE_ULLONG_g"),
+                      np.dtype(np.int8): g("H5T_NATIVE_SCHAR_g"), np.dtype(np.uint32): g("H5T_NATIVE_UINT_g")}
 
     @staticmethod
     def _dims(shape):

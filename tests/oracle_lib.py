@@ -21,6 +21,10 @@ DTYPE_CODE = {np.dtype(np.float32): 0, np.dtype(np.float64): 1, np.dtype(np.uint
 def dtype_code(dt):
     return DTYPE_CODE[np.dtype(dt)]
 
+
+INT_CTYPES = {"u8": C.c_uint8, "i8": C.c_int8, "u16": C.c_uint16, "i16": C.c_int16, "u32": C.c_uint32,
+              "i32": C.c_int32, "u64": C.c_uint64, "i64": C.c_int64}
+
 _u8p = C.POINTER(C.c_uint8)
 _i16p = C.POINTER(C.c_int16)
 _szp = C.POINTER(C.c_size_t)
@@ -190,6 +194,23 @@ class Oracle:
             setattr(c, k, v)
         return c
 
+    def gen_compress(self, data, cubic, direction, anchor, alpha, beta, eb):
+        """The generic traversal alone (fo_gen_compress): (codes, unpredictables, anchor used)."""
+        data = np.ascontiguousarray(data).copy()
+        dims = (C.c_size_t * 4)(*data.shape)
+        codes = np.zeros(data.size + 16, dtype=np.int16)
+        un = np.zeros(data.size + 16, dtype=data.dtype)
+        nq, nun, au = C.c_size_t(0), C.c_size_t(0), C.c_size_t(0)
+        self.L.fo_gen_compress.restype = C.c_int
+        self.L.fo_gen_compress.argtypes = [C.c_int, C.c_int, _szp, C.c_int, C.c_int, C.c_long, C.c_double,
+                                           C.c_double, C.c_double, C.c_void_p, _i16p, _szp, C.c_void_p, C.c_size_t,
+                                           _szp, _szp]
+        rc = self.L.fo_gen_compress(dtype_code(data.dtype), data.ndim, dims, cubic, direction, anchor, alpha, beta,
+                                    eb, data.ctypes.data, _ptr(codes, _i16p), C.byref(nq), un.ctypes.data,
+                                    un.size, C.byref(nun), C.byref(au))
+        assert rc == 0, rc
+        return codes[: nq.value], un[: nun.value], au.value
+
     def tune_sample(self, data, conf):
         """The parameter search's sampler: (block side, profiling branch, profiled starts, selected
         starts), starts as (count, 3) arrays; None when the field is not tuned."""
@@ -276,6 +297,34 @@ class Ref:
         L.ref_sample.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_size_t), C.c_void_p, C.c_size_t, C.c_double,
                                  C.c_double, C.c_int, C.POINTER(C.c_size_t), C.c_size_t, C.POINTER(C.c_size_t),
                                  C.c_void_p, C.c_size_t]
+
+    def _int_fns(self, tag):
+        ct = INT_CTYPES[tag]
+        q = getattr(self.L, f"ref_quantize_{tag}")
+        q.restype, q.argtypes = C.c_int, [ct, ct, C.c_double, C.POINTER(ct), C.POINTER(C.c_int)]
+        r = getattr(self.L, f"ref_recover_{tag}")
+        r.restype, r.argtypes = C.c_int, [ct, C.c_int, C.c_double, C.POINTER(ct)]
+        i = getattr(self.L, f"ref_interp_{tag}")
+        i.restype, i.argtypes = ct, [C.c_int, ct, ct, ct, ct]
+        return ct, q, r, i
+
+    def quantize_int(self, tag, data, pred, eb):
+        """LinearQuantizer<T>::quantize_and_overwrite: (code, overwritten value, escaped)."""
+        ct, q, _, _ = self._int_fns(tag)
+        out, un = ct(0), C.c_int(0)
+        code = q(ct(int(data)), ct(int(pred)), float(eb), C.byref(out), C.byref(un))
+        return code, out.value, un.value
+
+    def recover_int(self, tag, pred, code, eb):
+        """LinearQuantizer<T>::recover_pred: (fits, value)."""
+        ct, _, r, _ = self._int_fns(tag)
+        out = ct(0)
+        ok = r(ct(int(pred)), int(code), float(eb), C.byref(out))
+        return ok, out.value if ok else 0
+
+    def interp_int(self, tag, kind, a, b, c, d):
+        ct, _, _, i = self._int_fns(tag)
+        return i(kind, ct(int(a)), ct(int(b)), ct(int(c)), ct(int(d)))
 
     def sample(self, data, sbs, abseb, rate=0.005, profiling=-1):
         """profiling_block + sampleBlocks of the reference (Sample.hpp): (profiling branch, profiled

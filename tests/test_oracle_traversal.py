@@ -227,3 +227,38 @@ def test_compare_class_placement(orc, shape, direction, cubic, simd):
         assert (pc == 1).any()  # vector lanes accept them
     else:
         assert (pc == -32768).mean() > 0.6  # the scalar build escapes them
+
+
+GEN_SHAPES = [(1,), (2,), (5,), (17,), (100,), (5, 5), (9, 16), (17, 33), (2, 3, 4), (5, 6, 7), (9, 10, 11),
+              (2, 3, 4, 5), (5, 6, 7, 8), (9, 4, 6, 17), (17, 5, 5, 9)]
+
+
+def _gen_dirs(n):
+    return [0] if n == 1 else ([0, 1] if n == 2 else ([0, 3, 5] if n == 3 else [0, 7, 16, 23]))
+
+
+GEN_CASES = [(s, cub, d, a) for s in GEN_SHAPES for cub in (0, 1) for d in _gen_dirs(len(s)) for a in (0, 4)]
+
+
+@pytest.mark.parametrize("shape,cubic,direction,anchor", GEN_CASES,
+                         ids=[f"{c[0]}-c{c[1]}-d{c[2]}-a{c[3]}" for c in GEN_CASES])
+def test_generic_code_order_matches_spec(orc, shape, cubic, direction, anchor):
+    """The generic traversal (integer elements and 4-D fields): with widely spaced distinct int64 values
+    and a bound far below one unit every point escapes, so the unpredictable list spells out the order;
+    it must equal the anchors followed by traversal_spec.generic_order()."""
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(n * 11 + cubic * 5 + direction + anchor)
+    data = (rng.permutation(n).astype(np.int64) << 20) + rng.integers(1, 1 << 19, n)
+    data = data.reshape(shape)
+    codes, unpred, a_used = orc.gen_compress(data, cubic, direction, anchor, 1.0, 1.0, 1e-300)
+    want, a_spec = spec.generic_order(shape, cubic, direction, anchor)
+    assert a_used == a_spec
+    anc = spec.anchors(shape, a_spec) if a_spec else []
+    assert len(codes) == n - len(anc)
+    assert np.all(codes == -32768)
+    flat = data.reshape(-1)
+    assert np.array_equal(unpred[: len(anc)], flat[anc])
+    pos = {int(v): i for i, v in enumerate(flat)}
+    got = [pos[int(v)] for v in unpred[len(anc):]]
+    assert got == want
+    assert sorted(got + anc) == list(range(n))

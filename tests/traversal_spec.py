@@ -123,3 +123,71 @@ def anchors(shape, a):
     for i in range(N - 2, -1, -1):
         strides_el[i] = strides_el[i + 1] * shape[i + 1]
     return [sum(c * s for c, s in zip(cs, strides_el)) for cs in itertools.product(*[range(0, d, a) for d in shape])]
+
+
+def _line_order(n, cubic):
+    """Grid indices interpolation_1d codes on one line of n grid points (IDH:374-419): odd indices
+    upward (the last of an even count by its own rule); cubic with n >= 5: 3, 5, ... while i + 3 < n,
+    then 1, then the first i with i + 3 >= n, then n - 1 for even n."""
+    if n <= 1:
+        return []
+    if not cubic or n < 5:
+        return list(range(1, n - 1, 2)) + ([n - 1] if n % 2 == 0 else [])
+    bulk = list(range(3, n - 3, 2))
+    last = bulk[-1] + 2 if bulk else 3
+    return bulk + [1, last] + ([n - 1] if n % 2 == 0 else [])
+
+
+def _fdf_targets(n, cubic):
+    """Pass-axis index sets of the foreach boxes interpolation_1d_fastest_dim_first runs, in order
+    (IDH:436-528): linear: odd 1..n-2, then n-1 for even n; cubic: odd 3..n-4, then the boundary rows
+    1, n-2 (odd n > 3), n-3 (even n > 4), n-1 (even n > 2), one box each."""
+    if n <= 1:
+        return []
+    if not cubic:
+        return [list(range(1, n - 1, 2))] + ([[n - 1]] if n % 2 == 0 else [])
+    boxes = [list(range(3, n - 3, 2))]
+    bl = [1]
+    if n % 2 == 1 and n > 3:
+        bl.append(n - 2)
+    if n % 2 == 0 and n > 4:
+        bl.append(n - 3)
+    if n % 2 == 0 and n > 2:
+        bl.append(n - 1)
+    return boxes + [[b] for b in bl]
+
+
+def generic_order(shape, cubic, direction, anchor):
+    """Code order of the generic traversal (integer elements, 4-D fields; IDH:2314-2519): per level,
+    1-D one line; 2-D every 2*stride-th line along dims[0], then every stride-th line along dims[1];
+    3-D / 4-D foreach boxes along dims[0], dims[1], ... with the axes not yet passed at step 2*stride
+    and those passed at step stride, every box looping axis 0 outermost. dims is the direction-th
+    permutation in lexicographic order."""
+    import itertools
+    N = len(shape)
+    lv, a = levels_and_anchor(shape, anchor)
+    el = [1] * N
+    for i in range(N - 2, -1, -1):
+        el[i] = el[i + 1] * shape[i + 1]
+    dims = list(itertools.permutations(range(N)))[direction]
+    out = [] if a else [0]
+    for level in range(lv, 0, -1):
+        st = 1 << (level - 1)
+        if N == 1:
+            out += [g * st for g in _line_order(_grid(shape[0], st), cubic)]
+        elif N == 2:
+            d0, d1 = dims
+            for j in range(0, shape[d1], 2 * st):
+                out += [j * el[d1] + g * st * el[d0] for g in _line_order(_grid(shape[d0], st), cubic)]
+            for i in range(0, shape[d0], st):
+                out += [i * el[d0] + g * st * el[d1] for g in _line_order(_grid(shape[d1], st), cubic)]
+        else:
+            step = {d: 2 * st for d in range(N)}
+            for k, pd in enumerate(dims):
+                if k > 0:
+                    step[dims[k - 1]] = st
+                for targets in _fdf_targets(_grid(shape[pd], st), cubic):
+                    axes = [[g * st for g in targets] if d == pd else list(range(0, shape[d], step[d]))
+                            for d in range(N)]
+                    out += [sum(c * e for c, e in zip(cs, el)) for cs in itertools.product(*axes)]
+    return out, a
