@@ -618,7 +618,8 @@ size_t gpu_compress(fz_ctx* ctx, DevArena& ws, fz_config& c, int dtype, void* d_
     const uint64_t nq = ncodes;
     FZ_HIP(hipMemcpyAsync(d_pre + head + nun * es, &nq, 8, hipMemcpyHostToDevice, st));
     EntropyStats es_stats;
-    const size_t enc_len = entropy_encode(codes, ncodes, d_pre + head + nun * es + 8, enc_cap, ws, st, &es_stats);
+    const size_t enc_len =
+        entropy_encode(codes, ncodes, d_pre + head + nun * es + 8, enc_cap, ws, st, &es_stats, ctx->aux);
     const double tdev = tm.stop();
     const size_t total = head + nun * es + 8 + enc_len;
     if (pre_out) *pre_out = d_pre;

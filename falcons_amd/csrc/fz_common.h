@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -71,14 +72,22 @@ class LaunchProfiler {
             (void)hipEventDestroy(r.b);
         }
     }
-    void begin(const char* name, uint64_t bytes, hipStream_t st) {
+    // A call may launch from two host threads (work on its second stream); records are kept by
+    // index under a lock.
+    size_t begin(const char* name, uint64_t bytes, hipStream_t st) {
+        std::lock_guard<std::mutex> g(mu_);
         Rec r{name, bytes, take(), take()};
         FZ_HIP(hipEventRecord(r.a, st));
         pending_.push_back(r);
+        return pending_.size() - 1;
     }
-    void end(hipStream_t st) { FZ_HIP(hipEventRecord(pending_.back().b, st)); }
-    // Call after the stream has finished the recorded work.
+    void end(size_t idx, hipStream_t st) {
+        std::lock_guard<std::mutex> g(mu_);
+        FZ_HIP(hipEventRecord(pending_[idx].b, st));
+    }
+    // Call after the streams have finished the recorded work.
     void collect() {
+        std::lock_guard<std::mutex> g(mu_);
         for (const Rec& r : pending_) {
             float ms = 0;
             FZ_HIP(hipEventElapsedTime(&ms, r.a, r.b));
@@ -113,6 +122,7 @@ class LaunchProfiler {
         pool_.pop_back();
         return e;
     }
+    std::mutex mu_;
     std::vector<hipEvent_t> pool_;
     std::vector<Rec> pending_;
     std::map<std::string, Total> totals_;
@@ -123,11 +133,13 @@ extern thread_local LaunchProfiler* g_prof;
 
 struct ProfScope {
     hipStream_t st;
-    ProfScope(const char* name, uint64_t bytes, hipStream_t s) : st(s) {
-        if (g_prof) g_prof->begin(name, bytes, st);
+    LaunchProfiler* prof;
+    size_t idx = 0;
+    ProfScope(const char* name, uint64_t bytes, hipStream_t s) : st(s), prof(g_prof) {
+        if (prof) idx = prof->begin(name, bytes, st);
     }
     ~ProfScope() {
-        if (g_prof) g_prof->end(st);
+        if (prof) prof->end(idx, st);
     }
 };
 

@@ -18,6 +18,8 @@
 
 #include <algorithm>
 #include <array>
+#include <exception>
+#include <thread>
 
 #include "fz_common.h"
 #include "fz_fse.h"
@@ -2309,6 +2311,49 @@ static void escape_streams(const int16_t* ovf, const std::vector<uint64_t>& off,
     sync(st);  // heads live on this stack frame
 }
 
+// An upper bound of the device workspace escape_streams takes for n escaped values in the
+// segments bounded by off (every take rounded up to 256 bytes).
+static size_t escape_workspace_bound(const std::vector<uint64_t>& off, uint64_t S) {
+    const uint64_t n = off[S];
+    size_t bytes = 0;
+    auto add = [&](size_t b) { bytes += (b + 255) & ~size_t(255); };
+    add(n + 1);
+    add(8 * (n + 1));
+    add(8 * (n + 1));
+    size_t scan = 0;
+    FZ_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                            (int64_t)std::max<uint64_t>(n, 1)));
+    add(scan);
+    uint64_t nb = 0;
+    for (uint64_t g = 0; g < S; g++) {
+        const uint64_t m = off[g + 1] - off[g];
+        const uint64_t fse_cap = m + 5 * ((m + kFblk - 1) / kFblk + 1) + 64, vbp = 2 * m + 1;  // <= 16 bits a value
+        add(std::max<uint64_t>(1 + 2 * m, 17 + fse_cap + vbp) + 8);
+        add((vbp / 4 + 2) * 4);
+        nb += (m + kFblk - 1) / kFblk;
+    }
+    add(nb * (kFblk + 1024));
+    add(8 * nb);
+    add(sizeof(FseBlk) * nb);
+    return bytes + (1u << 20);
+}
+
+// escape_streams on the second stream, driven by a helper thread (its layout takes host round
+// trips) with a workspace of its own carved out of the call's; joined, and its error rethrown,
+// before the encoder output is assembled.
+struct EscapeWorker {
+    std::thread th;
+    std::exception_ptr err;
+    DevArena sub;
+    ~EscapeWorker() {
+        if (th.joinable()) th.join();
+    }
+    void join() {
+        if (th.joinable()) th.join();
+        if (err) std::rethrow_exception(err);
+    }
+};
+
 // below[q] = the number of sorted positions pos[0..n) under ends[q] (one lane per end).
 __global__ void k_count_below(const uint64_t* pos, uint64_t n, const uint64_t* ends, int nq, uint64_t* below) {
     for (int q = threadIdx.x; q < nq; q += blockDim.x) {
@@ -2323,7 +2368,7 @@ __global__ void k_count_below(const uint64_t* pos, uint64_t n, const uint64_t* e
 }
 
 static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint64_t cap, DevArena& ws,
-                             hipStream_t st) {
+                             hipStream_t st, const AuxStream& aux) {
     uint64_t S = n / kO1Seg;
     if (S < 1) S = 1;
     if (S > 16) S = 16;
@@ -2392,6 +2437,30 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
         }
     }
     ovf_off[S] = e;
+    // the escape side streams need only the escaped values and their segment offsets: with a
+    // second stream they are coded there, beside the chains
+    std::vector<uint64_t> holen(S);
+    std::vector<uint8_t*> oblock(S);
+    EscapeWorker ew;
+    if (aux.stream) {
+        const size_t need = escape_workspace_bound(ovf_off, S);
+        ew.sub.bind(ws.take<uint8_t>(need), need);
+        FZ_HIP(hipEventRecord(aux.fork, st));
+        FZ_HIP(hipStreamWaitEvent(aux.stream, aux.fork, 0));
+        int dev = 0;
+        FZ_HIP(hipGetDevice(&dev));
+        LaunchProfiler* prof = g_prof;
+        ew.th = std::thread([&ew, &ovf, &ovf_off, &holen, &oblock, &aux, S, dev, prof]() {
+            try {
+                FZ_HIP(hipSetDevice(dev));
+                g_prof = prof;
+                escape_streams(ovf, ovf_off, S, holen, oblock, ew.sub, aux.stream);
+            } catch (...) {
+                ew.err = std::current_exception();
+            }
+            g_prof = nullptr;
+        });
+    }
     FZ_LAUNCH("o1_tables", 0, k_o1_tables, dim3(kO1Ctx, (unsigned)S), dim3(256), 0, st, cnt, nf, ct);
     FZ_HIP(hipGetLastError());
     const int nch = (int)(S * kO1K);
@@ -2476,9 +2545,8 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
     }
     FZ_HIP(hipGetLastError());
     // escape side streams, one block per segment
-    std::vector<uint64_t> holen(S);
-    std::vector<uint8_t*> oblock(S);
-    escape_streams(ovf, ovf_off, S, holen, oblock, ws, st);
+    if (aux.stream) ew.join();
+    else escape_streams(ovf, ovf_off, S, holen, oblock, ws, st);
     std::vector<uint64_t> hblen(nch);
     d2h(hblen.data(), blen, nch, st);
     // assemble: [u64 S] then per segment [n][K][blen x4][novf x4][nf][streams][ovf block]
@@ -2519,7 +2587,7 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
 }
 
 size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out_cap, DevArena& ws, hipStream_t st,
-                      EntropyStats* stats) {
+                      EntropyStats* stats, const AuxStream& aux) {
     need_room(5, out_cap);
     // preprocess_encode: 256 windows of 32 codes (RleFse:850-895)
     uint64_t NW = 256, WL = 32;
@@ -2547,7 +2615,7 @@ size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out
         FZ_HIP(hipGetLastError());
         bins = d;
     }
-    const size_t body = (method == 1) ? encode_method1(bins, n, out + 5, out_cap - 5, ws, st)
+    const size_t body = (method == 1) ? encode_method1(bins, n, out + 5, out_cap - 5, ws, st, aux)
                                       : encode_method0(bins, n, out + 5, out_cap - 5, ws, st);
     return 5 + body;
 }

@@ -132,20 +132,21 @@ struct EntropyStats {
 // a symbol and 2 side-stream bytes on an escaped code; RLE+FSE at most 3 bytes per code; plus the
 // per-segment and per-block headers and tables.
 inline size_t entropy_encode_bound(uint64_t n) { return (size_t)n * 4 + (1u << 20); }
-// Encodes n codes (device) into out (device, capacity out_cap; std::length_error, before any write
-// past it, when the output would not fit). Returns the byte count of
-// [u8 method|delta<<7][u16 lo][u16 mode][payload] (RleFseEncoder::save + ::encode).
-size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out_cap, DevArena& ws, hipStream_t st,
-                      EntropyStats* stats);
-// Decodes the encoder bytes (host copy `host`, device copy `dev`, length len) into n codes (device).
 // A second stream for work that can run beside the main one (null: run on the main stream), with the
 // two events that fork it from and join it back to the main stream.
 struct AuxStream {
     hipStream_t stream = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
 };
+// Encodes n codes (device) into out (device, capacity out_cap; std::length_error, before any write
+// past it, when the output would not fit). Returns the byte count of
+// [u8 method|delta<<7][u16 lo][u16 mode][payload] (RleFseEncoder::save + ::encode). With a second
+// stream, the order-1 escape side streams are coded on it, beside the chains.
+size_t entropy_encode(const int16_t* codes, uint64_t n, uint8_t* out, size_t out_cap, DevArena& ws, hipStream_t st,
+                      EntropyStats* stats, const AuxStream& aux = AuxStream{});
 // Host-side check of an entropy stream's headers and framing (throws std::invalid_argument).
 void entropy_check(const uint8_t* host, size_t len, uint64_t n);
+// Decodes the encoder bytes (host copy `host`, device copy `dev`, length len) into n codes (device).
 void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
                     hipStream_t st, const AuxStream& aux);
 
