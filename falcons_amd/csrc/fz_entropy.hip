@@ -64,6 +64,8 @@ __device__ __forceinline__ FZ_LDS T* lds(T* p) { return (FZ_LDS T*)p; }
 // its own L2. Kernels whose neighbouring workgroups share tables (the tANS tables of one segment)
 // renumber their workgroups so that every XCD takes one contiguous range: XCD x gets the logical
 // blocks x*(n/8) + min(x, n%8) onward, in order.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ uint32_t xcd_block() {
     const uint32_t n = gridDim.x, b = blockIdx.x, x = b & 7u, k = b >> 3;
     return x * (n >> 3) + min(x, n & 7u) + k;
@@ -958,26 +960,97 @@ __device__ inline uint32_t o1_step(uint32_t& state, const CTable& ct, unsigned s
     return nbOut;
 }
 
+// The per-symbol arrays of one segment's 15 encoding tables, staged in LDS by a workgroup whose
+// pieces lie in that segment (a workgroup's 256 pieces span at most a segment boundary; pieces of
+// another segment read their tables from device memory). The state tables stay in device memory.
+struct O1EncLds {
+    uint32_t dnb[kO1Ctx * kO1Sym];
+    int32_t dfs[kO1Ctx * kO1Sym];
+};
+// Stages the arrays of the segment of the workgroup's first piece; returns that segment. Every
+// thread of the workgroup calls it.
+__device__ inline uint64_t o1_stage_tables(O1EncLds& T, const O1Chains& C, uint64_t npieces, const CTable* ct_all,
+                                           uint64_t first_pid) {
+    const uint64_t seg = C.c[4 * o1_piece_chain(C, min(first_pid, npieces - 1)) + 3];
+    const CTable* ct = ct_all + seg * kO1Ctx;
+    for (uint32_t u = threadIdx.x; u < kO1Ctx * kO1Sym; u += blockDim.x) {
+        T.dnb[u] = ct[u / kO1Sym].deltaNbBits[u % kO1Sym];
+        T.dfs[u] = ct[u / kO1Sym].deltaFindState[u % kO1Sym];
+    }
+    __syncthreads();
+    return seg;
+}
+// One encoder step in context cx: per-symbol values from LDS (staged) or device memory.
+__device__ __forceinline__ uint32_t o1_step_t(uint32_t& state, const CTable* ct, const FZ_LDS uint32_t* dnb,
+                                              const FZ_LDS int32_t* dfs, bool staged, int cx, unsigned sym) {
+    const uint32_t u = (uint32_t)cx * kO1Sym + sym;
+    const uint32_t d = staged ? dnb[u] : ct[cx].deltaNbBits[sym];
+    const int32_t f = staged ? dfs[u] : ct[cx].deltaFindState[sym];
+    const uint32_t nbOut = (uint32_t)(((uint64_t)state + d) >> 16);
+    state = ct[cx].stateTable[(state >> nbOut) + f];
+    return nbOut;
+}
+
+// Runs f(i, code, previous code) for the steps i = s_hi down to s_lo of a piece (previous code 0 at
+// the chain's first symbol lo), stopping early when f returns false. The codes come in aligned
+// 16-byte loads, eight at a time, the next eight in flight: one load per eight steps instead of two
+// scattered 2-byte loads per step (the lanes of a wave run pieces 2 KB apart). bins is 16-byte
+// aligned and its allocation extends to the next multiple of 16 bytes.
+template <typename F>
+__device__ __forceinline__ void o1_piece_codes(const int16_t* bins, uint64_t lo, uint64_t s_hi, uint64_t s_lo, F&& f) {
+    const u32x4* b4 = reinterpret_cast<const u32x4*>(bins);
+    uint64_t c = s_hi >> 3;
+    const uint64_t c_lo = s_lo >> 3;
+    u32x4 cur = b4[c];
+    u32x4 nxt = c > 0 ? b4[c - 1] : u32x4{0u, 0u, 0u, 0u};
+    for (;;) {
+        int v[9];  // v[k + 1]: code 8c + k; v[0]: code 8c - 1
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[2 * k + 1] = (int)(int16_t)(cur[k] & 0xFFFFu);
+            v[2 * k + 2] = (int)(int16_t)(cur[k] >> 16);
+        }
+        v[0] = (int)(int16_t)(nxt[3] >> 16);
+        const uint64_t base = 8 * c;
+#pragma unroll
+        for (int k = 7; k >= 0; k--) {
+            const uint64_t i = base + (uint64_t)k;
+            if (i > s_hi || i < s_lo) continue;
+            if (!f(i, v[k + 1], i > lo ? v[k] : 0)) return;
+        }
+        if (c == c_lo) return;
+        c--;
+        cur = nxt;
+        nxt = c > 0 ? b4[c - 1] : u32x4{0u, 0u, 0u, 0u};
+    }
+}
+
 __global__ void __launch_bounds__(256) k_o1_spec(const int16_t* bins, O1Chains C, uint64_t npieces,
                                                  const CTable* ct_all, uint16_t* slog, uint32_t* end_state,
                                                  uint64_t* nbits) {
-    const uint64_t pid = (uint64_t)xcd_block() * blockDim.x + threadIdx.x;
+    __shared__ O1EncLds T;
+    const uint64_t first = (uint64_t)xcd_block() * blockDim.x;
+    const uint64_t wseg = o1_stage_tables(T, C, npieces, ct_all, first);
+    const uint64_t pid = first + threadIdx.x;
     if (pid >= npieces) return;
     const int q = o1_piece_chain(C, pid);
     const uint64_t lo = C.c[4 * q + 1], hi = C.c[4 * q + 2];
     const CTable* ct = ct_all + C.c[4 * q + 3] * kO1Ctx;
+    const bool staged = C.c[4 * q + 3] == wseg;
     const uint64_t p = pid - C.pfirst[q];
     uint64_t s_hi, s_lo;
     o1_piece_range(lo, hi, p, s_hi, s_lo);
     // piece 0 starts from the true seed; the others from the seed their own first symbol would give
     uint32_t st = o1_seed_state(bins, lo, p == 0 ? hi - 1 : s_hi, ct);
     uint64_t bits = 0;
+    const FZ_LDS uint32_t* dnb = lds(T.dnb);
+    const FZ_LDS int32_t* dfs = lds(T.dfs);
     // the log is step-major (step j of every piece side by side), so a wave's writes are coalesced
-    for (uint64_t i = s_hi + 1; i-- > s_lo;) {
+    o1_piece_codes(bins, lo, s_hi, s_lo, [&](uint64_t i, int code, int pc) {
         slog[(s_hi - i) * npieces + pid] = (uint16_t)st;
-        const int pc = (i > lo) ? bins[i - 1] : 0;
-        bits += o1_step(st, ct[o1_ctx(pc)], o1_sym(bins[i]));
-    }
+        bits += o1_step_t(st, ct, dnb, dfs, staged, o1_ctx(pc), (unsigned)o1_sym(code));
+        return true;
+    });
     end_state[pid] = st;
     nbits[pid] = bits;
 }
@@ -994,7 +1067,10 @@ __global__ void __launch_bounds__(256) k_o1_fix(const int16_t* bins, O1Chains C,
                                                 const uint64_t* spec_bits, const uint32_t* end_prev,
                                                 uint32_t* end_next, uint64_t* nbits, const uint8_t* changed_prev,
                                                 uint8_t* changed_next, unsigned int* any_changed) {
-    const uint64_t pid = (uint64_t)xcd_block() * blockDim.x + threadIdx.x;
+    __shared__ O1EncLds T;
+    const uint64_t first = (uint64_t)xcd_block() * blockDim.x;
+    const uint64_t wseg = o1_stage_tables(T, C, npieces, ct_all, first);
+    const uint64_t pid = first + threadIdx.x;
     if (pid >= npieces) return;
     const int q = o1_piece_chain(C, pid);
     const uint64_t p = pid - C.pfirst[q];
@@ -1003,21 +1079,27 @@ __global__ void __launch_bounds__(256) k_o1_fix(const int16_t* bins, O1Chains C,
     if (p > 0 && (changed_prev == nullptr || changed_prev[pid - 1])) {
         const uint64_t lo = C.c[4 * q + 1], hi = C.c[4 * q + 2];
         const CTable* ct = ct_all + C.c[4 * q + 3] * kO1Ctx;
+        const bool staged = C.c[4 * q + 3] == wseg;
+        const FZ_LDS uint32_t* dnb = lds(T.dnb);
+        const FZ_LDS int32_t* dfs = lds(T.dfs);
         uint64_t s_hi, s_lo;
         o1_piece_range(lo, hi, p, s_hi, s_lo);
         uint32_t st = end_prev[pid - 1];
         int64_t delta = 0;
         bool met = false;
-        for (uint64_t i = s_hi + 1; i-- > s_lo;) {
+        o1_piece_codes(bins, lo, s_hi, s_lo, [&](uint64_t i, int code, int pc) {
             const uint32_t logged = slog[(s_hi - i) * npieces + pid];
-            if (st == logged) { met = true; break; }
-            const int pc = (i > lo) ? bins[i - 1] : 0;
-            const CTable& t = ct[o1_ctx(pc)];
-            const unsigned sym = o1_sym(bins[i]);
+            if (st == logged) {
+                met = true;
+                return false;
+            }
+            const int cx = o1_ctx(pc);
+            const unsigned sym = (unsigned)o1_sym(code);
             uint32_t ls = logged;
-            delta -= (int64_t)o1_step(ls, t, sym);
-            delta += (int64_t)o1_step(st, t, sym);
-        }
+            delta -= (int64_t)o1_step_t(ls, ct, dnb, dfs, staged, cx, sym);
+            delta += (int64_t)o1_step_t(st, ct, dnb, dfs, staged, cx, sym);
+            return true;
+        });
         nbits[pid] = (uint64_t)((int64_t)spec_bits[pid] + delta);
         const uint32_t e = met ? spec_end[pid] : st;
         if (e != end) {
@@ -1087,25 +1169,29 @@ __device__ inline void ww_flush(WordWriter& W) {
 __global__ void __launch_bounds__(256) k_o1_emit(const int16_t* bins, O1Chains C, uint64_t npieces,
                                                  const CTable* ct_all, const uint32_t* end_state,
                                                  const uint64_t* bitoff, uint32_t* words, uint64_t stride_words) {
-    const uint64_t pid = (uint64_t)xcd_block() * blockDim.x + threadIdx.x;
+    __shared__ O1EncLds T;
+    const uint64_t first = (uint64_t)xcd_block() * blockDim.x;
+    const uint64_t wseg = o1_stage_tables(T, C, npieces, ct_all, first);
+    const uint64_t pid = first + threadIdx.x;
     if (pid >= npieces) return;
     const int q = o1_piece_chain(C, pid);
     const uint64_t lo = C.c[4 * q + 1], hi = C.c[4 * q + 2];
     const CTable* ct = ct_all + C.c[4 * q + 3] * kO1Ctx;
+    const bool staged = C.c[4 * q + 3] == wseg;
+    const FZ_LDS uint32_t* dnb = lds(T.dnb);
+    const FZ_LDS int32_t* dfs = lds(T.dfs);
     const uint64_t p = pid - C.pfirst[q];
     uint64_t s_hi, s_lo;
     o1_piece_range(lo, hi, p, s_hi, s_lo);
     uint32_t st = p == 0 ? o1_seed_state(bins, lo, hi - 1, ct) : end_state[pid - 1];
     const uint64_t b0 = bitoff[pid];
     WordWriter W{words + q * stride_words, b0 >> 5, 0, (uint32_t)(b0 & 31), true};
-    for (uint64_t i = s_hi + 1; i-- > s_lo;) {
-        const int pc = (i > lo) ? bins[i - 1] : 0;
-        const CTable& t = ct[o1_ctx(pc)];
-        const unsigned sym = o1_sym(bins[i]);
-        const uint32_t nbOut = (uint32_t)(((uint64_t)st + t.deltaNbBits[sym]) >> 16);
-        ww_put(W, st, nbOut);
-        st = t.stateTable[(st >> nbOut) + t.deltaFindState[sym]];
-    }
+    o1_piece_codes(bins, lo, s_hi, s_lo, [&](uint64_t, int code, int pc) {
+        const uint32_t st0 = st;
+        const uint32_t nbOut = o1_step_t(st, ct, dnb, dfs, staged, o1_ctx(pc), (unsigned)o1_sym(code));
+        ww_put(W, st0, nbOut);
+        return true;
+    });
     ww_flush(W);
 }
 
@@ -1312,7 +1398,6 @@ __device__ __forceinline__ uint64_t seq_decode(SeqState& S, const FZ_LDS uint8_t
 // block; blocks that cross the chain's first or last byte go byte by byte, and bytes outside
 // [0, len) read as zero). Returns w0: win[i] holds chain byte w0 + i. win must hold
 // hi - lo + kWinPad + 32 bytes.
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ int64_t stage_window(FZ_LDS uint8_t* win, const uint8_t* cb, int64_t len, int64_t lo,
                                                 int64_t hi) {
     const int64_t s0 = lo - kWinPad;
@@ -2369,6 +2454,9 @@ __global__ void k_count_below(const uint64_t* pos, uint64_t n, const uint64_t* e
 
 static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint64_t cap, DevArena& ws,
                              hipStream_t st, const AuxStream& aux) {
+    // the piece kernels read the codes in aligned 16-byte loads (arena allocations are 256-byte
+    // aligned and rounded up to 256 bytes)
+    if (((uintptr_t)bins & 15u) != 0) throw std::logic_error("order-1 encode: codes not 16-byte aligned");
     uint64_t S = n / kO1Seg;
     if (S < 1) S = 1;
     if (S > 16) S = 16;
