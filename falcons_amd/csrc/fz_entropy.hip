@@ -349,14 +349,16 @@ struct FseLds {
     uint32_t count[256];
     TableLdsT<kFblkLog> tab;
     CTableT<kFblkLog> ct;
-    uint64_t result, nc;
+    uint64_t result, nc, sbits;
     uint32_t tl, maxSym;
+    uint32_t fin[2];     // the final values of state 2 and state 1
+    uint32_t part[256];  // per-thread bit counts, then their exclusive scan
 };
 
-// Writes the FSE bytes (when the result is > 1) to fo, which has room for n + 1024 bytes (a block
-// stored raw may leave up to n + 16 of them written). Returns 0 (not compressible), 1 (one symbol),
-// kFseErr, or the compressed size.
-__device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_t* fo, FseLds& L) {
+// Writes the FSE bytes (when the result is > 1) to fo, which has room for n + 1024 bytes. pre holds
+// n 16-bit values of scratch (the state before every symbol's step). Returns 0 (not compressible),
+// 1 (one symbol), kFseErr, or the compressed size.
+__device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_t* fo, FseLds& L, uint16_t* pre) {
     constexpr uint64_t kPending = ~1ull;
     for (int s = threadIdx.x; s < 256; s += blockDim.x) L.count[s] = 0;
     __syncthreads();
@@ -397,115 +399,127 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_
     }
     ctable_parallel(L.tab, tl, L.maxSym, &L.ct);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint64_t nc = L.nc;
-        const uint8_t* in = src;
-        BitW w{fo + nc, 0, 0, 0};
-        uint32_t ip = n, s1, s2;
-        if (n & 1) {
-            s1 = init_state2(&L.ct, in[--ip]);
-            s2 = init_state2(&L.ct, in[--ip]);
-            fse_encode(w, s1, L.ct, in[--ip]);
-        } else {
-            s2 = init_state2(&L.ct, in[--ip]);
-            s1 = init_state2(&L.ct, in[--ip]);
-        }
-        if ((n - 2) & 2) {
-            fse_encode(w, s2, L.ct, in[--ip]);
-            fse_encode(w, s1, L.ct, in[--ip]);
-        }
-        // A block whose output reaches n - 1 bytes is stored raw: stop once the flushed bytes reach
-        // n (fo has room for n + 1024 bytes, the check lags the bits by under 4 bytes and one round
-        // flushes 8). The two states are independent chains; the bits go out in the
-        // reference's order (s2 then s1). Four symbols per round (ip is a multiple of four here): one word of input, their table
-        // values read a round ahead (they do not depend on the states), the two state chains
-        // through LDS address-space pointers. The bits go to a 64-bit accumulator flushed 32 bits at
-        // a time (the byte stream is that of a byte-wise flush), after each pair of symbols.
-        bool full = false;
-        // the word of bytes [x - 4, x) of the block (x a multiple of four; 0 below byte 4), read
-        // three rounds before its round so that its load is in flight meanwhile
-        const bool al4 = ((uintptr_t)src & 3u) == 0;
-        auto word = [&](int64_t x) -> uint32_t {
-            if (x < 4) return 0u;
-            if (al4) return *reinterpret_cast<const uint32_t*>(src + x - 4);
-            return (uint32_t)src[x - 4] | ((uint32_t)src[x - 3] << 8) | ((uint32_t)src[x - 2] << 16) |
-                   ((uint32_t)src[x - 1] << 24);
-        };
-        uint32_t qa = word(ip), qb = word((int64_t)ip - 4), qc = word((int64_t)ip - 8);
-        const FZ_LDS uint32_t* dnb = lds(L.ct.deltaNbBits);
-        const FZ_LDS int32_t* dfs = lds(L.ct.deltaFindState);
-        const FZ_LDS uint16_t* stt = lds(L.ct.stateTable);
-        uint8_t* ob = w.buf;
-        uint64_t acc = w.acc;
-        uint32_t an = w.n;
-        uint32_t pos = (uint32_t)w.pos;
-        uint32_t da[4], fa[4];  // [0]: in[ip-1] (s2), [1]: in[ip-2] (s1), [2]: in[ip-3] (s2), [3]: in[ip-4] (s1)
-        auto load = [&](uint32_t ipx) {  // called for ipx = ip, ip - 4, ... in turn
-            const uint32_t v = qa;
-            qa = qb;
-            qb = qc;
-            qc = word((int64_t)ipx - 12);
+    // The encode (fse_compress.c:578-628) emits symbols n-3 down to 0 alternately from two states,
+    // the first of them seeded by symbol n-1 and the second by n-2 (in zfse's names: state 2 then
+    // state 1 for an even n, state 1 then state 2 for an odd one), then state 2's and state 1's
+    // final values (tl bits each) and the end mark; the bytes are those bits LSB-first. The two
+    // states never interact, so lanes 0 and 1 run one chain each, logging every symbol's state
+    // before its step (pre); the workgroup then scans the bit counts in emission order and packs
+    // the bits in parallel.
+    const FZ_LDS uint32_t* dnb = lds(L.ct.deltaNbBits);
+    const FZ_LDS int32_t* dfs = lds(L.ct.deltaFindState);
+    const FZ_LDS uint16_t* stt = lds(L.ct.stateTable);
+    if (threadIdx.x < 2) {
+        const uint32_t k = threadIdx.x;  // chain k emits e = k, k + 2, ... (symbol n - 3 - e)
+        uint32_t s = init_state2(&L.ct, src[n - 1 - k]);
+        int64_t j = (int64_t)n - 3 - k;
+        // four steps per round; the round's symbols and their table values are read first (they do
+        // not depend on the state)
+        while (j >= 6) {
+            uint32_t sy[4], d[4];
+            int32_t f[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint32_t sym = (v >> (8 * (3 - q))) & 0xFFu;
-                da[q] = dnb[sym];
-                fa[q] = (uint32_t)dfs[sym];
+            for (int r = 0; r < 4; r++) sy[r] = src[j - 2 * r];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                d[r] = dnb[sy[r]];
+                f[r] = dfs[sy[r]];
             }
-        };
-        auto put = [&](uint32_t st, uint32_t nbo) {
-            acc |= (uint64_t)(st & ((1u << nbo) - 1u)) << an;
-            an += nbo;
-        };
-        // under 32 bits pending before a pair, at most 2 * 12 added: the 64-bit accumulator holds it
-        auto flush = [&]() {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                pre[j - 2 * r] = (uint16_t)s;
+                const uint32_t nb = (s + d[r]) >> 16;
+                s = stt[(s >> nb) + f[r]];
+            }
+            j -= 8;
+        }
+        for (; j >= 0; j -= 2) {
+            const uint32_t sym = src[j];
+            pre[j] = (uint16_t)s;
+            const uint32_t nb = (s + dnb[sym]) >> 16;
+            s = stt[(s >> nb) + dfs[sym]];
+        }
+        // state 2 is chain 0 for an even n, chain 1 for an odd one
+        L.fin[((n & 1u) == 0) == (k == 0) ? 0 : 1] = s;
+    }
+    __syncthreads();
+    // bit counts in emission order: thread t takes emissions [t * per, (t + 1) * per)
+    const uint32_t m = n - 2, per = (m + blockDim.x - 1) / blockDim.x;
+    const uint32_t e0 = min(m, threadIdx.x * per), e1 = min(m, e0 + per);
+    auto nbits_of = [&](uint32_t e) -> uint32_t {
+        const uint32_t j = n - 3 - e;
+        return (pre[j] + dnb[src[j]]) >> 16;
+    };
+    uint32_t mine = 0;
+    for (uint32_t e = e0; e < e1; e++) mine += nbits_of(e);
+    L.part[threadIdx.x] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t acc = 0;
+        for (uint32_t t = 0; t < blockDim.x; t++) {
+            const uint32_t v = L.part[t];
+            L.part[t] = (uint32_t)acc;
+            acc += v;
+        }
+        const uint64_t bits = acc + 2 * (uint64_t)tl + 1;
+        L.sbits = acc;
+        const uint64_t total = L.nc + (bits + 7) / 8;
+        L.result = (total >= n - 1) ? 0 : total;
+    }
+    __syncthreads();
+    if (L.result == 0) return 0;
+    // pack: the stream's bytes from fo + nc are cleared, then each thread writes the bits of its
+    // emissions; words wholly inside one thread's bits are stored, shared ones OR-ed in
+    uint8_t* sb = fo + L.nc;
+    const uint64_t nbytes = L.result - L.nc;
+    for (uint64_t i = threadIdx.x; i < nbytes + 4; i += blockDim.x) sb[i] = 0;
+    __syncthreads();
+    const uint32_t lead = (uint32_t)((uintptr_t)sb & 3u);  // the stream's first bit within its word
+    uint32_t* wbase = reinterpret_cast<uint32_t*>(sb - lead);
+    auto emit_bits = [&](uint64_t b0, auto&& next, uint32_t count) {
+        // count (value, nbits) pairs from next(), starting at stream bit b0
+        uint64_t pos = b0 + 8 * (uint64_t)lead;
+        uint64_t widx = pos >> 5;
+        uint64_t acc = 0;
+        uint32_t an = (uint32_t)(pos & 31u);
+        bool first = true;
+        for (uint32_t c = 0; c < count; c++) {
+            uint32_t v, nb;
+            next(v, nb);
+            if (!nb) continue;
+            acc |= (uint64_t)(v & ((1u << nb) - 1u)) << an;
+            an += nb;
             if (an >= 32) {
-                const uint32_t v = (uint32_t)acc;
-                ob[pos] = (uint8_t)v;
-                ob[pos + 1] = (uint8_t)(v >> 8);
-                ob[pos + 2] = (uint8_t)(v >> 16);
-                ob[pos + 3] = (uint8_t)(v >> 24);
-                pos += 4;
+                if (first) {
+                    atomicOr(&wbase[widx], (uint32_t)acc);
+                    first = false;
+                } else {
+                    wbase[widx] = (uint32_t)acc;
+                }
+                widx++;
                 acc >>= 32;
                 an -= 32;
             }
-        };
-        if (ip > 0) load(ip);
-        while (ip > 0) {
-            if (nc + pos >= n) { full = true; break; }
-            const uint32_t d0 = da[0], d1 = da[1], d2 = da[2], d3 = da[3];
-            const int32_t f0 = (int32_t)fa[0], f1 = (int32_t)fa[1], f2 = (int32_t)fa[2], f3 = (int32_t)fa[3];
-            ip -= 4;
-            if (ip > 0) load(ip);
-            const uint32_t n2 = (s2 + d0) >> 16, n1 = (s1 + d1) >> 16;
-            put(s2, n2);
-            put(s1, n1);
-            s2 = stt[(s2 >> n2) + f0];
-            s1 = stt[(s1 >> n1) + f1];
-            flush();
-            const uint32_t m2 = (s2 + d2) >> 16, m1 = (s1 + d3) >> 16;
-            put(s2, m2);
-            put(s1, m1);
-            s2 = stt[(s2 >> m2) + f2];
-            s1 = stt[(s1 >> m1) + f3];
-            flush();
         }
-        w.acc = acc;
-        w.n = an;
-        w.pos = pos;
-        while (w.n >= 8) {
-            w.buf[w.pos++] = (uint8_t)w.acc;
-            w.acc >>= 8;
-            w.n -= 8;
-        }
-        if (full) {
-            L.result = 0;
-        } else {
-            bw_add(w, s2, tl);
-            bw_add(w, s1, tl);
-            const uint64_t bs = bw_close(w);
-            const uint64_t total = nc + bs;
-            L.result = (total >= n - 1) ? 0 : total;
-        }
+        if (an) atomicOr(&wbase[widx], (uint32_t)acc);
+    };
+    {
+        uint32_t e = e0;
+        emit_bits((uint64_t)L.part[threadIdx.x], [&](uint32_t& v, uint32_t& nb) {
+            const uint32_t j = n - 3 - e++;
+            const uint32_t st = pre[j];
+            nb = (st + dnb[src[j]]) >> 16;
+            v = st;
+        }, e1 - e0);
+    }
+    if (threadIdx.x == blockDim.x - 1) {
+        // state 2's and state 1's final values, then the end mark (BIT_closeCStream)
+        uint32_t c = 0;
+        emit_bits(L.sbits, [&](uint32_t& v, uint32_t& nb) {
+            if (c < 2) { v = L.fin[c]; nb = tl; }
+            else { v = 1; nb = 1; }
+            c++;
+        }, 3);
     }
     __syncthreads();
     return L.result;
@@ -513,8 +527,8 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_
 
 // Frames one class-byte block as in RleFse:150-176: [0][raw] | [2][sym] | [1][u32 len][FSE bytes].
 // Returns the framed length written to out.
-__device__ uint64_t frame_block_wg(const uint8_t* src, uint32_t len, uint8_t* out, FseLds& L) {
-    const uint64_t r = fse_block_compress_wg(src, len, out + 5, L);
+__device__ uint64_t frame_block_wg(const uint8_t* src, uint32_t len, uint8_t* out, FseLds& L, uint16_t* pre) {
+    const uint64_t r = fse_block_compress_wg(src, len, out + 5, L, pre);
     uint64_t sz;
     if (r == kFseErr || r == 0 || r >= len) {
         if (threadIdx.x == 0) out[0] = 0;
@@ -545,10 +559,10 @@ struct FseBlk {
 
 // One workgroup per 64 K block (the blocks of every stream of a call in one launch); framed
 // blocks land in fixed slots.
-__global__ void __launch_bounds__(256) k_fse_blocks(const FseBlk* blks, uint8_t* slots, uint64_t* sizes) {
+__global__ void __launch_bounds__(256) k_fse_blocks(const FseBlk* blks, uint8_t* slots, uint64_t* sizes, uint16_t* pre) {
     __shared__ FseLds L;
     const uint64_t b = blockIdx.x;
-    const uint64_t sz = frame_block_wg(blks[b].src, blks[b].len, slots + b * (kFblk + 1024), L);
+    const uint64_t sz = frame_block_wg(blks[b].src, blks[b].len, slots + b * (kFblk + 1024), L, pre + b * kFblk);
     if (threadIdx.x == 0) sizes[b] = sz;
 }
 
@@ -2216,8 +2230,9 @@ static void fse_enc_multi(std::vector<FseEncItem>& items, DevArena& ws, hipStrea
     uint8_t* slots = ws.take<uint8_t>(nb * (kFblk + 1024));
     uint64_t* sizes = ws.take<uint64_t>(nb);
     FseBlk* dblks = ws.take<FseBlk>(nb);
+    uint16_t* pre = ws.take<uint16_t>(nb * kFblk);
     h2d(dblks, blks.data(), nb, st);
-    FZ_LAUNCH("fse_blocks", 0, k_fse_blocks, dim3((unsigned)nb), dim3(256), 0, st, dblks, slots, sizes);
+    FZ_LAUNCH("fse_blocks", 0, k_fse_blocks, dim3((unsigned)nb), dim3(256), 0, st, dblks, slots, sizes, pre);
     FZ_HIP(hipGetLastError());
     std::vector<uint64_t> hs(nb);
     d2h(hs.data(), sizes, nb, st);
@@ -2420,6 +2435,7 @@ static size_t escape_workspace_bound(const std::vector<uint64_t>& off, uint64_t 
     add(nb * (kFblk + 1024));
     add(8 * nb);
     add(sizeof(FseBlk) * nb);
+    add(2 * nb * kFblk);
     return bytes + (1u << 20);
 }
 
