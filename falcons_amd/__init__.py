@@ -174,7 +174,7 @@ _L.fz_server_destroy.argtypes = [C.c_void_p]
 
 EXPORTED_SYMBOLS = ("fz_ctx_create", "fz_ctx_destroy", "fz_last_error", "fz_opts_init", "fz_config_init",
                     "fz_config_size_est", "fz_config_save", "fz_read_config", "fz_config_set_dims",
-                    "fz_config_load", "fz_compress_size_bound",
+                    "fz_config_load", "fz_compress_size_bound", "fz_config_on_gpu",
                     "fz_compress", "fz_decompress", "fz_prezstd_bound", "fz_compress_device",
                     "fz_compress_device_noalloc",
                     "fz_decompress_device", "fz_device_alloc", "fz_device_free", "fz_memcpy", "fz_synchronize",

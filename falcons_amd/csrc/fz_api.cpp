@@ -257,6 +257,14 @@ extern "C" int fz_read_config(const uint8_t* cmp, size_t len, fz_config* out, ch
     }
 }
 
+extern "C" int fz_config_on_gpu(const fz_config* c, int dtype) {
+    if (!c) return FZ_ERR_INVALID_ARGUMENT;
+    if (dtype < FZ_FLOAT || dtype > FZ_INT64 || c->N < 1 || c->N > 4 || c->openmp || c->quantbin_cnt != 65536 ||
+        (c->cmpr_algo != 1 && c->cmpr_algo != 2))
+        return FZ_ERR_UNSUPPORTED;
+    return FZ_OK;
+}
+
 extern "C" size_t fz_compress_size_bound(const fz_config* c, int dtype) {
     return 4096 + fz_config_size_est(c) + fz::zstd().compressBound(c->num * (elem_size(dtype)));
 }

@@ -13,11 +13,14 @@
 //   Falcons_decompress(conf, cmpData, cmpSize)           falcons.hpp:208
 //
 // Falcons_read_config, Falcons_compress_size_bound, Config and the rest stay Falcons' own.
-// Float and double fields run on the GPU with the same bytes, the same in-place update of the input
-// and the same exceptions (std::invalid_argument where Falcons throws it). What the GPU path does not
-// produce (FZ_ERR_UNSUPPORTED: integer types, 4-D fields, the OpenMP stream format, the Lorenzo /
-// regression algorithms, quantbinCnt other than 65536) runs Falcons' own template, under its
-// renamed name. One library context per thread, on the GPU FALCONS_MI355X_DEVICE names (default 0).
+// Fields of the ten Falcons element types (float, double, 8- to 64-bit integers), 1-D to 4-D, run on
+// the GPU with the same bytes, the same in-place update of the input and the same exceptions
+// (std::invalid_argument where Falcons throws it). What the GPU path does not produce runs Falcons'
+// own template, under its renamed name: configurations fz_config_on_gpu rules out (the OpenMP stream
+// format, the Lorenzo / regression and lossless algorithms, quantbinCnt other than 65536) go there
+// without opening a GPU context, and a call that returns FZ_ERR_UNSUPPORTED for its data (a zero
+// bound, a parameter search that picks Lorenzo) goes there afterwards. One library context per
+// thread, on the GPU FALCONS_MI355X_DEVICE names (default 0).
 #ifndef FALCONS_MI355X_API_FRONT_HPP
 #define FALCONS_MI355X_API_FRONT_HPP
 
@@ -56,9 +59,20 @@ inline fz_ctx *context() {
     return h.c;
 }
 
+// the Falcons element type of T (Config.hpp:29-38), or -1
 template <class T>
 constexpr int dtype_of() {
-    return std::is_same<T, float>::value ? FZ_FLOAT : std::is_same<T, double>::value ? FZ_DOUBLE : -1;
+    return std::is_same<T, float>::value      ? FZ_FLOAT
+           : std::is_same<T, double>::value   ? FZ_DOUBLE
+           : std::is_same<T, uint8_t>::value  ? FZ_UINT8
+           : std::is_same<T, int8_t>::value   ? FZ_INT8
+           : std::is_same<T, uint16_t>::value ? FZ_UINT16
+           : std::is_same<T, int16_t>::value  ? FZ_INT16
+           : std::is_same<T, uint32_t>::value ? FZ_UINT32
+           : std::is_same<T, int32_t>::value  ? FZ_INT32
+           : std::is_same<T, uint64_t>::value ? FZ_UINT64
+           : std::is_same<T, int64_t>::value  ? FZ_INT64
+                                              : -1;
 }
 
 // Config -> the C-ABI's fz_config (the fields Config::save writes, Config.hpp:316-362)
@@ -100,9 +114,9 @@ inline void to_fz(const Config &c, fz_config &f) {
 template <class T>
 size_t Falcons_compress(const Falcons::Config &config, const T *data, char *cmpData, size_t cmpCap) {
     namespace M = Falcons::mi355x;
-    if (M::dtype_of<T>() >= 0) {
-        fz_config f;
-        M::to_fz(config, f);
+    fz_config f;
+    M::to_fz(config, f);
+    if (M::dtype_of<T>() >= 0 && fz_config_on_gpu(&f, M::dtype_of<T>()) == FZ_OK) {
         size_t n = 0;
         // the input is overwritten with the reconstruction, as Falcons does through its const pointer
         const int rc = fz_compress(M::context(), &f, M::dtype_of<T>(), const_cast<T *>(data), 0,
@@ -131,7 +145,9 @@ void Falcons_decompress(Falcons::Config &config, const char *cmpData, size_t cmp
     namespace M = Falcons::mi355x;
     Falcons_read_config(config, cmpData, cmpSize);
     if (decData == nullptr) decData = new T[config.num];
-    if (M::dtype_of<T>() >= 0) {
+    fz_config f;
+    M::to_fz(config, f);
+    if (M::dtype_of<T>() >= 0 && fz_config_on_gpu(&f, M::dtype_of<T>()) == FZ_OK) {
         const int rc = fz_decompress(M::context(), reinterpret_cast<const uint8_t *>(cmpData), cmpSize,
                                      M::dtype_of<T>(), decData, config.num, 0, nullptr, nullptr);
         if (rc == FZ_OK) return;

@@ -101,6 +101,14 @@ int fz_config_load(const uint8_t *in, size_t len, fz_config *out, char *err, siz
 int fz_read_config(const uint8_t *cmp, size_t len, fz_config *out, char *err, size_t errlen);
 /* Falcons_compress_size_bound<T> (FalconsImpl.hpp:44-54) */
 size_t fz_compress_size_bound(const fz_config *c, int dtype);
+/* Host-only (no device is touched): FZ_ERR_UNSUPPORTED when a configuration is certainly outside the
+ * GPU path (element type, more than 4 dimensions, the OpenMP stream format (Config.hpp:347-349),
+ * quantbinCnt other than 65536, an algorithm other than ALGO_INTERP / ALGO_INTERP_LORENZO), FZ_OK
+ * when it may run there (a call can still return FZ_ERR_UNSUPPORTED for data-dependent reasons: a
+ * zero bound, or a parameter search that picks Lorenzo). No reference counterpart: the API front
+ * (include/Falcons/api/falcons.hpp) asks it before it opens a context, so that configurations Falcons'
+ * own code handles never need a GPU. */
+int fz_config_on_gpu(const fz_config *c, int dtype);
 
 /* Falcons_compress<T>(conf, data, cmpData, cmpCap) (falcons.hpp:54-97). data may be host or device
  * memory (data_on_device); it is overwritten with the reconstructed values like the reference's

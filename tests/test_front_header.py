@@ -60,17 +60,37 @@ def test_h5z_front_registers_filter_40024(built):
     assert "fz_compress" in syms and "fz_decompress" in syms
 
 
-def test_cli_integer_field_runs_the_reference_template(built, tmp_path):
-    """An int32 field is outside the GPU path: the front hands it to Falcons' own template (no GPU
-    call), and the round trip stays within the bound."""
-    rng = np.random.default_rng(3)
-    a = np.cumsum(rng.integers(-3, 4, size=(16, 24, 40)), axis=2).astype(np.int32)
+def test_cli_openmp_mode_runs_the_reference_template(built, tmp_path):
+    """The OpenMP stream format (-P) is outside the GPU path: fz_config_on_gpu rules it out, so the
+    front hands the call to Falcons' own template without opening a GPU context (none is visible
+    here), and the round trip stays within the bound."""
+    a = np.cumsum(np.random.default_rng(3).standard_normal((16, 24, 40)), axis=2).astype(np.float32)
     src = tmp_path / "in.dat"
     a.tofile(src)
     z, o = tmp_path / "in.falcons", tmp_path / "out.dat"
-    r = subprocess.run([CLI, "-I", "32", "-i", str(src), "-z", str(z), "-o", str(o), "-3", "40", "24", "16",
-                        "-M", "ABS", "2"], capture_output=True, text=True, env=dict(os.environ, HIP_VISIBLE_DEVICES=""))
+    r = subprocess.run([CLI, "-f", "-P", "-i", str(src), "-z", str(z), "-o", str(o), "-3", "40", "24", "16",
+                        "-M", "ABS", "1e-2"], capture_output=True, text=True, env=dict(os.environ, HIP_VISIBLE_DEVICES=""))
     assert r.returncode == 0, r.stdout + r.stderr
-    dec = np.fromfile(o, dtype=np.int32).reshape(a.shape)
-    assert np.abs(dec.astype(np.int64) - a).max() <= 2
+    dec = np.fromfile(o, dtype=np.float32).reshape(a.shape)
+    assert np.abs(dec.astype(np.float64) - a).max() <= 1e-2
     assert z.stat().st_size < a.nbytes
+
+
+def test_config_on_gpu_rules():
+    """fz_config_on_gpu: host-only answer to "may this configuration run on the GPU path"."""
+    import falcons_amd as fa
+    L = C.CDLL(fa.LIB_PATH)
+    L.fz_config_on_gpu.restype = C.c_int
+    L.fz_config_on_gpu.argtypes = [C.c_void_p, C.c_int]
+
+    def on(dtype=0, **kw):
+        c = fa.falconsConfig(20, 30, 40)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return L.fz_config_on_gpu(C.byref(c.conf), dtype)
+
+    assert on() == 0 and on(dtype=9) == 0 and on(cmprAlgo=1) == 0
+    assert on(dtype=10) != 0 and on(dtype=-1) != 0
+    assert on(openmp=True) != 0
+    assert on(quantbinCnt=1024) != 0
+    assert on(cmprAlgo=0) != 0 and on(cmprAlgo=3) != 0 and on(cmprAlgo=4) != 0
