@@ -345,6 +345,7 @@ constexpr unsigned kFblkLog = kFseDefaultTableLog;
 // The encoder's LDS: the block's histogram and its tables (a few KB, so that many blocks run on a
 // CU at once); the block's bytes are read from device memory and the FSE bytes go straight to the
 // block's output slot.
+constexpr uint32_t kFseTileWords = (256 * kFblkLog + 31) / 32 + 3;
 struct FseLds {
     uint32_t count[256];
     TableLdsT<kFblkLog> tab;
@@ -352,7 +353,8 @@ struct FseLds {
     uint64_t result, nc, sbits;
     uint32_t tl, maxSym;
     uint32_t fin[2];     // the final values of state 2 and state 1
-    uint32_t part[256];  // per-thread bit counts, then their exclusive scan
+    uint32_t part[4];    // per-wave sums
+    uint32_t tw[kFseTileWords];  // a tile's packed words (256 emissions of at most 11 bits, a carried word)
 };
 
 // Writes the FSE bytes (when the result is > 1) to fo, which has room for n + 1024 bytes. pre holds
@@ -443,24 +445,21 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_
         L.fin[((n & 1u) == 0) == (k == 0) ? 0 : 1] = s;
     }
     __syncthreads();
-    // bit counts in emission order: thread t takes emissions [t * per, (t + 1) * per)
-    const uint32_t m = n - 2, per = (m + blockDim.x - 1) / blockDim.x;
-    const uint32_t e0 = min(m, threadIdx.x * per), e1 = min(m, e0 + per);
+    // the bit counts in emission order, summed over the workgroup (one emission per thread per
+    // tile of blockDim.x, so the reads of pre and src coalesce)
+    const uint32_t m = n - 2;  // emissions e: symbol n - 3 - e
     auto nbits_of = [&](uint32_t e) -> uint32_t {
         const uint32_t j = n - 3 - e;
         return (pre[j] + dnb[src[j]]) >> 16;
     };
     uint32_t mine = 0;
-    for (uint32_t e = e0; e < e1; e++) mine += nbits_of(e);
-    L.part[threadIdx.x] = mine;
+    for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) mine += nbits_of(e);
+    for (uint32_t d = 32; d; d >>= 1) mine += __shfl_xor(mine, d, 64);
+    if ((threadIdx.x & 63u) == 0) L.part[threadIdx.x >> 6] = mine;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t acc = 0;
-        for (uint32_t t = 0; t < blockDim.x; t++) {
-            const uint32_t v = L.part[t];
-            L.part[t] = (uint32_t)acc;
-            acc += v;
-        }
+        for (uint32_t w = 0; w < blockDim.x / 64; w++) acc += L.part[w];
         const uint64_t bits = acc + 2 * (uint64_t)tl + 1;
         L.sbits = acc;
         const uint64_t total = L.nc + (bits + 7) / 8;
@@ -468,60 +467,70 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_
     }
     __syncthreads();
     if (L.result == 0) return 0;
-    // pack: the stream's bytes from fo + nc are cleared, then each thread writes the bits of its
-    // emissions; words wholly inside one thread's bits are stored, shared ones OR-ed in
+    // pack, tile by tile: each thread places its emission's bits in an LDS word buffer at its
+    // scanned offset; the tile's whole words are stored, the partial last one carries over
     uint8_t* sb = fo + L.nc;
-    const uint64_t nbytes = L.result - L.nc;
-    for (uint64_t i = threadIdx.x; i < nbytes + 4; i += blockDim.x) sb[i] = 0;
-    __syncthreads();
-    const uint32_t lead = (uint32_t)((uintptr_t)sb & 3u);  // the stream's first bit within its word
+    const uint32_t lead = (uint32_t)((uintptr_t)sb & 3u);  // stream bytes in front, in the first word
     uint32_t* wbase = reinterpret_cast<uint32_t*>(sb - lead);
-    auto emit_bits = [&](uint64_t b0, auto&& next, uint32_t count) {
-        // count (value, nbits) pairs from next(), starting at stream bit b0
-        uint64_t pos = b0 + 8 * (uint64_t)lead;
-        uint64_t widx = pos >> 5;
-        uint64_t acc = 0;
-        uint32_t an = (uint32_t)(pos & 31u);
-        bool first = true;
-        for (uint32_t c = 0; c < count; c++) {
-            uint32_t v, nb;
-            next(v, nb);
-            if (!nb) continue;
-            acc |= (uint64_t)(v & ((1u << nb) - 1u)) << an;
-            an += nb;
-            if (an >= 32) {
-                if (first) {
-                    atomicOr(&wbase[widx], (uint32_t)acc);
-                    first = false;
-                } else {
-                    wbase[widx] = (uint32_t)acc;
-                }
-                widx++;
-                acc >>= 32;
-                an -= 32;
-            }
-        }
-        if (an) atomicOr(&wbase[widx], (uint32_t)acc);
-    };
-    {
-        uint32_t e = e0;
-        emit_bits((uint64_t)L.part[threadIdx.x], [&](uint32_t& v, uint32_t& nb) {
-            const uint32_t j = n - 3 - e++;
-            const uint32_t st = pre[j];
-            nb = (st + dnb[src[j]]) >> 16;
-            v = st;
-        }, e1 - e0);
-    }
-    if (threadIdx.x == blockDim.x - 1) {
-        // state 2's and state 1's final values, then the end mark (BIT_closeCStream)
-        uint32_t c = 0;
-        emit_bits(L.sbits, [&](uint32_t& v, uint32_t& nb) {
-            if (c < 2) { v = L.fin[c]; nb = tl; }
-            else { v = 1; nb = 1; }
-            c++;
-        }, 3);
-    }
+    uint32_t* tb = L.tw;
+    for (uint32_t w = threadIdx.x; w < kFseTileWords; w += blockDim.x) tb[w] = 0;
     __syncthreads();
+    if (threadIdx.x == 0 && lead) tb[0] = wbase[0] & ((1u << (8 * lead)) - 1u);  // the header's last bytes
+    uint64_t cur = 8 * (uint64_t)lead;  // bit position (from wbase) of the next emission
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    auto place = [&](uint64_t b, uint32_t v, uint32_t nb) {  // b: bit position from word cur >> 5
+        if (!nb) return;
+        v &= (1u << nb) - 1u;
+        const uint32_t wi = (uint32_t)(b >> 5), sh = (uint32_t)(b & 31u);
+        atomicOr(&tb[wi], v << sh);
+        if (sh + nb > 32) atomicOr(&tb[wi + 1], v >> (32 - sh));
+    };
+    auto drain = [&](uint64_t end) {  // store the whole words below bit `end`, keep the partial one
+        const uint64_t w0 = cur >> 5;
+        const uint32_t nw = (uint32_t)((end >> 5) - w0);
+        __syncthreads();
+        for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) wbase[w0 + w] = tb[w];
+        __syncthreads();
+        const uint32_t carry = tb[nw];
+        __syncthreads();
+        for (uint32_t w = threadIdx.x; w < kFseTileWords; w += blockDim.x) tb[w] = w == 0 ? carry : 0u;
+        __syncthreads();
+        cur = end;
+    };
+    for (uint32_t t0 = 0; t0 < m; t0 += blockDim.x) {
+        const uint32_t e = t0 + threadIdx.x;
+        uint32_t v = 0, nb = 0;
+        if (e < m) {
+            const uint32_t j = n - 3 - e;
+            v = pre[j];
+            nb = (v + dnb[src[j]]) >> 16;
+        }
+        // exclusive scan of nb over the tile: within the wave, then across the waves
+        uint32_t inc = nb;
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t y = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += y;
+        }
+        if (lane == 63) L.part[wv] = inc;
+        __syncthreads();
+        uint32_t before = 0, tile = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64; w++) {
+            const uint32_t pw = L.part[w];
+            if (w < wv) before += pw;
+            tile += pw;
+        }
+        place((cur & 31u) + before + inc - nb, v, nb);
+        drain(cur + tile);
+    }
+    if (threadIdx.x == 0) {
+        // state 2's and state 1's final values, then the end mark (BIT_closeCStream)
+        const uint64_t b = cur & 31u;
+        place(b, L.fin[0], tl);
+        place(b + tl, L.fin[1], tl);
+        place(b + 2 * tl, 1u, 1);
+    }
+    drain(cur + 2 * (uint64_t)tl + 1 + 31);  // the last partial word too
     return L.result;
 }
 
@@ -617,11 +626,10 @@ __device__ __forceinline__ void fse_decode_run(const DEntry* dt, const uint8_t* 
 }
 
 // The same loop with the table and the bitstream in LDS (in: the bitstream, bsz bytes, 4-byte
-// aligned with 12 readable bytes past its end); the symbols go to out in device memory, four to a
-// store when out is 4-byte aligned (kAligned4), else byte by byte. The bits below the read position
-// sit in a 64-bit buffer refilled from aligned LDS words once per two symbol pairs (two pairs read
-// at most 4 * 11 bits); the state values stay in vector registers.
-template <bool kAligned4>
+// aligned with 12 readable bytes past its end); the symbols go to out in device memory (4-byte
+// aligned), four to a store. The bits below the read position sit in a 64-bit buffer refilled from
+// aligned LDS words once per two symbol pairs (two pairs read at most 4 * 11 bits); the state values
+// stay in vector registers.
 __device__ __forceinline__ void fse_decode_run_lds(const FZ_LDS DEntry* dt, const FZ_LDS uint8_t* in, uint64_t bsz,
                                                    unsigned tl, uint8_t* out, uint32_t len) {
     const FZ_LDS uint32_t* w32 = reinterpret_cast<const FZ_LDS uint32_t*>(in);
@@ -661,14 +669,7 @@ __device__ __forceinline__ void fse_decode_run_lds(const FZ_LDS DEntry* dt, cons
         s1 = (f1 & 0xFFFFu) + read(f1 >> 24);
         const uint32_t v = ((e0 >> 16) & 0xFFu) | (((e1 >> 16) & 0xFFu) << 8) | (((f0 >> 16) & 0xFFu) << 16) |
                            (((f1 >> 16) & 0xFFu) << 24);
-        if constexpr (kAligned4) {
-            *reinterpret_cast<uint32_t*>(out + i) = v;
-        } else {
-            out[i] = (uint8_t)v;
-            out[i + 1] = (uint8_t)(v >> 8);
-            out[i + 2] = (uint8_t)(v >> 16);
-            out[i + 3] = (uint8_t)(v >> 24);
-        }
+        *reinterpret_cast<uint32_t*>(out + i) = v;
     }
     for (; i + 2 <= len; i += 2) {
         refill();
@@ -728,8 +729,7 @@ __global__ void __launch_bounds__(256) k_fse_dec_blocks(const FseBlk* blks, int*
         __syncthreads();
         if (threadIdx.x == 0) {
             if (S.in[bsz - 1] == 0) *err = 1;
-            else if (((uintptr_t)dst & 3u) == 0) fse_decode_run_lds<true>(lds(S.dt), lds(S.in), bsz, tl, dst, len);
-            else fse_decode_run_lds<false>(lds(S.dt), lds(S.in), bsz, tl, dst, len);
+            else fse_decode_run_lds(lds(S.dt), lds(S.in), bsz, tl, dst, len);
         }
         return;
     }
@@ -1180,10 +1180,61 @@ __device__ inline void ww_flush(WordWriter& W) {
     if (W.n) atomicOr(&W.w[W.idx], (uint32_t)W.acc);
 }
 
+// k_o1_emit's output: a lane's words pass through a 16-word LDS ring and leave in aligned 64-byte
+// runs (the lanes of a wave write 131 K pieces' lines at once, and single-word stores leave partly
+// written lines to L2 evictions). The first word (shared with the previous piece) is OR-ed in; the
+// words before the first 64-byte boundary, and those after the last full run, go one by one; the last,
+// partial word (shared with the next piece) is OR-ed in.
+struct RunWriter {
+    uint32_t* w;            // the chain's words
+    FZ_LDS uint32_t* ring;  // this lane's 16 words
+    uint64_t idx;           // the word being filled
+    uint64_t acc;
+    uint32_t n;
+    bool first, runs;       // runs: a 64-byte boundary has been passed
+};
+__device__ inline void rw_word(RunWriter& W, uint32_t v) {
+    if (W.first) {
+        atomicOr(&W.w[W.idx], v);
+        W.first = false;
+    } else if (!W.runs) {
+        W.w[W.idx] = v;
+    } else {
+        W.ring[W.idx & 15u] = v;
+        if ((W.idx & 15u) == 15u) {
+            const FZ_LDS u32x4* r = reinterpret_cast<const FZ_LDS u32x4*>(W.ring);
+            u32x4* d = reinterpret_cast<u32x4*>(W.w + (W.idx - 15));
+            const u32x4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+            d[0] = r0;
+            d[1] = r1;
+            d[2] = r2;
+            d[3] = r3;
+        }
+    }
+    W.idx++;
+    if ((W.idx & 15u) == 0) W.runs = true;
+}
+__device__ inline void rw_put(RunWriter& W, uint32_t v, uint32_t nb) {
+    if (!nb) return;
+    W.acc |= (uint64_t)(v & ((1u << nb) - 1)) << W.n;
+    W.n += nb;
+    if (W.n >= 32) {
+        rw_word(W, (uint32_t)W.acc);
+        W.acc >>= 32;
+        W.n -= 32;
+    }
+}
+__device__ inline void rw_flush(RunWriter& W) {
+    if (W.runs)
+        for (uint64_t k = W.idx & ~(uint64_t)15; k < W.idx; k++) W.w[k] = W.ring[k & 15u];
+    if (W.n) atomicOr(&W.w[W.idx], (uint32_t)W.acc);
+}
+
 __global__ void __launch_bounds__(256) k_o1_emit(const int16_t* bins, O1Chains C, uint64_t npieces,
                                                  const CTable* ct_all, const uint32_t* end_state,
                                                  const uint64_t* bitoff, uint32_t* words, uint64_t stride_words) {
     __shared__ O1EncLds T;
+    __shared__ __align__(16) uint32_t ring[256 * 16];
     const uint64_t first = (uint64_t)xcd_block() * blockDim.x;
     const uint64_t wseg = o1_stage_tables(T, C, npieces, ct_all, first);
     const uint64_t pid = first + threadIdx.x;
@@ -1199,14 +1250,14 @@ __global__ void __launch_bounds__(256) k_o1_emit(const int16_t* bins, O1Chains C
     o1_piece_range(lo, hi, p, s_hi, s_lo);
     uint32_t st = p == 0 ? o1_seed_state(bins, lo, hi - 1, ct) : end_state[pid - 1];
     const uint64_t b0 = bitoff[pid];
-    WordWriter W{words + q * stride_words, b0 >> 5, 0, (uint32_t)(b0 & 31), true};
+    RunWriter W{words + q * stride_words, lds(ring + threadIdx.x * 16), b0 >> 5, 0, (uint32_t)(b0 & 31), true, false};
     o1_piece_codes(bins, lo, s_hi, s_lo, [&](uint64_t, int code, int pc) {
         const uint32_t st0 = st;
         const uint32_t nbOut = o1_step_t(st, ct, dnb, dfs, staged, o1_ctx(pc), (unsigned)o1_sym(code));
-        ww_put(W, st0, nbOut);
+        rw_put(W, st0, nbOut);
         return true;
     });
-    ww_flush(W);
+    rw_flush(W);
 }
 
 // Final state (kO1Log bits) and the end mark of every chain (BIT_closeCStream); byte length out.
@@ -2572,8 +2623,9 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
     h2d(dchains, chains.data(), chains.size(), st);
     uint64_t maxcs = 0;
     for (uint64_t g = 0; g < S; g++) maxcs = std::max(maxcs, (seg_n[g] + kO1K - 1) / kO1K);
-    // chain staging: 13 bits per symbol at most, plus the closing state
-    const uint64_t stride_words = (maxcs * 2 + 4096 + 3) / 4;
+    // chain staging: 13 bits per symbol at most, plus the closing state; every chain's words start
+    // 64-byte aligned (k_o1_emit stores whole aligned 64-byte runs)
+    const uint64_t stride_words = ((maxcs * 2 + 4096 + 3) / 4 + 15) & ~(uint64_t)15;
     const uint64_t bstride = stride_words * 4;
     uint32_t* words = ws.take<uint32_t>(nch * stride_words);
     uint8_t* bufs = reinterpret_cast<uint8_t*>(words);
@@ -2779,6 +2831,8 @@ static void fse_dec_multi(const std::vector<FseDecItem>& items, int* err, DevAre
     std::vector<uint64_t> offs;
     for (const FseDecItem& it : items) {
         if (it.n == 0) continue;
+        // the block decoder stores four symbols at a time (blocks start 64 KiB apart in out)
+        if (((uintptr_t)it.out & 3u) != 0) throw std::logic_error("FSE block decode: output not 4-byte aligned");
         frame_offsets(it.hsrc, it.n, it.avail, offs);
         for (size_t k = 0; k < offs.size(); k++)
             blks.push_back(FseBlk{it.dsrc + offs[k], it.out + k * kFblk,
