@@ -1513,6 +1513,7 @@ constexpr int kSeqLanes = 64;  // the sequential decoders run as one wave
 constexpr int kGBr = 16;     // live branches
 constexpr int kGNode = 24;   // escapes per subsequence graph
 constexpr int kGPend = 48;   // open edges
+constexpr uint64_t kGWork = 2;  // decode steps per graph, in average subsequences
 
 struct O1GEdge {
     int32_t tgt;   // >= 0: node, < 0: exit -1-tgt
@@ -1527,7 +1528,7 @@ struct O1StartSet {
 
 struct O1Graph {
     O1DecState exit[kGBr];
-    int nexit, nnode, status, norigin;  // status 0: complete, 1: capacity exceeded
+    int nexit, nnode, status, norigin;  // status 0: complete, else capacity or step budget exceeded
     O1DecState ostate[kGBr];            // the start states the graph was built from
     O1GEdge origin[kGBr];
     O1GEdge node[kGNode][2];
@@ -1605,6 +1606,12 @@ __global__ void __launch_bounds__(kGLanes) k_o1d_graph(const uint8_t* in, const 
     const uint8_t* buf = in + c.off;
     const int64_t stop = c.pos0 - (int64_t)(r + 1) * c.sub_bits;
     const uint32_t cap = (uint32_t)min<uint64_t>(c.hi - c.lo, 0xFFFFFFFFull);
+    // A step budget: kGWork times the symbols a subsequence holds on average, plus a margin. Graphs
+    // that keep several branches live across the whole subsequence cost up to kGBr times one exact
+    // decode; past the budget the walk decodes the subsequence exactly instead (status 5).
+    const uint64_t expect = c.pos0 > 0 ? (uint64_t)c.sub_bits * (c.hi - c.lo) / (uint64_t)c.pos0 : 0;
+    const uint64_t budget = kGWork * expect + 512;
+    uint64_t work = 0;
     O1Graph& G = graph[t];
     int nb = 0, np = 0, nexit = 0, nnode = 0, status = 0;
     {
@@ -1667,9 +1674,14 @@ __global__ void __launch_bounds__(kGLanes) k_o1d_graph(const uint8_t* in, const 
         nb++;
     };
     while (nb > 0 && status == 0) {
+        if (work > budget) {
+            status = 5;
+            break;
+        }
         if (nb == 1) {  // single branch: registers only
             BitWin rd = bwin_at(buf, bB[0][l]);
             uint32_t X = bX[0][l], cx = bC[0][l], cnt = bN[0][l];
+            const uint32_t cnt0 = cnt;
             bool esc = false;
             while (rd.pos > stop && cnt < cap) {
                 const DEntry d = dt[(size_t)cx * (1u << kO1Log) + (X & ((1u << kO1Log) - 1))];
@@ -1680,6 +1692,7 @@ __global__ void __launch_bounds__(kGLanes) k_o1d_graph(const uint8_t* in, const 
                 cx = (uint32_t)o1_ctx((int)((sy >> 1) ^ (~(sy & 1) + 1)));
             }
             bB[0][l] = rd.pos; bX[0][l] = X; bC[0][l] = (uint8_t)cx; bN[0][l] = cnt;
+            work += cnt - cnt0;
             if (esc) fork_branch(0);
             else exit_branch(0);
             continue;
@@ -1693,6 +1706,7 @@ __global__ void __launch_bounds__(kGLanes) k_o1d_graph(const uint8_t* in, const 
         }
         const uint32_t cx0 = bC[i][l], X0 = bX[i][l];
         const DEntry d = dt[(size_t)cx0 * (1u << kO1Log) + (X0 & ((1u << kO1Log) - 1))];
+        work++;
         BitWin rd = bwin_at(buf, bB[i][l]);
         const uint32_t X = d.newState + bwin_read(rd, d.nbBits & 15u);
         const int64_t B = rd.pos;
