@@ -280,6 +280,7 @@ struct Plan {
     uint64_t ncodes = 0, nanchors = 0;
     uint64_t acnt[3] = {1, 1, 1}, astr[3] = {0, 0, 0};
     std::vector<PassDesc> passes;
+    std::vector<LevelDesc> levels3;  // 3D and 2D, direction 0: the passes of each level, for interp_level
 };
 
 PassDesc rows_pass(int cubic, uint64_t n, uint64_t sp, uint64_t no, uint64_t so, uint64_t len, uint64_t sk, int skip,
@@ -375,6 +376,20 @@ Plan make_plan(int N, const uint64_t* dims, long anchor_in, int cubic, int dir) 
             p.code_base = code;
             code += p.ncodes;
             P.passes.push_back(p);
+        }
+        if ((N == 3 || N == 2) && dir == 0) {
+            LevelDesc d{};
+            d.ex = st;
+            if (N == 3) {
+                d.pz = lv[0]; d.py = lv[1]; d.px = lv[2];
+                d.nz = (dims[0] - 1) / st + 1; d.ny = (dims[1] - 1) / st + 1; d.nx = (dims[2] - 1) / st + 1;
+                d.ez = st * dims[1] * dims[2]; d.ey = st * dims[2];
+            } else {
+                d.py = lv[0]; d.px = lv[1];  // d.pz stays empty (n = 0)
+                d.nz = 1; d.ny = (dims[0] - 1) / st + 1; d.nx = (dims[1] - 1) / st + 1;
+                d.ez = 0; d.ey = st * dims[1];
+            }
+            P.levels3.push_back(d);
         }
     }
     P.ncodes = code;
@@ -587,8 +602,13 @@ size_t gpu_compress(fz_ctx* ctx, DevArena& ws, fz_config& c, int dtype, void* d_
         if (P.anchor) interp_anchors(dtype, false, d_data, unpred, P.acnt, P.astr, st);
         else interp_first(dtype, false, d_data, codes, escval, eb_dev, nullptr, 0, nullptr, st);
         const int avx2 = simd ? 1 : 0;
-        for (const auto& p : P.passes)
-            interp_pass(dtype, false, p, d_data, codes, escval, lv, avx2, nullptr, 0, nullptr, st);
+        if (!P.levels3.empty()) {
+            for (const auto& d : P.levels3)
+                interp_level(dtype, false, d, d_data, codes, escval, lv, avx2, nullptr, 0, nullptr, st);
+        } else {
+            for (const auto& p : P.passes)
+                interp_pass(dtype, false, p, d_data, codes, escval, lv, avx2, nullptr, 0, nullptr, st);
+        }
         select_escapes(codes, P.ncodes, esc_idx, nesc_dev, sel_tmp, sel_bytes, st);
         gather_escapes(dtype, esc_idx, nesc_dev, P.ncodes, escval, static_cast<uint8_t*>(unpred) + P.nanchors * es,
                        st);
@@ -788,8 +808,13 @@ void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h
         const void* esc_vals = static_cast<const uint8_t*>(d_unpred) + P.nanchors * es;
         if (P.anchor) interp_anchors(dtype, true, d_out, d_unpred, P.acnt, P.astr, st);
         else interp_first(dtype, true, d_out, codes, nullptr, eb_dev, esc_idx, nesc, esc_vals, st);
-        for (const auto& ps : P.passes)
-            interp_pass(dtype, true, ps, d_out, codes, nullptr, lv, 1, esc_idx, nesc, esc_vals, st);
+        if (!P.levels3.empty()) {
+            for (const auto& d : P.levels3)
+                interp_level(dtype, true, d, d_out, codes, nullptr, lv, 1, esc_idx, nesc, esc_vals, st);
+        } else {
+            for (const auto& ps : P.passes)
+                interp_pass(dtype, true, ps, d_out, codes, nullptr, lv, 1, esc_idx, nesc, esc_vals, st);
+        }
     }
     const double tdev = tm.stop();
     if (o) {

@@ -52,7 +52,27 @@ struct PassDesc {
     uint64_t ncodes;           // codes this pass emits
 };
 
+// One level of a 3D or 2D field whose passes run z, y, x (direction 0): the z rows pass (3D only),
+// the y rows pass and the x lines pass over a level grid of nz x ny x nx points (nz = 1 in 2D) lying
+// ez, ey, ex elements apart. The y and x passes run in one launch over tiles of ty whole rows of a
+// plane (nyb tiles a plane, nxe even columns a row); ty = 0: they run as separate passes.
+struct LevelDesc {
+    PassDesc pz, py, px;
+    uint64_t nz, ny, nx;
+    uint64_t ez, ey, ex;
+    uint32_t ty, nyb, nxe, lds_bytes;
+};
+
 enum Dtype { kF32 = 0, kF64 = 1 };
+
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b mod 8), and each XCD has
+// its own L2. Kernels whose neighbouring workgroups share data (the tANS tables of one segment, the
+// planes next to an interpolation tile) renumber their workgroups so that every XCD takes one
+// contiguous range: XCD x gets the logical blocks x*(n/8) + min(x, n%8) onward, in order.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t n = gridDim.x, b = blockIdx.x, x = b & 7u, k = b >> 3;
+    return x * (n >> 3) + min(x, n & 7u) + k;
+}
 
 // Per-launch timing with HIP events on the launching stream, switched on per context. Every launch
 // site names its kernel and the algorithmic bytes of that launch (the bytes the kernel must read

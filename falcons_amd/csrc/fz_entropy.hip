@@ -60,16 +60,7 @@ template <typename T>
 __device__ __forceinline__ FZ_LDS T* lds(T* p) { return (FZ_LDS T*)p; }
 
 // ---------------------------------------------------------------- small device helpers
-// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b mod 8), and each XCD has
-// its own L2. Kernels whose neighbouring workgroups share tables (the tANS tables of one segment)
-// renumber their workgroups so that every XCD takes one contiguous range: XCD x gets the logical
-// blocks x*(n/8) + min(x, n%8) onward, in order.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ uint32_t xcd_block() {
-    const uint32_t n = gridDim.x, b = blockIdx.x, x = b & 7u, k = b >> 3;
-    return x * (n >> 3) + min(x, n & 7u) + k;
-}
 
 struct BitW {  // LSB-first FSE bitstream writer (zfse/bitstream.h)
     uint8_t* buf;

@@ -37,14 +37,17 @@ struct Escapes {  // decompression side: sorted escaped stream positions and the
     uint64_t n;
 };
 
-// One target: compress (DECOMP = false) or reconstruct (DECOMP = true).
+// One target: compress (DECOMP = false) or reconstruct (DECOMP = true), in place; returns the
+// reconstructed value (an escaped point keeps its original).
 template <typename T, bool DECOMP>
-__device__ __forceinline__ void code_point(T* __restrict__ data, uint64_t pos, T pred, uint64_t s, bool vlane,
-                                           bool skip, const LevelEb& L, int16_t* __restrict__ codes,
-                                           T* __restrict__ escval, Escapes esc, const T* __restrict__ unpred) {
+__device__ __forceinline__ T code_point(T* __restrict__ data, uint64_t pos, T pred, uint64_t s, bool vlane, bool skip,
+                                        const LevelEb& L, int16_t* __restrict__ codes, T* __restrict__ escval,
+                                        const Escapes& esc, const T* __restrict__ unpred) {
     if (DECOMP) {
         const int q = codes[s];
-        data[pos] = (q == -32768) ? escaped_value(s, esc.idx, esc.n, unpred) : (T)fma((double)q, L.ebx2, (double)pred);
+        const T v = (q == -32768) ? escaped_value(s, esc.idx, esc.n, unpred) : (T)fma((double)q, L.ebx2, (double)pred);
+        data[pos] = v;
+        return v;
     } else {
         const T ori = data[pos];
         T dec = ori;
@@ -52,113 +55,215 @@ __device__ __forceinline__ void code_point(T* __restrict__ data, uint64_t pos, T
         codes[s] = (int16_t)q;
         if (q == -32768) {
             escval[s] = ori;
-        } else if (!skip) {
-            data[pos] = dec;
+            return ori;
         }
+        if (!skip) data[pos] = dec;
+        return dec;
     }
 }
 
-// Row passes (InterpolationDecomposition.hpp:604-768, 819-1135, 1138-1375).
-// mode 0: every target row except the even-n linear end row when it reads a same-pass row;
-// mode 1: that linear end row; mode 2: the 3D cubic even-n boundary plane (scalar quantizer).
-template <typename T, bool DECOMP>
-__global__ void __launch_bounds__(256) k_rows(PassDesc p, int mode, uint64_t total, T* __restrict__ data,
-                                              int16_t* __restrict__ codes, T* __restrict__ escval,
-                                              const LevelEb* __restrict__ lvt, int avx2, Escapes esc,
-                                              const T* __restrict__ unpred) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const LevelEb L = lvt[p.level];
-    const uint64_t n = p.n, sp = p.sp, len = p.len;
-    const uint64_t k = i % len;
-    const uint64_t t = i / len;
-    const bool cubic = p.cubic != 0, skip = p.skip != 0;
-    const bool vl = avx2 != 0;
-    const bool cub3_tail = cubic && !p.is2d && (n % 2 == 0) && n > 2;
-    const uint64_t rows = n / 2 - (cub3_tail ? 1 : 0);  // stream rows per outer index
-    uint64_t o, g;
-    if (mode == 0) {
-        const bool lin_dep_tail = !cubic && !skip && (n % 2 == 0) && n >= 4;
-        const uint64_t rows_main = rows - (lin_dep_tail ? 1 : 0);
-        o = t / rows_main;
-        g = 2 * (t % rows_main) + 1;
-    } else {
-        o = t;
-        g = n - 1;
+// Row-pass geometry (InterpolationDecomposition.hpp:604-768): stream rows per outer index, and the
+// end rows that are coded after the others. cub3_tail: cubic 3D pass with even n, whose row n-1 is
+// coded by the scalar quantizer after every other row of the pass. lin_dep_tail: linear pass with
+// even n, whose row n-1 reads row n-3 of the same pass.
+struct RowsGeom {
+    uint64_t rows, rows_main;
+    bool cub3_tail, lin_dep_tail;
+};
+__host__ __device__ inline RowsGeom rows_geom(const PassDesc& p) {
+    RowsGeom G;
+    G.cub3_tail = p.cubic && !p.is2d && (p.n % 2 == 0) && p.n > 2;
+    G.rows = p.n / 2 - (G.cub3_tail ? 1 : 0);
+    G.lin_dep_tail = !p.cubic && !p.skip && (p.n % 2 == 0) && p.n >= 4;
+    G.rows_main = G.rows - (G.lin_dep_tail ? 1 : 0);
+    return G;
+}
+
+// Prediction of the row-pass target at odd pass-axis index g; R(gg) reads index gg of the target's
+// row (InterpolationDecomposition.hpp:819-1135 cubic, 1138-1375 linear). mode2: the cub3_tail row.
+template <typename T, typename RF>
+__device__ __forceinline__ T rows_pred(const PassDesc& p, bool mode2, uint64_t g, bool& vlane, RF R) {
+    const uint64_t n = p.n;
+    if (!p.cubic) {
+        if (g + 1 < n) return p_lin(R(g - 1), R(g + 1));
+        if (n < 3) return R(n - 2);
+        return p_l1(R(n - 1 - (p.skip ? 3 : 2)), R(n - 2));
     }
-    const uint64_t ob = o * p.so + k * p.sk;
-    auto R = [&](uint64_t gg) { return data[ob + gg * sp]; };
-    uint64_t s;
-    if (mode == 2)
-        s = p.code_base + (p.no * rows + o) * len + k;
-    else
-        s = p.code_base + (o * rows + (g - 1) / 2) * len + k;
-    T pred;
-    bool vlane = vl;
-    if (!cubic) {
-        if (g + 1 < n) pred = p_lin(R(g - 1), R(g + 1));
-        else if (n < 3) pred = R(n - 2);
-        else pred = p_l1(R(n - 1 - (skip ? 3 : 2)), R(n - 2));
-    } else if (mode == 2) {
-        pred = p_l1(R(n - 4), R(n - 2));
+    if (mode2) {
         vlane = false;
-    } else if (g == 1) {
-        pred = (n >= 5) ? p_q1(R(0), R(2), R(4)) : (n >= 3) ? p_lin(R(0), R(2)) : R(0);
-    } else if (g + 3 < n) {
-        pred = p_cub(R(g - 3), R(g - 1), R(g + 1), R(g + 3));
-    } else if (n % 2 == 1) {
-        pred = p_q2(R(n - 5), R(n - 3), R(n - 1));
-    } else if (g == n - 3) {
-        pred = p_q2(R(n - 6), R(n - 4), R(n - 2));
-    } else {
-        pred = p_l1(R(n - 4), R(n - 2));  // 2D end row, vector kernel
+        return p_l1(R(n - 4), R(n - 2));
     }
-    code_point<T, DECOMP>(data, ob + g * sp, pred, s, vlane, skip, L, codes, escval, esc, unpred);
+    if (g == 1) return (n >= 5) ? p_q1(R(0), R(2), R(4)) : (n >= 3) ? p_lin(R(0), R(2)) : R(0);
+    if (g + 3 < n) return p_cub(R(g - 3), R(g - 1), R(g + 1), R(g + 3));
+    if (n % 2 == 1) return p_q2(R(n - 5), R(n - 3), R(n - 1));
+    if (g == n - 3) return p_q2(R(n - 6), R(n - 4), R(n - 2));
+    return p_l1(R(n - 4), R(n - 2));  // 2D end row, vector kernel
 }
 
-// Line passes (Interpolation_quantizer.inl:32-180 linear, :184-377 cubic).
-template <typename T, bool DECOMP>
-__global__ void __launch_bounds__(256) k_lines(PassDesc p, uint64_t total, T* __restrict__ data,
-                                               int16_t* __restrict__ codes, T* __restrict__ escval,
-                                               const LevelEb* __restrict__ lvt, int avx2, Escapes esc,
-                                               const T* __restrict__ unpred) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const LevelEb L = lvt[p.level];
-    const uint64_t n = p.n, sp = p.sp;
-    const uint64_t odd = n / 2, even = n - odd;
-    const uint64_t t = i % odd, line = i / odd;
-    const uint64_t base = (line / p.n2) * p.s1 + (line % p.n2) * p.s2;
-    auto X = [&](uint64_t e) { return data[base + e * sp]; };
-    const uint64_t s = p.code_base + line * odd + t;
-    const bool vl = avx2 != 0;
-    T pred;
-    bool vlane;
+// Stream index of row-pass target (o, g, k): the pass's rows in (o, g) order, the cub3_tail rows last.
+__device__ __forceinline__ uint64_t rows_code(const PassDesc& p, const RowsGeom& G, bool mode2, uint64_t o, uint64_t g,
+                                              uint64_t k) {
+    return mode2 ? p.code_base + (p.no * G.rows + o) * p.len + k : p.code_base + (o * G.rows + (g - 1) / 2) * p.len + k;
+}
+
+// Prediction of line-pass target t (index 2t+1 of its line); X(e) reads index e of the line
+// (Interpolation_quantizer.inl:32-180 linear, :184-377 cubic). vlane: coded by a vector lane.
+template <typename T, typename XF>
+__device__ __forceinline__ T lines_pred(const PassDesc& p, uint64_t t, bool vl, bool& vlane, XF X) {
+    const uint64_t n = p.n, odd = n / 2, even = n - odd;
     if (!p.cubic) {
         const uint64_t vec_end = (sizeof(T) == 4) ? 8 * ((odd - 1) / 8) : 4 * ((odd - 1) / 4);
         if (t + 1 < odd) {
-            pred = p_lin(X(2 * t), X(2 * t + 2));
             vlane = vl && t < vec_end;
-        } else {
-            pred = (n < 3) ? X(2 * (even - 1)) : p_l1(X(2 * (even - 2)), X(2 * (even - 1)));
-            vlane = false;
+            return p_lin(X(2 * t), X(2 * t + 2));
         }
-    } else {
-        const uint64_t vec = (sizeof(T) == 4) ? 8 : 4;
-        const uint64_t vec_t = even >= vec + 3 ? vec * ((even - 3) / vec) : 0;  // targets t in [1, vec_t]
         vlane = false;
-        if (t == 0) {
-            pred = (even < 2) ? X(0) : (even < 3) ? p_lin(X(0), X(2)) : p_q1(X(0), X(2), X(4));
-        } else if (t + 2 < even) {
-            pred = p_cub(X(2 * t - 2), X(2 * t), X(2 * t + 2), X(2 * t + 4));
-            vlane = vl && t <= vec_t;
-        } else if (odd < even || t + 2 == odd) {
-            pred = p_q2(X(2 * (even - 3)), X(2 * (even - 2)), X(2 * (even - 1)));
-        } else {
-            pred = p_l1(X(2 * (even - 2)), X(2 * (even - 1)));
+        return (n < 3) ? X(2 * (even - 1)) : p_l1(X(2 * (even - 2)), X(2 * (even - 1)));
+    }
+    const uint64_t vec = (sizeof(T) == 4) ? 8 : 4;
+    const uint64_t vec_t = even >= vec + 3 ? vec * ((even - 3) / vec) : 0;  // targets t in [1, vec_t]
+    vlane = false;
+    if (t == 0) return (even < 2) ? X(0) : (even < 3) ? p_lin(X(0), X(2)) : p_q1(X(0), X(2), X(4));
+    if (t + 2 < even) {
+        vlane = vl && t <= vec_t;
+        return p_cub(X(2 * t - 2), X(2 * t), X(2 * t + 2), X(2 * t + 4));
+    }
+    if (odd < even || t + 2 == odd) return p_q2(X(2 * (even - 3)), X(2 * (even - 2)), X(2 * (even - 1)));
+    return p_l1(X(2 * (even - 2)), X(2 * (even - 1)));
+}
+
+// Row passes, one thread a target (every field shape and direction).
+// mode 0: every target row except the lin_dep_tail row; mode 1: that row; mode 2: the cub3_tail row.
+template <typename T, bool DECOMP>
+__global__ void __launch_bounds__(256) k_rows(PassDesc p, int mode, uint64_t total, T* data,
+                                              int16_t* __restrict__ codes, T* __restrict__ escval,
+                                              const LevelEb* __restrict__ lvt, int avx2, Escapes esc,
+                                              const T* __restrict__ unpred) {
+    const uint64_t i = (uint64_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const LevelEb L = lvt[p.level];
+    const RowsGeom G = rows_geom(p);
+    const uint64_t k = i % p.len, t = i / p.len;
+    uint64_t o, g;
+    if (mode == 0) {
+        o = t / G.rows_main;
+        g = 2 * (t % G.rows_main) + 1;
+    } else {
+        o = t;
+        g = p.n - 1;
+    }
+    const uint64_t ob = o * p.so + k * p.sk;
+    bool vlane = avx2 != 0;
+    const T pred = rows_pred<T>(p, mode == 2, g, vlane, [&](uint64_t gg) { return data[ob + gg * p.sp]; });
+    code_point<T, DECOMP>(data, ob + g * p.sp, pred, rows_code(p, G, mode == 2, o, g, k), vlane, p.skip != 0, L, codes,
+                          escval, esc, unpred);
+}
+
+// Line passes, one thread a target (every field shape and direction).
+template <typename T, bool DECOMP>
+__global__ void __launch_bounds__(256) k_lines(PassDesc p, uint64_t total, T* data, int16_t* __restrict__ codes,
+                                               T* __restrict__ escval, const LevelEb* __restrict__ lvt, int avx2,
+                                               Escapes esc, const T* __restrict__ unpred) {
+    const uint64_t i = (uint64_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const LevelEb L = lvt[p.level];
+    const uint64_t odd = p.n / 2;
+    const uint64_t t = i % odd, line = i / odd;
+    const uint64_t base = (line / p.n2) * p.s1 + (line % p.n2) * p.s2;
+    bool vlane;
+    const T pred = lines_pred<T>(p, t, avx2 != 0, vlane, [&](uint64_t e) { return data[base + e * p.sp]; });
+    code_point<T, DECOMP>(data, base + (2 * t + 1) * p.sp, pred, p.code_base + line * odd + t, vlane, p.skip != 0, L,
+                          codes, escval, esc, unpred);
+}
+
+// ---- the y and x passes of a level in one kernel: 3D and 2D fields, passes z, y, x ---------------
+// Level-grid point (gz, gy, gx) is element gz*ez + gy*ey + gx*ex. After the level's z pass (its own
+// launch; it reads only even planes), the y pass codes (any gz, odd gy, even gx) from rows of the
+// same plane and the x pass codes (any gz, any gy, odd gx) from its own line. A workgroup owns a tile
+// of ty whole rows of one plane and runs both passes on it in place:
+//   1. it stages the even columns of the even rows of the tile and of kHalo rows either side (the
+//      coarser points and the z pass's results) in LDS;
+//   2. it codes the y-pass targets of its odd rows, their neighbours read from LDS, and keeps their
+//      values there;
+//   3. it codes the x-pass targets of its rows, their neighbours (the even columns) read from LDS.
+// No tile reads a point that another tile of the launch writes. The linear pass with even n codes row
+// n-1 from row n-3 of the same pass; the tiles are cut so that one tile holds both, and the end row
+// is coded after the others. Every point is predicted, quantized (same lane rules) and placed in the
+// stream exactly as the per-pass kernels do it.
+constexpr int kHalo = 3;
+
+// Rows of the tile of y block yb: [y0, y1). The last block takes the remainder, at most ty + 2 rows.
+__host__ __device__ inline void level_rows(const LevelDesc& d, uint32_t yb, int64_t& y0, int64_t& y1) {
+    y0 = (int64_t)yb * d.ty;
+    y1 = (yb + 1 == d.nyb) ? (int64_t)d.ny : y0 + d.ty;
+}
+
+template <typename T, bool DECOMP>
+__global__ void __launch_bounds__(256) k_yx(LevelDesc d, T* __restrict__ data, int16_t* __restrict__ codes,
+                                            T* __restrict__ escval, const LevelEb* __restrict__ lvt, int avx2,
+                                            Escapes esc, const T* __restrict__ unpred) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* P = reinterpret_cast<T*>(smem);
+    const uint64_t b = xcd_block();  // consecutive workgroups of an XCD take neighbouring row blocks
+    const uint32_t yb = (uint32_t)(b % d.nyb);
+    const uint64_t gz = b / d.nyb;
+    const int64_t ny = (int64_t)d.ny;
+    int64_t y0, y1;
+    level_rows(d, yb, y0, y1);
+    const int nxe = (int)d.nxe;  // even columns
+    auto at = [&](int64_t gy, int64_t gx) { return (int)(gy - y0 + kHalo) * nxe + (int)(gx >> 1); };
+    const LevelEb L = lvt[d.py.level];
+    const bool vl = avx2 != 0;
+    const uint64_t pbase = gz * d.ez;
+
+    // 1. even rows of [y0 - kHalo, y1 + kHalo), even columns
+    {
+        const int64_t ylo = max((int64_t)0, y0 - kHalo), yhi = min(ny, y1 + kHalo);
+        const int64_t ey0 = (ylo + 1) & ~(int64_t)1;
+        const int nre = yhi > ey0 ? (int)((yhi - ey0 + 1) / 2) : 0;
+        for (int j = threadIdx.x; j < nre * nxe; j += blockDim.x) {
+            const int64_t gy = ey0 + 2 * (j / nxe), gx = 2 * (j % nxe);
+            P[at(gy, gx)] = data[pbase + (uint64_t)gy * d.ey + (uint64_t)gx * d.ex];
         }
     }
-    code_point<T, DECOMP>(data, base + (2 * t + 1) * sp, pred, s, vlane, p.skip != 0, L, codes, escval, esc, unpred);
+    __syncthreads();
+
+    // 2. y pass over the odd rows of the tile; the linear even-n end row last
+    {
+        const RowsGeom G = rows_geom(d.py);
+        const bool ytail = G.lin_dep_tail && ny - 1 >= y0 && ny - 1 < y1;
+        const int64_t oy0 = y0 + 1;  // y0 is even
+        const int nro = y1 > oy0 ? (int)((y1 - oy0 + 1) / 2) - (ytail ? 1 : 0) : 0;
+        auto y_target = [&](int64_t gy, int64_t gx) {
+            const bool mode2 = G.cub3_tail && gy == ny - 1;
+            bool vlane = vl;
+            const T pred = rows_pred<T>(d.py, mode2, (uint64_t)gy, vlane, [&](uint64_t gg) { return P[at((int64_t)gg, gx)]; });
+            P[at(gy, gx)] = code_point<T, DECOMP>(data, pbase + (uint64_t)gy * d.ey + (uint64_t)gx * d.ex, pred,
+                                                  rows_code(d.py, G, mode2, gz, (uint64_t)gy, (uint64_t)gx / 2), vlane,
+                                                  d.py.skip != 0, L, codes, escval, esc, unpred);
+        };
+        for (int j = threadIdx.x; j < nro * nxe; j += blockDim.x) y_target(oy0 + 2 * (j / nxe), 2 * (j % nxe));
+        if (ytail) {
+            __syncthreads();
+            for (int j = threadIdx.x; j < nxe; j += blockDim.x) y_target(ny - 1, 2 * j);
+        }
+    }
+    __syncthreads();
+
+    // 3. x pass over the rows of the tile
+    {
+        const uint64_t odd = d.px.n / 2;
+        const int nt = (int)odd, nrow = (int)(y1 - y0);
+        for (int j = threadIdx.x; j < nrow * nt; j += blockDim.x) {
+            const int64_t gy = y0 + j / nt;
+            const uint64_t t = (uint64_t)(j % nt);
+            bool vlane;
+            const T pred = lines_pred<T>(d.px, t, vl, vlane, [&](uint64_t e) { return P[at(gy, (int64_t)e)]; });
+            const uint64_t line = gz * d.ny + (uint64_t)gy;
+            code_point<T, DECOMP>(data, pbase + (uint64_t)gy * d.ey + (2 * t + 1) * d.ex, pred,
+                                  d.px.code_base + line * odd + t, vlane, d.px.skip != 0, L, codes, escval, esc, unpred);
+        }
+    }
 }
 
 // First point when the field has no anchor grid (InterpolationDecomposition.hpp:171-172, 67-68):
@@ -403,20 +508,16 @@ static void launch_pass(const PassDesc& p, void* data, int16_t* codes, void* esc
                       st, p, total, d, codes, ev, lv, avx2, e, un);
         return;
     }
-    const uint64_t n = p.n;
-    const bool cub3_tail = p.cubic && !p.is2d && (n % 2 == 0) && n > 2;
-    const uint64_t rows = n / 2 - (cub3_tail ? 1 : 0);
-    const bool lin_dep_tail = !p.cubic && !p.skip && (n % 2 == 0) && n >= 4;
-    const uint64_t rows_main = rows - (lin_dep_tail ? 1 : 0);
-    const uint64_t t0 = p.no * rows_main * p.len;
+    const RowsGeom G = rows_geom(p);
+    const uint64_t t0 = p.no * G.rows_main * p.len;
     if (t0)
         FZ_LAUNCH("interp_rows", t0 * per_target, (k_rows<T, DECOMP>), dim3(blocks_for(t0)), dim3(256), 0, st, p, 0,
                   t0, d, codes, ev, lv, avx2, e, un);
     const uint64_t t1 = p.no * p.len;
-    if (lin_dep_tail && t1)
+    if (G.lin_dep_tail && t1)
         FZ_LAUNCH("interp_rows_tail", t1 * per_target, (k_rows<T, DECOMP>), dim3(blocks_for(t1)), dim3(256), 0, st,
                   p, 1, t1, d, codes, ev, lv, avx2, e, un);
-    if (cub3_tail && t1)
+    if (G.cub3_tail && t1)
         FZ_LAUNCH("interp_rows_plane", t1 * per_target, (k_rows<T, DECOMP>), dim3(blocks_for(t1)), dim3(256), 0, st,
                   p, 2, t1, d, codes, ev, lv, avx2, e, un);
 }
@@ -430,6 +531,57 @@ void interp_pass(int dtype, bool decomp, const PassDesc& p, void* data, int16_t*
     } else {
         if (decomp) launch_pass<double, true>(p, data, codes, escval, lv, avx2, esc_idx, nesc, unpred, st);
         else launch_pass<double, false>(p, data, codes, escval, lv, avx2, esc_idx, nesc, unpred, st);
+    }
+    FZ_HIP(hipGetLastError());
+}
+
+// Tiles of a level's y+x launch for elements of es bytes: ty rows (even, at most 32) such that the
+// rows of a tile, its last-block surplus of 2 and the 2 kHalo halo rows fit 64 KiB of LDS; ty = 0
+// when fewer than 4 rows fit (the passes then run separately).
+static LevelDesc level_tiles(LevelDesc d, size_t es) {
+    constexpr size_t kBudget = size_t(64) << 10;
+    d.nxe = (uint32_t)((d.nx + 1) / 2);
+    const size_t row = (size_t)d.nxe * es;
+    const size_t fit = row ? kBudget / row : 0;
+    d.ty = fit >= 2 + 2 * kHalo + 4 ? (uint32_t)std::min<size_t>(32, (fit - 2 - 2 * kHalo) & ~size_t(1)) : 0;
+    if (!d.ty) return d;
+    d.nyb = (uint32_t)((d.ny + d.ty - 1) / d.ty);
+    // the linear even-n end row reads row ny-3 of the same pass: keep both in the last tile
+    if (d.nyb > 1 && rows_geom(d.py).lin_dep_tail && d.ny - (uint64_t)(d.nyb - 1) * d.ty <= 2) d.nyb--;
+    const uint64_t rows = std::min<uint64_t>(d.ny, (uint64_t)d.ty + 2) + 2 * kHalo;
+    d.lds_bytes = (uint32_t)(rows * row);
+    return d;
+}
+
+template <typename T, bool DECOMP>
+static void launch_yx(const LevelDesc& d, T* data, int16_t* codes, T* escval, const LevelEb* lv, int avx2,
+                      Escapes e, const T* unpred, hipStream_t st) {
+    const uint64_t tiles = d.nz * d.nyb;
+    const uint64_t ncodes = d.py.ncodes + d.px.ncodes;
+    if (!tiles || !ncodes) return;
+    if (tiles > 0x7fffffffu) throw std::length_error("level has more tiles than one launch takes");
+    const uint64_t per_target = DECOMP ? sizeof(T) + 2 : 2 * sizeof(T) + 2;
+    FZ_LAUNCH("interp_yx", ncodes * per_target, (k_yx<T, DECOMP>), dim3((unsigned)tiles), dim3(256), d.lds_bytes, st,
+              d, data, codes, escval, lv, avx2, e, unpred);
+}
+
+void interp_level(int dtype, bool decomp, const LevelDesc& level, void* data, int16_t* codes, void* escval,
+                  const LevelEb* lv, int avx2, const uint64_t* esc_idx, uint64_t nesc, const void* unpred,
+                  hipStream_t st) {
+    interp_pass(dtype, decomp, level.pz, data, codes, escval, lv, avx2, esc_idx, nesc, unpred, st);
+    const LevelDesc d = level_tiles(level, dtype == kF32 ? 4 : 8);
+    if (!d.ty) {
+        interp_pass(dtype, decomp, d.py, data, codes, escval, lv, avx2, esc_idx, nesc, unpred, st);
+        interp_pass(dtype, decomp, d.px, data, codes, escval, lv, avx2, esc_idx, nesc, unpred, st);
+        return;
+    }
+    const Escapes e{esc_idx, nesc};
+    if (dtype == kF32) {
+        if (decomp) launch_yx<float, true>(d, (float*)data, codes, (float*)escval, lv, avx2, e, (const float*)unpred, st);
+        else launch_yx<float, false>(d, (float*)data, codes, (float*)escval, lv, avx2, e, (const float*)unpred, st);
+    } else {
+        if (decomp) launch_yx<double, true>(d, (double*)data, codes, (double*)escval, lv, avx2, e, (const double*)unpred, st);
+        else launch_yx<double, false>(d, (double*)data, codes, (double*)escval, lv, avx2, e, (const double*)unpred, st);
     }
     FZ_HIP(hipGetLastError());
 }

@@ -109,6 +109,11 @@ class DevArena {
 void interp_pass(int dtype, bool decomp, const PassDesc& p, void* data, int16_t* codes, void* escval,
                  const LevelEb* lv, int avx2, const uint64_t* esc_idx, uint64_t nesc, const void* unpred,
                  hipStream_t st);
+// One level of a 3D or 2D direction-0 field: the z pass, then the y and x passes in one launch (or
+// as separate passes when a row is too wide for the tiles' LDS). In place, like interp_pass.
+void interp_level(int dtype, bool decomp, const LevelDesc& level, void* data, int16_t* codes, void* escval,
+                  const LevelEb* lv, int avx2, const uint64_t* esc_idx, uint64_t nesc, const void* unpred,
+                  hipStream_t st);
 void interp_first(int dtype, bool decomp, void* data, int16_t* codes, void* escval, const double* eb_dev,
                   const uint64_t* esc_idx, uint64_t nesc, const void* unpred, hipStream_t st);
 void interp_anchors(int dtype, bool decomp, void* data, void* unpred, const uint64_t cnt[3], const uint64_t str[3],
