@@ -1,12 +1,14 @@
 // Interpolation-predict + linear-quantize kernels for gfx950.
 //
-// Every pass of every level is one kernel launch (two for row passes whose even-n linear end row
-// reads a row the same pass reconstructs). A thread owns one target point: it forms the prediction
-// from the already reconstructed coarser points, quantizes it exactly as the reference does
-// (InterpolationDecomposition.hpp with the AVX2 kernels of Interpolation_quantizer.inl, or the
-// scalar LinearQuantizer for the points the reference handles one at a time), writes the
-// reconstructed value back to the field and writes its int16 code straight to its position in the
-// reference's code stream, computed in closed form from the pass's loop nest.
+// Every target point is predicted from the already reconstructed coarser points, quantized exactly
+// as the reference does (InterpolationDecomposition.hpp with the AVX2 kernels of
+// Interpolation_quantizer.inl, or the scalar LinearQuantizer for the points the reference handles
+// one at a time), reconstructed in place, and its int16 code written straight to its position in the
+// reference's code stream, computed in closed form from the pass's loop nest. Two kernel shapes:
+//  - k_rows / k_lines: one launch per pass (two for row passes whose even-n linear end row reads a
+//    row the same pass reconstructs), one thread a target; every field shape and direction.
+//  - k_yx: the y and x passes of a level of a 3D or 2D field with direction 0 in one launch over
+//    LDS-staged tiles of whole rows (after the level's z pass, a k_rows launch).
 //
 // Compiled with -ffp-contract=off: every multiply/add is rounded on its own unless the reference
 // fuses it (explicit fma).
@@ -180,17 +182,19 @@ __global__ void __launch_bounds__(256) k_lines(PassDesc p, uint64_t total, T* da
 // Level-grid point (gz, gy, gx) is element gz*ez + gy*ey + gx*ex. After the level's z pass (its own
 // launch; it reads only even planes), the y pass codes (any gz, odd gy, even gx) from rows of the
 // same plane and the x pass codes (any gz, any gy, odd gx) from its own line. A workgroup owns a tile
-// of ty whole rows of one plane and runs both passes on it in place:
-//   1. it stages the even columns of the even rows of the tile and of kHalo rows either side (the
-//      coarser points and the z pass's results) in LDS;
-//   2. it codes the y-pass targets of its odd rows, their neighbours read from LDS, and keeps their
-//      values there;
-//   3. it codes the x-pass targets of its rows, their neighbours (the even columns) read from LDS.
-// No tile reads a point that another tile of the launch writes. The linear pass with even n codes row
-// n-1 from row n-3 of the same pass; the tiles are cut so that one tile holds both, and the end row
-// is coded after the others. Every point is predicted, quantized (same lane rules) and placed in the
-// stream exactly as the per-pass kernels do it.
-constexpr int kHalo = 3;
+// of whole rows [y0, y1) of one plane and runs both passes on it in LDS:
+//   1. it loads the tile's rows (compress: every column, originals included; decompress: the even
+//      columns of the even rows, the coarser points and the z pass's results), and the even columns
+//      of the even rows within 2 rows of the tile, which the y pass reads;
+//   2. it codes the y-pass targets of its odd rows, neighbours from LDS, values kept in LDS;
+//   3. it codes the x-pass targets of its rows, neighbours from LDS;
+//   4. it stores its rows back whole (compress: the rows holding reconstructed values).
+// Every line of the tile is read once and written once. No tile reads a point that another tile of
+// the launch writes (a tile's stores leave the points its neighbours read unchanged). The linear
+// pass with even n codes row n-1 from row n-3 of the same pass; the tiles are cut so that one tile
+// holds both, and the end row is coded after the others. Every point is predicted, quantized (same
+// lane rules) and placed in the stream exactly as the per-pass kernels do it.
+constexpr int kHaloRows = 2;  // even rows y0-2 and y1, y1+2 are read; LDS rows span [y0-2, y1+2]
 
 // Rows of the tile of y block yb: [y0, y1). The last block takes the remainder, at most ty + 2 rows.
 __host__ __device__ inline void level_rows(const LevelDesc& d, uint32_t yb, int64_t& y0, int64_t& y1) {
@@ -198,32 +202,108 @@ __host__ __device__ inline void level_rows(const LevelDesc& d, uint32_t yb, int6
     y1 = (yb + 1 == d.nyb) ? (int64_t)d.ny : y0 + d.ty;
 }
 
+// Codes one target held in LDS: compress quantizes the original in `cell` and leaves the
+// reconstruction there (the original when escaped or when the pass writes nothing back); decompress
+// puts there the value of code q, which the caller has loaded.
 template <typename T, bool DECOMP>
-__global__ void __launch_bounds__(256) k_yx(LevelDesc d, T* __restrict__ data, int16_t* __restrict__ codes,
+__device__ __forceinline__ void code_cell(T& cell, T pred, int q, uint64_t s, bool vlane, bool skip, const LevelEb& L,
+                                          int16_t* __restrict__ codes, T* __restrict__ escval, const Escapes& esc,
+                                          const T* __restrict__ unpred) {
+    if (DECOMP) {
+        cell = (q == -32768) ? escaped_value(s, esc.idx, esc.n, unpred) : (T)fma((double)q, L.ebx2, (double)pred);
+    } else {
+        const T ori = cell;
+        T dec = ori;
+        const int qq = quantize(ori, pred, L, vlane, dec);
+        codes[s] = (int16_t)qq;
+        if (qq == -32768) escval[s] = ori;
+        else if (!skip) cell = dec;
+    }
+}
+
+// Walks j = threadIdx.x, threadIdx.x + blockDim.x, ... over a grid of rows of n columns, row-major,
+// without a division per step.
+struct Walk {
+    int row, col, dr, dc, n;
+    __device__ explicit Walk(int n_) : n(n_) {
+        row = (int)threadIdx.x / n;
+        col = (int)threadIdx.x % n;
+        dr = (int)blockDim.x / n;
+        dc = (int)blockDim.x % n;
+    }
+    __device__ void next() {
+        row += dr;
+        col += dc;
+        if (col >= n) {
+            col -= n;
+            row++;
+        }
+    }
+};
+constexpr int kBatch = 8;  // loads a thread keeps in flight
+
+constexpr int kYxThreads = 512;
+template <typename T, bool DECOMP>
+__global__ void __launch_bounds__(kYxThreads) k_yx(LevelDesc d, T* __restrict__ data, int16_t* __restrict__ codes,
                                             T* __restrict__ escval, const LevelEb* __restrict__ lvt, int avx2,
                                             Escapes esc, const T* __restrict__ unpred) {
     extern __shared__ __align__(16) unsigned char smem[];
-    T* P = reinterpret_cast<T*>(smem);
+    T* F = reinterpret_cast<T*>(smem);
     const uint64_t b = xcd_block();  // consecutive workgroups of an XCD take neighbouring row blocks
     const uint32_t yb = (uint32_t)(b % d.nyb);
     const uint64_t gz = b / d.nyb;
     const int64_t ny = (int64_t)d.ny;
+    const int nx = (int)d.nx, nxe = (int)d.nxe;
     int64_t y0, y1;
     level_rows(d, yb, y0, y1);
-    const int nxe = (int)d.nxe;  // even columns
-    auto at = [&](int64_t gy, int64_t gx) { return (int)(gy - y0 + kHalo) * nxe + (int)(gx >> 1); };
+    const int nrow = (int)(y1 - y0);
+    auto at = [&](int64_t gy, int64_t gx) { return (int)(gy - y0 + kHaloRows) * nx + (int)gx; };
+    const uint64_t pbase = gz * d.ez;
+    auto gpos = [&](int64_t gy, int64_t gx) { return pbase + (uint64_t)gy * d.ey + (uint64_t)gx * d.ex; };
     const LevelEb L = lvt[d.py.level];
     const bool vl = avx2 != 0;
-    const uint64_t pbase = gz * d.ez;
 
-    // 1. even rows of [y0 - kHalo, y1 + kHalo), even columns
+    // 1. loads, kBatch in flight per thread
+    if (!DECOMP && nrow > 0) {
+        Walk w(nx);
+        while (w.row < nrow) {
+            T v[kBatch];
+            int r[kBatch], c[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) {
+                r[u] = w.row;
+                c[u] = w.col;
+                if (r[u] < nrow) v[u] = data[gpos(y0 + r[u], c[u])];
+                w.next();
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; u++)
+                if (r[u] < nrow) F[at(y0 + r[u], c[u])] = v[u];
+        }
+    }
     {
-        const int64_t ylo = max((int64_t)0, y0 - kHalo), yhi = min(ny, y1 + kHalo);
-        const int64_t ey0 = (ylo + 1) & ~(int64_t)1;
-        const int nre = yhi > ey0 ? (int)((yhi - ey0 + 1) / 2) : 0;
-        for (int j = threadIdx.x; j < nre * nxe; j += blockDim.x) {
-            const int64_t gy = ey0 + 2 * (j / nxe), gx = 2 * (j % nxe);
-            P[at(gy, gx)] = data[pbase + (uint64_t)gy * d.ey + (uint64_t)gx * d.ex];
+        // even rows of [y0-2, y1+2] (decompress) or of the halo only (compress), even columns
+        const int64_t ylo = max((int64_t)0, y0 - kHaloRows), yhi = min(ny, y1 + kHaloRows + 1);
+        const int nre = (int)((yhi - ylo + 1) / 2);  // ylo is even
+        if (nre > 0) {
+            Walk w(nxe);
+            while (w.row < nre) {
+                T v[kBatch];
+                int64_t gy[kBatch];
+                int gx[kBatch];
+                bool ok[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) {
+                    gy[u] = ylo + 2 * w.row;
+                    gx[u] = 2 * w.col;
+                    ok[u] = w.row < nre && (DECOMP || gy[u] < y0 || gy[u] >= y1);
+                    if (ok[u]) v[u] = data[gpos(gy[u], gx[u])];
+                    w.next();
+                }
+#pragma unroll
+                for (int u = 0; u < kBatch; u++)
+                    if (ok[u]) F[at(gy[u], gx[u])] = v[u];
+            }
         }
     }
     __syncthreads();
@@ -234,34 +314,84 @@ __global__ void __launch_bounds__(256) k_yx(LevelDesc d, T* __restrict__ data, i
         const bool ytail = G.lin_dep_tail && ny - 1 >= y0 && ny - 1 < y1;
         const int64_t oy0 = y0 + 1;  // y0 is even
         const int nro = y1 > oy0 ? (int)((y1 - oy0 + 1) / 2) - (ytail ? 1 : 0) : 0;
-        auto y_target = [&](int64_t gy, int64_t gx) {
-            const bool mode2 = G.cub3_tail && gy == ny - 1;
-            bool vlane = vl;
-            const T pred = rows_pred<T>(d.py, mode2, (uint64_t)gy, vlane, [&](uint64_t gg) { return P[at((int64_t)gg, gx)]; });
-            P[at(gy, gx)] = code_point<T, DECOMP>(data, pbase + (uint64_t)gy * d.ey + (uint64_t)gx * d.ex, pred,
-                                                  rows_code(d.py, G, mode2, gz, (uint64_t)gy, (uint64_t)gx / 2), vlane,
-                                                  d.py.skip != 0, L, codes, escval, esc, unpred);
+        // codes of kBatch targets are loaded (decompress) before any of them is coded
+        auto y_rows = [&](int64_t first, int rows) {
+            if (rows <= 0) return;
+            Walk w(nxe);
+            while (w.row < rows) {
+                int q[kBatch];
+                int64_t gy[kBatch];
+                int gx[kBatch];
+                bool ok[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) {
+                    ok[u] = w.row < rows;
+                    gy[u] = first + 2 * w.row;
+                    gx[u] = 2 * w.col;
+                    const bool mode2 = G.cub3_tail && gy[u] == ny - 1;
+                    if (DECOMP && ok[u]) q[u] = codes[rows_code(d.py, G, mode2, gz, (uint64_t)gy[u], (uint64_t)gx[u] / 2)];
+                    w.next();
+                }
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) {
+                    if (!ok[u]) continue;
+                    const bool mode2 = G.cub3_tail && gy[u] == ny - 1;
+                    bool vlane = vl;
+                    const int64_t cx = gx[u];
+                    const T pred = rows_pred<T>(d.py, mode2, (uint64_t)gy[u], vlane,
+                                                [&](uint64_t gg) { return F[at((int64_t)gg, cx)]; });
+                    code_cell<T, DECOMP>(F[at(gy[u], cx)], pred, DECOMP ? q[u] : 0,
+                                         rows_code(d.py, G, mode2, gz, (uint64_t)gy[u], (uint64_t)cx / 2), vlane,
+                                         d.py.skip != 0, L, codes, escval, esc, unpred);
+                }
+            }
         };
-        for (int j = threadIdx.x; j < nro * nxe; j += blockDim.x) y_target(oy0 + 2 * (j / nxe), 2 * (j % nxe));
+        y_rows(oy0, nro);
         if (ytail) {
             __syncthreads();
-            for (int j = threadIdx.x; j < nxe; j += blockDim.x) y_target(ny - 1, 2 * j);
+            y_rows(ny - 1, 1);
         }
     }
     __syncthreads();
 
     // 3. x pass over the rows of the tile
+    const bool xskip = d.px.skip != 0;
     {
-        const uint64_t odd = d.px.n / 2;
-        const int nt = (int)odd, nrow = (int)(y1 - y0);
-        for (int j = threadIdx.x; j < nrow * nt; j += blockDim.x) {
-            const int64_t gy = y0 + j / nt;
-            const uint64_t t = (uint64_t)(j % nt);
-            bool vlane;
-            const T pred = lines_pred<T>(d.px, t, vl, vlane, [&](uint64_t e) { return P[at(gy, (int64_t)e)]; });
-            const uint64_t line = gz * d.ny + (uint64_t)gy;
-            code_point<T, DECOMP>(data, pbase + (uint64_t)gy * d.ey + (2 * t + 1) * d.ex, pred,
-                                  d.px.code_base + line * odd + t, vlane, d.px.skip != 0, L, codes, escval, esc, unpred);
+        const int nt = (int)(d.px.n / 2);
+        if (nt > 0 && nrow > 0) {
+            Walk w(nt);
+            while (w.row < nrow) {
+                int q[kBatch];
+                int r[kBatch], t[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) {
+                    r[u] = w.row;
+                    t[u] = w.col;
+                    if (DECOMP && r[u] < nrow) q[u] = codes[d.px.code_base + (gz * d.ny + (uint64_t)(y0 + r[u])) * nt + t[u]];
+                    w.next();
+                }
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) {
+                    if (r[u] >= nrow) continue;
+                    const int64_t gy = y0 + r[u];
+                    bool vlane;
+                    const T pred = lines_pred<T>(d.px, (uint64_t)t[u], vl, vlane, [&](uint64_t e) { return F[at(gy, (int64_t)e)]; });
+                    code_cell<T, DECOMP>(F[at(gy, 2 * t[u] + 1)], pred, DECOMP ? q[u] : 0,
+                                         d.px.code_base + (gz * d.ny + (uint64_t)gy) * nt + t[u], vlane, xskip, L, codes,
+                                         escval, esc, unpred);
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // 4. stores: every row (decompress), or the rows holding reconstructed values (compress: the odd
+    //    rows, and the even rows unless the x pass writes nothing back)
+    if (nrow > 0) {
+        Walk w(nx);
+        for (; w.row < nrow; w.next()) {
+            const int64_t gy = y0 + w.row;
+            if (DECOMP || (gy & 1) || !xskip) data[gpos(gy, w.col)] = F[at(gy, w.col)];
         }
     }
 }
@@ -536,19 +666,22 @@ void interp_pass(int dtype, bool decomp, const PassDesc& p, void* data, int16_t*
 }
 
 // Tiles of a level's y+x launch for elements of es bytes: ty rows (even, at most 32) such that the
-// rows of a tile, its last-block surplus of 2 and the 2 kHalo halo rows fit 64 KiB of LDS; ty = 0
-// when fewer than 4 rows fit (the passes then run separately).
+// tile's rows, its last-block surplus of 2 and the 2 halo rows a side fit 52 KiB of LDS as whole rows
+// (measured on 512^3 float32 against 30, 40 and 64 KiB: with 512 threads a workgroup, three
+// workgroups a CU overlap one tile's loads with another's arithmetic best); ty = 0 when fewer than 4
+// rows fit (the passes then run separately).
 static LevelDesc level_tiles(LevelDesc d, size_t es) {
-    constexpr size_t kBudget = size_t(64) << 10;
+    constexpr size_t kBudget = size_t(52) << 10;  // three workgroups a CU
+    constexpr size_t kExtra = 2 + 2 * kHaloRows + 1;  // surplus rows and LDS rows [y0-2, y1+2]
     d.nxe = (uint32_t)((d.nx + 1) / 2);
-    const size_t row = (size_t)d.nxe * es;
+    const size_t row = (size_t)d.nx * es;
     const size_t fit = row ? kBudget / row : 0;
-    d.ty = fit >= 2 + 2 * kHalo + 4 ? (uint32_t)std::min<size_t>(32, (fit - 2 - 2 * kHalo) & ~size_t(1)) : 0;
+    d.ty = fit >= kExtra + 4 ? (uint32_t)std::min<size_t>(32, (fit - kExtra) & ~size_t(1)) : 0;
     if (!d.ty) return d;
     d.nyb = (uint32_t)((d.ny + d.ty - 1) / d.ty);
     // the linear even-n end row reads row ny-3 of the same pass: keep both in the last tile
     if (d.nyb > 1 && rows_geom(d.py).lin_dep_tail && d.ny - (uint64_t)(d.nyb - 1) * d.ty <= 2) d.nyb--;
-    const uint64_t rows = std::min<uint64_t>(d.ny, (uint64_t)d.ty + 2) + 2 * kHalo;
+    const uint64_t rows = std::min<uint64_t>(d.ny, (uint64_t)d.ty + 2) + 2 * kHaloRows + 1;
     d.lds_bytes = (uint32_t)(rows * row);
     return d;
 }
@@ -561,7 +694,7 @@ static void launch_yx(const LevelDesc& d, T* data, int16_t* codes, T* escval, co
     if (!tiles || !ncodes) return;
     if (tiles > 0x7fffffffu) throw std::length_error("level has more tiles than one launch takes");
     const uint64_t per_target = DECOMP ? sizeof(T) + 2 : 2 * sizeof(T) + 2;
-    FZ_LAUNCH("interp_yx", ncodes * per_target, (k_yx<T, DECOMP>), dim3((unsigned)tiles), dim3(256), d.lds_bytes, st,
+    FZ_LAUNCH("interp_yx", ncodes * per_target, (k_yx<T, DECOMP>), dim3((unsigned)tiles), dim3(kYxThreads), d.lds_bytes, st,
               d, data, codes, escval, lv, avx2, e, unpred);
 }
 

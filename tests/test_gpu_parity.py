@@ -74,6 +74,40 @@ def test_stream_parity(ctx, oracle, case, simd):
     assert np.abs(dec.reshape(shape).astype(np.float64) - data.astype(np.float64)).max() <= bound
 
 
+# Direction-0 3D/2D fields run each level's y and x passes as one launch over tiles of whole rows (ty
+# rows from the row width and element size). Shapes that reach the tiling's edges:
+TILE_CASES = [
+    # shape, dtype, interp_algo
+    ((20, 66, 100), np.float32, 0),   # linear even ny: the last tile takes the end row with row ny-3 (34 rows)
+    ((20, 34, 100), np.float32, 0),   # ... one tile of ty + 2 rows
+    ((18, 70, 100), np.float32, 0),   # linear even ny, last tile of 6 rows
+    ((16, 66, 200), np.float64, 0),   # float64 tiles
+    ((30, 54, 600), np.float64, 0),   # 4-row tiles; the last merged to 6
+    ((66, 1500), np.float32, 0),      # 2D: level-1 rows too wide for LDS (separate passes), coarser levels tiled
+    ((98, 700), np.float32, 1),       # 2D cubic, 12-row tiles
+    ((30, 40, 1), np.float32, 1),     # one column
+    ((9, 1, 50), np.float32, 0),      # one row a plane
+    ((1, 70, 64), np.float32, 1),     # one plane
+]
+
+
+@pytest.mark.parametrize("case", TILE_CASES, ids=[f"{c[0]}-{np.dtype(c[1]).name}-a{c[2]}" for c in TILE_CASES])
+def test_level_tile_edges(ctx, oracle, case):
+    shape, dtype, ialgo = case
+    data = synth.sinusoids(shape, seed=11, dtype=dtype)
+    kw = dict(cmpr_algo=2, eb_mode=0, abs_eb=1e-3, interp_algo=ialgo, interp_direction=0, interp_anchor_stride=-1)
+    c, oc = _cfg_pair(oracle, shape, **kw)
+    o_stream, o_mut, o_pre = oracle.compress(data, oc, want_pre=True)
+    g_data = data.copy()
+    g_stream, _, g_pre = ctx.compress_ex(g_data, c, want_prezstd=True)
+    assert g_pre == o_pre
+    assert bytes(g_stream) == o_stream
+    assert np.array_equal(g_data.view(np.uint8), o_mut.view(np.uint8))
+    dec, _, _ = ctx.decompress_ex(g_stream, dtype)
+    o_dec, _ = oracle.decompress(o_stream, dtype)
+    assert np.array_equal(dec.reshape(shape).view(np.uint8), o_dec.reshape(shape).view(np.uint8))
+
+
 def _window_cases():
     from test_oracle_traversal import WINDOW
     return [(s, d, c) for s, d in WINDOW for c in (0, 1)]
