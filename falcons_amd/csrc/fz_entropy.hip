@@ -2866,6 +2866,14 @@ constexpr uint64_t kMaxRounds = 6;
 constexpr int64_t kWarmSubs = 4;
 // Chains below this many bits a symbol are decoded from their start.
 constexpr double kSeqBitsPerSymbol = 2.5;
+// So are chains of at most this many symbols (fields up to about 128^3, one segment of four chains):
+// the sequential decoder takes about 85 ns a symbol on one CU, while the warm-up, graph rounds and
+// walk cost several milliseconds on many CUs whatever the length. Measured per call on k^-3 fields
+// (REL 1e-2 / 1e-3 / 1e-4): 64^3 15.0 / 7.0 / 10.0 ms against 9.8 / 7.0 / 10.1 decoded from the
+// start, 128^3 84.7 / 45.1 / 51.5 against 65.6 / 47.9 / 51.5, 200^3 243 / 169 / 182 against
+// 235 / 185 / 182; many such fields decoded side by side gain most, the sequential chains
+// occupying four CUs a field.
+constexpr uint64_t kSeqShortSymbols = 1ull << 19;
 
 // Stationary distribution of segment g's context chain (a code's context follows from the previous
 // code; an escape leads to 13 or 14, counted half each), from the normalized counts of its 15
@@ -2995,12 +3003,15 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
     // Chains of fewer than 2.5 bits a symbol go to the sequential decoder as well: there a decoder
     // started from a guessed state seldom falls into step with the true one (few bits are read per
     // symbol), so subsequence start sets do not settle and the walk would decode most subsequences
-    // one after another anyway, more slowly.
+    // one after another anyway, more slowly. So do short chains (kSeqShortSymbols).
     {
+        // FALCONS_O1_PARALLEL_SHORT=1 keeps short chains on the parallel decoder (its tests on small fields)
+        const char* par_env = getenv("FALCONS_O1_PARALLEL_SHORT");
+        const bool short_seq = !(par_env && atoi(par_env) != 0);
         std::vector<O1DecChain> keep;
         for (O1DecChain c : hc) {
             const double bps = (double)c.pos0 / (double)(c.hi - c.lo);
-            if (c.nsub > 2 && bps < kSeqBitsPerSymbol) {
+            if ((c.nsub > 2 && bps < kSeqBitsPerSymbol) || (short_seq && c.hi - c.lo <= kSeqShortSymbols)) {
                 c.nsub = 1;
                 c.sub_bits = 0;
                 hd.push_back(c);

@@ -225,6 +225,24 @@ def test_o1_sequential_decoder(ctx, oracle, case, monkeypatch):
     check(ctx, oracle, data, **kw)
 
 
+PAR_CASES = SEQ_CASES[:5] + [
+    ("rel-1e-2", lambda: synth.power_law((100, 100, 100), -3.0, seed=5), dict(eb_mode=1, rel_eb=1e-2)),
+    ("rel-1e-3", lambda: synth.power_law((128, 128, 128), -3.0, seed=5), dict(eb_mode=1, rel_eb=1e-3)),
+    ("2d", lambda: synth.power_law((1024, 1024), -3.0, seed=6), dict(eb_mode=1, rel_eb=1e-3)),
+]
+
+
+@pytest.mark.parametrize("case", PAR_CASES, ids=[c[0] for c in PAR_CASES])
+def test_o1_parallel_decoder_short_chains(ctx, oracle, case, monkeypatch):
+    """Chains of at most 2^19 symbols go to the sequential decoder; FALCONS_O1_PARALLEL_SHORT=1 keeps
+    them on the parallel one (warm-up, graphs, walk), which then decodes these small fields: same
+    codes as the oracle."""
+    _, make, kw = case
+    data = make()
+    monkeypatch.setenv("FALCONS_O1_PARALLEL_SHORT", "1")
+    check(ctx, oracle, data, **kw)
+
+
 def nan_placed(dtype, where):
     """A (41, 50, 61) field (tails of 2 after the last full AVX2 vector) with NaNs placed where the
     AVX2 and scalar data_range differ (Statistic.hpp:38-157)."""
