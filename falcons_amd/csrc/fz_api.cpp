@@ -735,54 +735,91 @@ PreLayout parse_pre(const fz_config& c, int dtype, const uint8_t* h_pre, size_t 
     return L;
 }
 
-void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h_pre, const uint8_t* d_pre,
-                    size_t pre_len, void** d_out_io, fz_opts* o) {
-    hipStream_t st = ctx->stream;
+// A decompress call in two parts around the entropy decoder's sequential chains (entropy_decode_begin):
+// decompress_begin parses the buffer and queues everything up to those chains on st;
+// decompress_finish queues the rest (entropy_decode_end, escape positions, interpolation; the caller
+// waits for it). Between them the state lives here, its device memory in the call's arena.
+struct DecompressCall {
+    PreLayout PL;
+    int dtype = 0;
+    uint64_t n = 0;
+    const uint8_t* h_pre = nullptr;
+    void* d_out = nullptr;
+    void* d_unpred = nullptr;
+    double* eb_dev = nullptr;
+    double* ratio_dev = nullptr;
+    int rmode = 0;
+    LevelEb* lv = nullptr;
+    int16_t* codes = nullptr;
+    uint64_t* esc_idx = nullptr;
+    uint64_t* nesc_dev = nullptr;
+    void* sel_tmp = nullptr;
+    size_t sel_bytes = 0;
+    EntropyTail tail{};
+};
+
+// ws: the call's arena (reset here); defer: null to decode the sequential chains in the call.
+void decompress_begin(DecompressCall& D, DevArena& ws, hipStream_t st, const AuxStream& aux, const fz_config& c,
+                      int dtype, const uint8_t* h_pre, const uint8_t* d_pre, size_t pre_len, void* d_out,
+                      DenseBatch* defer, Timer* tm) {
     const size_t es = elem_size(dtype);
     const uint64_t n = c.num;
-    const PreLayout PL = parse_pre(c, dtype, h_pre, pre_len);
+    D.PL = parse_pre(c, dtype, h_pre, pre_len);
+    D.dtype = dtype;
+    D.n = n;
+    D.h_pre = h_pre;
+    const PreLayout& PL = D.PL;
     const int32_t nun = PL.nun;
     const size_t un_off = PL.un_off, enc_off = PL.enc_off, enc_len = PL.enc_len;
     const uint64_t ncodes = PL.ncodes;
-    const double eb = PL.eb, alpha = PL.alpha, beta = PL.beta;
     const Plan& P = PL.P;
-    DevArena& ws = ctx->arena;
     ws.reset();
     ws.reserve(decompress_workspace_bytes(n, dtype, (uint64_t)nun, pre_len));
     ws.clear_peak();
-    ArenaReport arena_report{ws, "decompress", n};
-    void* d_out = *d_out_io ? *d_out_io : ws.take<uint8_t>(n * es);
-    *d_out_io = d_out;
+    D.d_out = d_out ? d_out : ws.take<uint8_t>(n * es);
     const uint8_t* d_enc;
-    void* d_unpred = ws.take<uint8_t>((size_t)nun * es + 16);
+    D.d_unpred = ws.take<uint8_t>((size_t)nun * es + 16);
     if (d_pre) {
         d_enc = d_pre + enc_off;
     } else {
         uint8_t* e = ws.take<uint8_t>(enc_len + 16);
         FZ_HIP(hipMemcpyAsync(e, h_pre + enc_off, enc_len, hipMemcpyHostToDevice, st));
-        if (nun) FZ_HIP(hipMemcpyAsync(d_unpred, h_pre + un_off, (size_t)nun * es, hipMemcpyHostToDevice, st));
+        if (nun) FZ_HIP(hipMemcpyAsync(D.d_unpred, h_pre + un_off, (size_t)nun * es, hipMemcpyHostToDevice, st));
         d_enc = e;
     }
-    double* eb_dev = ws.take<double>(1);
-    FZ_HIP(hipMemcpyAsync(eb_dev, &eb, 8, hipMemcpyHostToDevice, st));
+    D.eb_dev = ws.take<double>(1);
+    FZ_HIP(hipMemcpyAsync(D.eb_dev, &PL.eb, 8, hipMemcpyHostToDevice, st));
     std::vector<double> ratio;
-    const int rmode = ratio_table(alpha, beta, P.levels, ratio);
-    double* ratio_dev = ws.take<double>(ratio.size());
-    FZ_HIP(hipMemcpyAsync(ratio_dev, ratio.data(), ratio.size() * 8, hipMemcpyHostToDevice, st));
-    LevelEb* lv = ws.take<LevelEb>(std::max(P.levels, 1));
-    int16_t* codes = ws.take<int16_t>(ncodes + 16);
-    uint64_t* esc_idx = ws.take<uint64_t>(ncodes + 1);
-    uint64_t* nesc_dev = ws.take<uint64_t>(1);
-    const size_t sel_bytes = select_escapes_scratch(ncodes);
-    void* sel_tmp = ws.take<uint8_t>(sel_bytes + 1);
-    FZ_HIP(hipStreamSynchronize(st));
-    Timer tm{st, ctx->ev0, ctx->ev1};
-    tm.start();
-    if (d_pre && nun) FZ_HIP(hipMemcpyAsync(d_unpred, d_pre + un_off, (size_t)nun * es, hipMemcpyDeviceToDevice, st));
-    entropy_decode(h_pre + enc_off, d_enc, enc_len, ncodes, codes, ws, st, ctx->aux);
-    select_escapes(codes, ncodes, esc_idx, nesc_dev, sel_tmp, sel_bytes, st);
+    D.rmode = ratio_table(PL.alpha, PL.beta, P.levels, ratio);
+    D.ratio_dev = ws.take<double>(ratio.size());
+    FZ_HIP(hipMemcpyAsync(D.ratio_dev, ratio.data(), ratio.size() * 8, hipMemcpyHostToDevice, st));
+    D.lv = ws.take<LevelEb>(std::max(P.levels, 1));
+    D.codes = ws.take<int16_t>(ncodes + 16);
+    D.esc_idx = ws.take<uint64_t>(ncodes + 1);
+    D.nesc_dev = ws.take<uint64_t>(1);
+    D.sel_bytes = select_escapes_scratch(ncodes);
+    D.sel_tmp = ws.take<uint8_t>(D.sel_bytes + 1);
+    FZ_HIP(hipStreamSynchronize(st));  // the host sources of the copies above (ratio) leave scope
+    if (tm) tm->start();
+    if (d_pre && nun) FZ_HIP(hipMemcpyAsync(D.d_unpred, d_pre + un_off, (size_t)nun * es, hipMemcpyDeviceToDevice, st));
+    D.tail = entropy_decode_begin(h_pre + enc_off, d_enc, enc_len, ncodes, D.codes, ws, st, aux, defer);
+}
+
+void decompress_finish(DecompressCall& D, DevArena& ws, hipStream_t st) {
+    const PreLayout& PL = D.PL;
+    const int dtype = D.dtype;
+    const size_t es = elem_size(dtype);
+    const uint64_t n = D.n, ncodes = PL.ncodes;
+    const int32_t nun = PL.nun;
+    const Plan& P = PL.P;
+    void* d_out = D.d_out;
+    void* d_unpred = D.d_unpred;
+    int16_t* codes = D.codes;
+    uint64_t* esc_idx = D.esc_idx;
+    entropy_decode_end(D.tail, ws, st);
+    select_escapes(codes, ncodes, esc_idx, D.nesc_dev, D.sel_tmp, D.sel_bytes, st);
     uint64_t nesc = 0;
-    FZ_HIP(hipMemcpyAsync(&nesc, nesc_dev, 8, hipMemcpyDeviceToHost, st));
+    FZ_HIP(hipMemcpyAsync(&nesc, D.nesc_dev, 8, hipMemcpyDeviceToHost, st));
     FZ_HIP(hipStreamSynchronize(st));
     if (PL.generic) {
         const GenPlan& G = PL.G;
@@ -804,10 +841,11 @@ void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h
         if (err) throw std::runtime_error("LinearQuantizer integral reconstruction overflow");
     } else {
         if (P.nanchors + nesc != (uint64_t)nun) throw ArgError("unpredictable count does not match the code stream");
-        if (P.levels > 0) level_constants(eb_dev, ratio_dev, rmode, P.levels, lv, st);
+        LevelEb* lv = D.lv;
+        if (P.levels > 0) level_constants(D.eb_dev, D.ratio_dev, D.rmode, P.levels, lv, st);
         const void* esc_vals = static_cast<const uint8_t*>(d_unpred) + P.nanchors * es;
         if (P.anchor) interp_anchors(dtype, true, d_out, d_unpred, P.acnt, P.astr, st);
-        else interp_first(dtype, true, d_out, codes, nullptr, eb_dev, esc_idx, nesc, esc_vals, st);
+        else interp_first(dtype, true, d_out, codes, nullptr, D.eb_dev, esc_idx, nesc, esc_vals, st);
         if (!P.levels3.empty()) {
             for (const auto& d : P.levels3)
                 interp_level(dtype, true, d, d_out, codes, nullptr, lv, 1, esc_idx, nesc, esc_vals, st);
@@ -816,15 +854,45 @@ void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h
                 interp_pass(dtype, true, ps, d_out, codes, nullptr, lv, 1, esc_idx, nesc, esc_vals, st);
         }
     }
+}
+
+void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h_pre, const uint8_t* d_pre,
+                    size_t pre_len, void** d_out_io, fz_opts* o) {
+    hipStream_t st = ctx->stream;
+    DevArena& ws = ctx->arena;
+    ArenaReport arena_report{ws, "decompress", c.num};
+    Timer tm{st, ctx->ev0, ctx->ev1};
+    DecompressCall D;
+    decompress_begin(D, ws, st, ctx->aux, c, dtype, h_pre, d_pre, pre_len, *d_out_io, nullptr, &tm);
+    *d_out_io = D.d_out;
+    decompress_finish(D, ws, st);
     const double tdev = tm.stop();
     if (o) {
         o->t_device_ms = tdev;
-        o->ncodes = ncodes;
-        o->nunpred = (uint64_t)nun;
+        o->ncodes = D.PL.ncodes;
+        o->nunpred = (uint64_t)D.PL.nun;
     }
 }
 
 }  // namespace
+
+struct fz::DecCall {
+    DecompressCall D;
+};
+fz::DecCall* fz::dec_call_new() { return new DecCall; }
+void fz::dec_call_free(DecCall* d) { delete d; }
+void fz::dec_call_begin(DecCall* d, DevArena& ws, hipStream_t st, const fz_config& c, int dtype, const uint8_t* h_pre,
+                        const uint8_t* d_pre, size_t pre_len, void* d_out, DenseBatch* defer) {
+    decompress_begin(d->D, ws, st, AuxStream{}, c, dtype, h_pre, d_pre, pre_len, d_out, defer, nullptr);
+}
+void fz::dec_call_finish(DecCall* d, DevArena& ws, hipStream_t st) {
+    decompress_finish(d->D, ws, st);
+    FZ_HIP(hipStreamSynchronize(st));
+}
+size_t fz::dec_call_workspace_bytes(uint64_t n, int dtype, uint64_t nunpred, size_t pre_len) {
+    const size_t es = elem_size(dtype);
+    return n * (es + 14) + nunpred * es + 2 * pre_len + (size_t(8) << 20);
+}
 
 extern "C" int fz_check_prezstd(const fz_config* conf, int dtype, const uint8_t* pre, size_t len, char* err,
                                 size_t errlen) {

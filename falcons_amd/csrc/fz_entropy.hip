@@ -1869,14 +1869,36 @@ __device__ __forceinline__ uint32_t dense_entry(uint32_t raw, const DenseMap& dm
 
 // Dense-format copies of the tables of renumbered contexts 4..14 for each dense chain: grid
 // (chains, 11).
+// Cold table blockIdx.y of one chain: the entries of renumbered context kWalkCtxN + y, re-encoded.
+__device__ __forceinline__ void dense_cold_table(const DEntry* dt_seg, const DenseMap& dm, uint32_t* dtd_chain) {
+    const uint32_t* dt = reinterpret_cast<const uint32_t*>(dt_seg) + ((size_t)dm.perm[kWalkCtxN + blockIdx.y] << kO1Log);
+    uint32_t* o = dtd_chain + (size_t)blockIdx.y * (1u << kO1Log);
+    for (uint32_t i = threadIdx.x; i < (1u << kO1Log); i += blockDim.x) o[i] = dense_entry(dt[i], dm);
+}
 __global__ void __launch_bounds__(256) k_o1d_dense_tables(const O1DecChain* ch, const DEntry* dt_all,
                                                           const DenseMap* maps, uint32_t* dtd) {
     const O1DecChain c = ch[blockIdx.x];
     const DenseMap dm = maps[blockIdx.x];
-    const uint32_t* dt = reinterpret_cast<const uint32_t*>(dt_all + c.dset * kO1Ctx * (1u << kO1Log)) +
-                         ((size_t)dm.perm[kWalkCtxN + blockIdx.y] << kO1Log);
-    uint32_t* o = dtd + ((size_t)blockIdx.x * kWalkCtx0 + blockIdx.y) * (1u << kO1Log);
-    for (uint32_t i = threadIdx.x; i < (1u << kO1Log); i += blockDim.x) o[i] = dense_entry(dt[i], dm);
+    dense_cold_table(dt_all + c.dset * kO1Ctx * (1u << kO1Log), dm,
+                     dtd + (size_t)blockIdx.x * kWalkCtx0 * (1u << kO1Log));
+}
+
+// A sequential chain of any call, with the call's buffers: the batched decompress collects the
+// chains of many fields and decodes them in one launch (k_o1d_dense_jobs).
+struct DenseJob {
+    const uint8_t* in;   // the call's entropy stream (chain offsets count from here)
+    const DEntry* dt;    // the call's decoding tables, every segment
+    const int16_t* ovf;  // the call's escaped codes
+    int16_t* out;        // the call's codes
+    int* err;
+    uint32_t* dtd;       // this chain's cold tables (kWalkCtx0 tables)
+    O1DecChain c;
+    DenseMap dm;
+};
+__global__ void __launch_bounds__(256) k_o1d_dense_tables_jobs(const DenseJob* jobs) {
+    const DenseJob& j = jobs[blockIdx.x];
+    const DenseMap dm = j.dm;
+    dense_cold_table(j.dt + j.c.dset * kO1Ctx * (1u << kO1Log), dm, j.dtd);
 }
 
 __device__ __forceinline__ uint32_t lds_u32(const FZ_LDS uint8_t* base, uint32_t off) {
@@ -2050,6 +2072,94 @@ __device__ __forceinline__ uint64_t dense_decode(SeqState& S, uint32_t& E, const
 // buffer of decoded codes sit in LDS; the workgroup refills the windows and writes the codes out
 // between rounds.
 
+// Where the workgroup of a sequential chain of the batched decompress finds its chain, its segment's
+// tables, its renumbering, its cold tables and its call's buffers.
+struct DenseJobSrc {
+    const DenseJob* j;
+    __device__ O1DecChain chain() const { return j->c; }
+    __device__ const uint32_t* tables(const O1DecChain& c) const {
+        return reinterpret_cast<const uint32_t*>(j->dt + c.dset * kO1Ctx * (1u << kO1Log));
+    }
+    __device__ const uint32_t* cold() const { return j->dtd; }
+    __device__ DenseMap map() const { return j->dm; }
+    __device__ const uint8_t* in() const { return j->in; }
+    __device__ const int16_t* ovf() const { return j->ovf; }
+    __device__ int16_t* out() const { return j->out; }
+    __device__ int* err() const { return j->err; }
+};
+
+// One sequential chain, by the workgroup (the body of k_o1d_dense, its values from src).
+template <class Src>
+__device__ __forceinline__ void dense_chain(const Src& src) {
+    __shared__ uint32_t hot[kWalkCtxN << kO1Log];
+    __shared__ __align__(16) uint32_t stage32[(kDenseStage + kWinPad + 32) / 4 + 4];
+    uint8_t* stage = reinterpret_cast<uint8_t*>(stage32);
+    __shared__ __align__(16) int16_t obuf[kDenseOut];
+    __shared__ int16_t vbuf[kDenseOvf];
+    const O1DecChain c = src.chain();
+    const uint32_t* dt = src.tables(c);
+    const uint32_t* dtd = src.cold();
+    const DenseMap dm = src.map();
+    const uint8_t* in = src.in();
+    const int16_t* ovf = src.ovf();
+    int16_t* out = src.out();
+    int* err = src.err();
+    // interleaved by state: hot[X * 4 + r] holds (state X, renumbered context r)
+    for (uint32_t i = threadIdx.x; i < (kWalkCtxN << kO1Log); i += blockDim.x)
+        hot[i] = dense_entry(dt[((size_t)dm.perm[i & 3u] << kO1Log) + (i >> 2)], dm);
+    __syncthreads();
+    const uint8_t* cb = in + c.off;
+    const int16_t* ov = ovf + c.ovf_base;
+    const uint64_t m = c.hi - c.lo;
+    SeqState z{c.pos0, c.x0, 0, 0, false};
+    const bool esc_dense = dm.esc != 0;
+    const uint32_t r13 = dm.inv[13], r14 = dm.inv[14];
+    // the chain starts in context 0
+    const uint32_t r0 = dm.inv[0], x0 = c.x0 & ((1u << kO1Log) - 1);
+    uint32_t E = r0 < (uint32_t)kWalkCtxN ? hot[(x0 << 2) | r0] : dtd[((size_t)(r0 - kWalkCtxN) << kO1Log) + x0];
+    uint64_t i = 0;
+    while (i < m && !z.bad) {
+        const int64_t hi = min<int64_t>((int64_t)c.len, (z.B + 7) / 8 + 1);
+        const int64_t lo = max<int64_t>(0, hi - (int64_t)kDenseStage);
+        const uint64_t e0 = z.e, ve = min<uint64_t>(c.novf, e0 + kDenseOvf);
+        __syncthreads();
+        const int64_t w0 = stage_window(lds(stage), cb, (int64_t)c.len, lo, hi);
+        {
+            // the round's escaped values, eight loads per lane in flight at a time
+            const uint32_t nv = (uint32_t)(ve - e0);  // at most kDenseOvf
+            for (uint32_t k0 = 0; k0 < nv; k0 += blockDim.x * 8u) {
+                int16_t t[8];
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    const uint32_t k = k0 + (uint32_t)b * blockDim.x + threadIdx.x;
+                    t[b] = k < nv ? ov[e0 + k] : (int16_t)0;
+                }
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    const uint32_t k = k0 + (uint32_t)b * blockDim.x + threadIdx.x;
+                    if (k < nv) vbuf[k] = t[b];
+                }
+            }
+        }
+        __syncthreads();
+        const int64_t floor_bits = lo == 0 ? INT64_MIN : lo * 8 + 64;
+        const uint64_t nk =
+            esc_dense ? dense_decode<true>(z, E, lds(stage), w0 + kWinPad, floor_bits, min<uint64_t>(m - i, kDenseOut),
+                                           lds(reinterpret_cast<const uint8_t*>(hot)), dtd, lds(vbuf), e0, ve, c.novf,
+                                           lds(obuf), r13, r14)
+                      : dense_decode<false>(z, E, lds(stage), w0 + kWinPad, floor_bits, min<uint64_t>(m - i, kDenseOut),
+                                            lds(reinterpret_cast<const uint8_t*>(hot)), dtd, lds(vbuf), e0, ve, c.novf,
+                                            lds(obuf), r13, r14);
+        __syncthreads();
+        for (uint64_t k = threadIdx.x; k < nk; k += blockDim.x) out[c.lo + i + k] = obuf[k];
+        i += nk;
+    }
+    if (threadIdx.x == 0 && (z.bad || i != m || z.e != c.novf)) *err = 1;
+}
+
+// The chains of one call. Its body is dense_chain's, written out: as a call of the template the
+// compiler allocates this kernel's registers differently, and it measured 1% slower (192.3-192.6 ms
+// against 193.8-194.8 ms a launch on the 512^3 field, the same box).
 __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, const O1DecChain* ch, const DEntry* dt_all,
                                                          const DenseMap* maps, const uint32_t* dtd_all,
                                                          const int16_t* ovf, int16_t* out, int* err,
@@ -2118,6 +2228,12 @@ __global__ void __launch_bounds__(kSeqLanes) k_o1d_dense(const uint8_t* in, cons
     if (threadIdx.x == 0 && (z.bad || i != m || z.e != c.novf)) *err = 1;
     if (times && threadIdx.x == 0) times[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
 }
+
+// The chains of many calls (the batched decompress), one workgroup a chain.
+__global__ void __launch_bounds__(kSeqLanes) k_o1d_dense_jobs(const DenseJob* jobs) {
+    dense_chain(DenseJobSrc{jobs + blockIdx.x});
+}
+
 
 // Final pass: every subsequence decodes from its settled start into its place in the code array.
 // Its code count is the distance to the next subsequence's output offset, so codes leave in
@@ -2954,9 +3070,46 @@ static DenseMap dense_map(const uint8_t* host, uint64_t nf_off, bool esc) {
     return dm;
 }
 
+// Sequential chains of several calls, decoded together (dense_batch_run).
+struct DenseBatch {
+    std::mutex mu;
+    std::vector<DenseJob> jobs;
+    uint64_t bytes = 0, codes = 0;
+};
+DenseBatch* dense_batch_create() { return new DenseBatch; }
+void dense_batch_destroy(DenseBatch* b) { delete b; }
+size_t dense_batch_size(DenseBatch* b) {
+    std::lock_guard<std::mutex> g(b->mu);
+    return b->jobs.size();
+}
+void dense_batch_clear(DenseBatch* b) {
+    std::lock_guard<std::mutex> g(b->mu);
+    b->jobs.clear();
+    b->bytes = b->codes = 0;
+}
+void dense_batch_run(DenseBatch* b, DevArena& ws, hipStream_t st) {
+    std::lock_guard<std::mutex> g(b->mu);
+    const size_t n = b->jobs.size();
+    if (!n) return;
+    if (n > 0x7fffffffu) throw std::length_error("too many sequential chains for one launch");
+    // longest chains first, so that the last workgroups to start are short ones
+    std::stable_sort(b->jobs.begin(), b->jobs.end(),
+                     [](const DenseJob& x, const DenseJob& y) { return x.c.hi - x.c.lo > y.c.hi - y.c.lo; });
+    DenseJob* dj = ws.take<DenseJob>(n);
+    // the job list stays in b until dense_batch_clear, after the launches have finished
+    FZ_HIP(hipMemcpyAsync(dj, b->jobs.data(), n * sizeof(DenseJob), hipMemcpyHostToDevice, st));
+    FZ_LAUNCH("o1_decode_dense_tables", 0, k_o1d_dense_tables_jobs, dim3((unsigned)n, (unsigned)kWalkCtx0), dim3(256), 0,
+              st, dj);
+    FZ_LAUNCH("o1_decode_dense", b->bytes + b->codes * 2, k_o1d_dense_jobs, dim3((unsigned)n), dim3(kSeqLanes), 0, st,
+              dj);
+    FZ_HIP(hipGetLastError());
+}
+
+// defer: the sequential chains are handed to it instead of launched (their decode then runs in
+// dense_batch_run, after the work this call has queued on st).
 static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const std::vector<uint64_t>& chains,
                                const std::vector<uint64_t>& nf_off, const DEntry* dt, const int16_t* ovf, int16_t* out,
-                               int* err, DevArena& ws, hipStream_t st, const AuxStream& aux) {
+                               int* err, DevArena& ws, hipStream_t st, const AuxStream& aux, DenseBatch* defer) {
     const int nch = (int)(chains.size() / 7);
     std::vector<O1DecChain> hc, hd;
     uint64_t nsub = 0, stream_bytes = 0, ncodes = 0, dense_bytes = 0, dense_codes = 0;
@@ -3032,7 +3185,17 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
     const bool stats_on = getenv("FALCONS_DECODE_STATS") != nullptr;
     uint64_t* dense_times = nullptr;  // per sequential chain: clock at start and end (statistics only)
     std::vector<DenseMap> maps;  // read by a copy in flight until the function returns
-    if (!hd.empty()) {
+    if (!hd.empty() && defer) {
+        uint32_t* dtd = ws.take<uint32_t>(hd.size() * kWalkCtx0 * (1u << kO1Log));
+        std::vector<DenseJob> jobs(hd.size());
+        for (size_t q = 0; q < hd.size(); q++)
+            jobs[q] = DenseJob{dev, dt, ovf, out, err, dtd + q * kWalkCtx0 * (1u << kO1Log), hd[q],
+                               dense_map(host, nf_off[hd[q].dset], hd[q].novf * 16 > hd[q].hi - hd[q].lo)};
+        std::lock_guard<std::mutex> g(defer->mu);
+        defer->jobs.insert(defer->jobs.end(), jobs.begin(), jobs.end());
+        defer->bytes += dense_bytes;
+        defer->codes += dense_codes;
+    } else if (!hd.empty()) {
         O1DecChain* ddc = ws.take<O1DecChain>(hd.size());
         h2d(ddc, hd.data(), hd.size(), st);
         // more than one escape in 16 symbols: the escape-dense step
@@ -3064,7 +3227,7 @@ static void o1_decode_parallel(const uint8_t* host, const uint8_t* dev, const st
                 (void)hipStreamWaitEvent(st, a.join, 0);
             }
         }
-    } join{aux, st, !hd.empty() && aux.stream != nullptr};
+    } join{aux, st, !hd.empty() && aux.stream != nullptr && !defer};
     if (hc.empty()) return;
     const int nc = (int)hc.size();
     O1DecChain* dch = ws.take<O1DecChain>(nc);
@@ -3264,6 +3427,11 @@ void entropy_check(const uint8_t* host, size_t len, uint64_t n) {
 
 void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
                     hipStream_t st, const AuxStream& aux) {
+    entropy_decode_end(entropy_decode_begin(host, dev, len, n, codes, ws, st, aux, nullptr), ws, st);
+}
+
+EntropyTail entropy_decode_begin(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes,
+                                 DevArena& ws, hipStream_t st, const AuxStream& aux, DenseBatch* defer) {
     EntropyLayout Lay;
     parse_entropy(host, len, n, Lay);
     const int method = Lay.method, use_delta = Lay.use_delta;
@@ -3317,7 +3485,7 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
             d2h(&herr, err, 1, st);
             if (herr) throw std::invalid_argument("corrupt o1 tANS tables");
         }
-        o1_decode_parallel(host, dev, chains, nf_off, dt, ovf, out, err, ws, st, aux);
+        o1_decode_parallel(host, dev, chains, nf_off, dt, ovf, out, err, ws, st, aux, defer);
     } else {
         const uint64_t nr = Lay.nr, nv = Lay.nv, s_rcl = Lay.s_rcl, s_vcl = Lay.s_vcl, rbp = Lay.rbp, vbp = Lay.vbp;
         const uint64_t o_rcl = Lay.o_rcl, o_vcl = Lay.o_vcl, o_rman = Lay.o_rman, o_vman = Lay.o_vman;
@@ -3360,14 +3528,18 @@ void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_
                           vbp * 8, nv, runsum, n, out, err);
         FZ_HIP(hipGetLastError());
     }
-    if (use_delta && n > 0) {
+    return EntropyTail{use_delta, out, codes, err, n};
+}
+
+void entropy_decode_end(const EntropyTail& t, DevArena& ws, hipStream_t st) {
+    if (t.use_delta && t.n > 0) {
         size_t b = 0;
-        FZ_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, b, out, codes, Add16{}, (int64_t)n, st));
-        void* t = cub_scratch(ws, b);
-        FZ_HIP(hipcub::DeviceScan::InclusiveScan(t, b, out, codes, Add16{}, (int64_t)n, st));
+        FZ_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, b, t.out, t.codes, Add16{}, (int64_t)t.n, st));
+        void* tmp = cub_scratch(ws, b);
+        FZ_HIP(hipcub::DeviceScan::InclusiveScan(tmp, b, t.out, t.codes, Add16{}, (int64_t)t.n, st));
     }
     int herr = 0;
-    d2h(&herr, err, 1, st);
+    d2h(&herr, t.err, 1, st);
     if (herr) throw std::runtime_error("corrupt entropy stream");
 }
 

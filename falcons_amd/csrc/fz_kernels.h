@@ -31,7 +31,7 @@ class DevArena {
     }
     void reserve(size_t bytes) {
         bytes = (bytes + 255) & ~size_t(255);
-        if (bytes <= cap_) return;
+        if (bytes <= cap_ || bound_) return;  // a bound arena cannot grow: its takes throw instead
         if (used_ == 0 && !blocks_.empty() && blocks_[0].cap < bytes) release();  // regrow as one block
         if (blocks_.empty()) add_block(bytes);
         else add_block(bytes - cap_);
@@ -154,5 +154,27 @@ void entropy_check(const uint8_t* host, size_t len, uint64_t n);
 // Decodes the encoder bytes (host copy `host`, device copy `dev`, length len) into n codes (device).
 void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
                     hipStream_t st, const AuxStream& aux);
+
+// The same in two parts around the order-1 decoder's sequential chains. With a DenseBatch, begin
+// hands those chains to it instead of decoding them; dense_batch_run then decodes the chains of
+// every call in the batch in one launch, after each call's begin work (the caller orders the
+// streams), and end runs once that launch is done. Without one, begin decodes them itself.
+struct DenseBatch;
+DenseBatch* dense_batch_create();
+void dense_batch_destroy(DenseBatch* b);
+size_t dense_batch_size(DenseBatch* b);
+void dense_batch_run(DenseBatch* b, DevArena& ws, hipStream_t st);
+// forgets the collected chains (after dense_batch_run's launches have finished)
+void dense_batch_clear(DenseBatch* b);
+struct EntropyTail {
+    int use_delta;
+    int16_t* out;
+    int16_t* codes;
+    int* err;
+    uint64_t n;
+};
+EntropyTail entropy_decode_begin(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes,
+                                 DevArena& ws, hipStream_t st, const AuxStream& aux, DenseBatch* defer);
+void entropy_decode_end(const EntropyTail& t, DevArena& ws, hipStream_t st);
 
 }  // namespace fz

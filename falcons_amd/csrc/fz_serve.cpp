@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <thread>
@@ -179,6 +180,116 @@ extern "C" int fz_compress_batch(fz_ctx* ctx, fz_batch_item* items, int n, fz_op
     })
 }
 
+namespace fz {
+namespace {
+
+// Most items a group holds, and most device memory its calls' workspaces take together.
+constexpr size_t kGroupItems = 1024;
+constexpr size_t kGroupBytes = size_t(16) << 30;
+
+// fz_decompress_batch in groups of items. Each group runs in three steps: the workers run every
+// item's call up to the entropy decoder's sequential chains (dec_call_begin, the call's workspace
+// a slice of one group allocation), handing those chains to one DenseBatch; the chains of the
+// whole group then decode in one launch on the batch's own stream, after every item's first part;
+// and the workers finish every item after that launch. A thread pool of per-item calls keeps at
+// most a few chains per worker on the GPU at a time; one launch holds all of them. An item whose
+// first part does not fit its slice or fails is decompressed by a plain call afterwards.
+void decompress_grouped(fz_ctx* ctx, WorkerSet& wset, fz_batch_dec_item* items, const std::vector<int>& order,
+                        FirstError& fe) {
+    const size_t n = order.size();
+    std::vector<size_t> need(n);
+    for (size_t k = 0; k < n; k++) {
+        const fz_batch_dec_item& it = items[order[k]];
+        need[k] = (dec_call_workspace_bytes(it.conf.num, it.dtype, it.conf.num, it.pre_len) + 255) & ~size_t(255);
+    }
+    size_t free_b = 0, total_b = 0;
+    FZ_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t budget = std::max<size_t>(size_t(256) << 20, std::min(kGroupBytes, free_b / 10 * 4));
+    struct Res {  // released however the batch ends
+        DenseBatch* batch = dense_batch_create();
+        void* pool = nullptr;
+        std::vector<hipEvent_t> ev;
+        hipEvent_t done = nullptr;
+        ~Res() {
+            if (pool) (void)hipFree(pool);
+            for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+            if (done) (void)hipEventDestroy(done);
+            dense_batch_destroy(batch);
+        }
+    } R;
+    FZ_HIP(hipEventCreateWithFlags(&R.done, hipEventDisableTiming));
+    size_t pool_bytes = 0;
+    hipStream_t cst = ctx->stream;
+    for (size_t k0 = 0; k0 < n;) {
+        size_t k1 = k0, sum = 0;
+        while (k1 < n && k1 - k0 < kGroupItems && (k1 == k0 || sum + need[k1] <= budget)) sum += need[k1++];
+        const size_t g = k1 - k0;
+        if (sum > pool_bytes) {
+            if (R.pool) FZ_HIP(hipFree(R.pool));
+            R.pool = nullptr;
+            FZ_HIP(hipMalloc(&R.pool, sum));
+            pool_bytes = sum;
+        }
+        while (R.ev.size() < g) {
+            hipEvent_t e;
+            FZ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            R.ev.push_back(e);
+        }
+        std::vector<std::unique_ptr<DevArena>> ar(g);
+        std::vector<std::unique_ptr<DecCall, void (*)(DecCall*)>> calls;
+        std::vector<char> ok(g, 0), rec(g, 0);
+        size_t off = 0;
+        for (size_t j = 0; j < g; j++) {
+            ar[j].reset(new DevArena);
+            ar[j]->bind(static_cast<char*>(R.pool) + off, need[k0 + j]);
+            off += need[k0 + j];
+            calls.emplace_back(dec_call_new(), dec_call_free);
+        }
+        std::vector<int> idx(g);
+        std::iota(idx.begin(), idx.end(), 0);
+        wset.run(idx, [&](fz_ctx* w, int j) {
+            fz_batch_dec_item& it = items[order[k0 + j]];
+            if (!it.d_out || it.conf.num > it.out_cap_elems) return;  // the plain call reports it
+            try {
+                dec_call_begin(calls[j].get(), *ar[j], w->stream, it.conf, it.dtype, it.h_pre, it.d_pre, it.pre_len,
+                               it.d_out, R.batch);
+                ok[j] = 1;
+            } catch (...) {
+            }
+            // also after a failure: chains the item handed over read what its stream has queued
+            rec[j] = hipEventRecord(R.ev[j], w->stream) == hipSuccess;
+        }, wset.size());
+        for (size_t j = 0; j < g; j++)
+            if (rec[j]) FZ_HIP(hipStreamWaitEvent(cst, R.ev[j], 0));
+        ctx->arena.reset();
+        dense_batch_run(R.batch, ctx->arena, cst);
+        FZ_HIP(hipEventRecord(R.done, cst));
+        wset.run(idx, [&](fz_ctx* w, int j) {
+            fz_batch_dec_item& it = items[order[k0 + j]];
+            if (ok[j]) {
+                try {
+                    FZ_HIP(hipStreamWaitEvent(w->stream, R.done, 0));
+                    dec_call_finish(calls[j].get(), *ar[j], w->stream);
+                    it.status = FZ_OK;
+                    return;
+                } catch (...) {
+                }
+            }
+            fz_opts o;
+            fz_opts_init(&o);
+            it.status = fz_decompress_device(w, &it.conf, it.dtype, it.h_pre, it.d_pre, it.pre_len, it.d_out,
+                                             it.out_cap_elems, &o);
+            fe.note(it.status, fz_last_error(w));
+        }, wset.size());
+        FZ_HIP(hipStreamSynchronize(cst));
+        dense_batch_clear(R.batch);
+        k0 = k1;
+    }
+}
+
+}  // namespace
+}  // namespace fz
+
 extern "C" int fz_decompress_batch(fz_ctx* ctx, fz_batch_dec_item* items, int n, fz_opts* opts) {
     FZ_GUARD({
         if (n < 0 || (n > 0 && !items)) throw ArgError("invalid batch");
@@ -186,21 +297,27 @@ extern "C" int fz_decompress_batch(fz_ctx* ctx, fz_batch_dec_item* items, int n,
         WorkerSet& ws = workers_of(ctx);
         const std::vector<int> order =
             largest_first(n, [&](int i) { return items[i].conf.num * es_of(items[i].dtype); });
-        size_t need = 0;
-        for (int i = 0; i < n; i++)
-            need = std::max(need, decompress_workspace_bytes(items[i].conf.num, items[i].dtype, items[i].conf.num,
-                                                             items[i].pre_len));
-        const int active = active_workers(ws, need, ws.held());
         FirstError fe;
         const double t0 = now_ms();
-        ws.run(order, [&](fz_ctx* w, int i) {
-            fz_batch_dec_item& it = items[i];
-            fz_opts o;
-            fz_opts_init(&o);
-            it.status = fz_decompress_device(w, &it.conf, it.dtype, it.h_pre, it.d_pre, it.pre_len, it.d_out,
-                                             it.out_cap_elems, &o);
-            fe.note(it.status, fz_last_error(w));
-        }, active);
+        // FALCONS_BATCH_PER_ITEM=1: every item through its own plain call on the workers (no shared launch)
+        const char* per_item = getenv("FALCONS_BATCH_PER_ITEM");
+        if (per_item && atoi(per_item) != 0) {
+            size_t need = 0;
+            for (int i = 0; i < n; i++)
+                need = std::max(need, decompress_workspace_bytes(items[i].conf.num, items[i].dtype, items[i].conf.num,
+                                                                 items[i].pre_len));
+            const int active = active_workers(ws, need, ws.held());
+            ws.run(order, [&](fz_ctx* w, int i) {
+                fz_batch_dec_item& it = items[i];
+                fz_opts o;
+                fz_opts_init(&o);
+                it.status = fz_decompress_device(w, &it.conf, it.dtype, it.h_pre, it.d_pre, it.pre_len, it.d_out,
+                                                 it.out_cap_elems, &o);
+                fe.note(it.status, fz_last_error(w));
+            }, active);
+        } else if (n > 0) {
+            decompress_grouped(ctx, ws, items, order, fe);
+        }
         if (opts) {
             opts->t_total_ms = now_ms() - t0;
             opts->t_device_ms = opts->t_total_ms;

@@ -88,6 +88,36 @@ def test_batch_matches_per_call(ctx):
         o.free()
 
 
+def test_decompress_batch_shared_launch_mixed(ctx):
+    """The batched decompress decodes the sequential chains of every item in one launch between the
+    items' two halves. Items whose chains are long enough for the parallel decoder (160^3, 2^19+
+    symbols a chain) run it inside their first half; every item's field must equal a plain call's."""
+    shapes = [((40, 50, 60), np.float32, 1, 1e-3), ((160, 160, 160), np.float32, 1, 1e-3),
+              ((1400, 1500), np.float32, 0, 1e-3), ((33, 34, 35), np.float64, 1, 1e-5),
+              ((150, 170, 160), np.float32, 1, 1e-2), ((20, 21, 22), np.float32, 0, 1e-4)]
+    items, refs, bufs = [], [], []
+    for k, (shape, dtype, mode, eb) in enumerate(shapes):
+        data = synth.power_law(shape, -3.0, seed=70 + k, dtype=dtype)
+        cfg = config_for(data, mode, eb)
+        d, p = ctx.alloc(data.nbytes), ctx.alloc(ctx.prezstd_bound(cfg, dtype))
+        d.upload(data)
+        n, _ = ctx.compress_device(cfg, dtype, d, p)
+        pre = p.download(n)
+        ref = ctx.alloc(data.nbytes)
+        ctx.decompress_device(cfg, dtype, pre, None, ref)
+        refs.append(ref.download(data.nbytes, dtype))
+        o = ctx.alloc(data.nbytes)
+        items.append((cfg, dtype, pre, p if k % 2 else None, o))
+        bufs += [d, p, ref, o]
+    ctx.set_workers(4)
+    status, _ = ctx.decompress_batch(items)
+    assert status == [0] * len(items)
+    for (cfg, dtype, _, _, o), ref in zip(items, refs):
+        assert np.array_equal(o.download(ref.nbytes, dtype).view(np.uint8), ref.view(np.uint8))
+    for b in bufs:
+        b.free()
+
+
 def test_batch_reports_item_errors(ctx):
     import falcons_amd
     data = synth.power_law((20, 21, 22), -3.0, seed=3, dtype=np.float32)
