@@ -197,10 +197,15 @@ constexpr size_t kGroupBytes = size_t(16) << 30;
 void decompress_grouped(fz_ctx* ctx, WorkerSet& wset, fz_batch_dec_item* items, const std::vector<int>& order,
                         FirstError& fe) {
     const size_t n = order.size();
+    // FALCONS_BATCH_SLICE_BYTES: every item's workspace slice this size instead (a test switch: items
+    // that do not fit go through the plain call)
+    const char* slice_env = getenv("FALCONS_BATCH_SLICE_BYTES");
+    const size_t slice = slice_env ? (size_t)std::strtoull(slice_env, nullptr, 10) : 0;
     std::vector<size_t> need(n);
     for (size_t k = 0; k < n; k++) {
         const fz_batch_dec_item& it = items[order[k]];
-        need[k] = (dec_call_workspace_bytes(it.conf.num, it.dtype, it.conf.num, it.pre_len) + 255) & ~size_t(255);
+        need[k] = slice ? slice : dec_call_workspace_bytes(it.conf.num, it.dtype, it.conf.num, it.pre_len);
+        need[k] = (std::max<size_t>(need[k], 4096) + 255) & ~size_t(255);
     }
     size_t free_b = 0, total_b = 0;
     FZ_HIP(hipMemGetInfo(&free_b, &total_b));

@@ -118,6 +118,35 @@ def test_decompress_batch_shared_launch_mixed(ctx):
         b.free()
 
 
+@pytest.mark.parametrize("slice_bytes", [1 << 20, 12 << 20])
+def test_decompress_batch_slice_fallback(ctx, monkeypatch, slice_bytes):
+    """Items whose first part does not fit their workspace slice are decompressed by a plain call
+    instead (1 MiB: every item; 12 MiB: the larger ones), beside items that go through the shared
+    launch: every field equals a plain call's."""
+    items, refs, bufs = [], [], []
+    for k, shape in enumerate([(24, 30, 36), (96, 100, 110), (40, 41, 42), (128, 120, 100), (20, 20, 20)]):
+        data = synth.power_law(shape, -3.0, seed=90 + k, dtype=np.float32)
+        cfg = config_for(data, 1, 1e-3)
+        d, p = ctx.alloc(data.nbytes), ctx.alloc(ctx.prezstd_bound(cfg, np.float32))
+        d.upload(data)
+        n, _ = ctx.compress_device(cfg, np.float32, d, p)
+        pre = p.download(n)
+        ref = ctx.alloc(data.nbytes)
+        ctx.decompress_device(cfg, np.float32, pre, None, ref)
+        refs.append(ref.download(data.nbytes, np.float32))
+        o = ctx.alloc(data.nbytes)
+        items.append((cfg, np.float32, pre, p, o))
+        bufs += [d, p, ref, o]
+    monkeypatch.setenv("FALCONS_BATCH_SLICE_BYTES", str(slice_bytes))
+    ctx.set_workers(3)
+    status, _ = ctx.decompress_batch(items)
+    assert status == [0] * len(items)
+    for (_, dtype, _, _, o), ref in zip(items, refs):
+        assert np.array_equal(o.download(ref.nbytes, dtype).view(np.uint8), ref.view(np.uint8))
+    for b in bufs:
+        b.free()
+
+
 def test_batch_reports_item_errors(ctx):
     import falcons_amd
     data = synth.power_law((20, 21, 22), -3.0, seed=3, dtype=np.float32)
