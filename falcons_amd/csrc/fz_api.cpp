@@ -735,10 +735,11 @@ PreLayout parse_pre(const fz_config& c, int dtype, const uint8_t* h_pre, size_t 
     return L;
 }
 
-// A decompress call in two parts around the entropy decoder's sequential chains (entropy_decode_begin):
-// decompress_begin parses the buffer and queues everything up to those chains on st;
-// decompress_finish queues the rest (entropy_decode_end, escape positions, interpolation; the caller
-// waits for it). Between them the state lives here, its device memory in the call's arena.
+// A decompress call in three parts around the entropy decoder's shared launches (entropy_decode_start,
+// middle, end): decompress_begin parses the buffer and queues the work up to the FSE blocks;
+// decompress_middle the entropy decode up to the sequential chains; decompress_finish the rest
+// (escape positions, interpolation; the caller waits for it). Between them the state lives here,
+// its device memory in the call's arena.
 struct DecompressCall {
     PreLayout PL;
     int dtype = 0;
@@ -755,13 +756,13 @@ struct DecompressCall {
     uint64_t* nesc_dev = nullptr;
     void* sel_tmp = nullptr;
     size_t sel_bytes = 0;
-    EntropyTail tail{};
+    std::unique_ptr<EntropyCall, void (*)(EntropyCall*)> ent{entropy_call_new(), entropy_call_free};
 };
 
-// ws: the call's arena (reset here); defer: null to decode the sequential chains in the call.
-void decompress_begin(DecompressCall& D, DevArena& ws, hipStream_t st, const AuxStream& aux, const fz_config& c,
-                      int dtype, const uint8_t* h_pre, const uint8_t* d_pre, size_t pre_len, void* d_out,
-                      DenseBatch* defer, Timer* tm) {
+// ws: the call's arena (reset here); fse_defer: null to decode the FSE blocks in the call.
+void decompress_begin(DecompressCall& D, DevArena& ws, hipStream_t st, const fz_config& c, int dtype,
+                      const uint8_t* h_pre, const uint8_t* d_pre, size_t pre_len, void* d_out, FseBatch* fse_defer,
+                      Timer* tm) {
     const size_t es = elem_size(dtype);
     const uint64_t n = c.num;
     D.PL = parse_pre(c, dtype, h_pre, pre_len);
@@ -802,7 +803,12 @@ void decompress_begin(DecompressCall& D, DevArena& ws, hipStream_t st, const Aux
     FZ_HIP(hipStreamSynchronize(st));  // the host sources of the copies above (ratio) leave scope
     if (tm) tm->start();
     if (d_pre && nun) FZ_HIP(hipMemcpyAsync(D.d_unpred, d_pre + un_off, (size_t)nun * es, hipMemcpyDeviceToDevice, st));
-    D.tail = entropy_decode_begin(h_pre + enc_off, d_enc, enc_len, ncodes, D.codes, ws, st, aux, defer);
+    entropy_decode_start(D.ent.get(), h_pre + enc_off, d_enc, enc_len, ncodes, D.codes, ws, st, fse_defer);
+}
+
+// defer: null to decode the sequential chains in the call.
+void decompress_middle(DecompressCall& D, DevArena& ws, hipStream_t st, const AuxStream& aux, DenseBatch* defer) {
+    entropy_decode_middle(D.ent.get(), ws, st, aux, defer);
 }
 
 void decompress_finish(DecompressCall& D, DevArena& ws, hipStream_t st) {
@@ -816,7 +822,7 @@ void decompress_finish(DecompressCall& D, DevArena& ws, hipStream_t st) {
     void* d_unpred = D.d_unpred;
     int16_t* codes = D.codes;
     uint64_t* esc_idx = D.esc_idx;
-    entropy_decode_end(D.tail, ws, st);
+    entropy_decode_end(D.ent.get(), ws, st);
     select_escapes(codes, ncodes, esc_idx, D.nesc_dev, D.sel_tmp, D.sel_bytes, st);
     uint64_t nesc = 0;
     FZ_HIP(hipMemcpyAsync(&nesc, D.nesc_dev, 8, hipMemcpyDeviceToHost, st));
@@ -863,8 +869,9 @@ void gpu_decompress(fz_ctx* ctx, const fz_config& c, int dtype, const uint8_t* h
     ArenaReport arena_report{ws, "decompress", c.num};
     Timer tm{st, ctx->ev0, ctx->ev1};
     DecompressCall D;
-    decompress_begin(D, ws, st, ctx->aux, c, dtype, h_pre, d_pre, pre_len, *d_out_io, nullptr, &tm);
+    decompress_begin(D, ws, st, c, dtype, h_pre, d_pre, pre_len, *d_out_io, nullptr, &tm);
     *d_out_io = D.d_out;
+    decompress_middle(D, ws, st, ctx->aux, nullptr);
     decompress_finish(D, ws, st);
     const double tdev = tm.stop();
     if (o) {
@@ -882,8 +889,11 @@ struct fz::DecCall {
 fz::DecCall* fz::dec_call_new() { return new DecCall; }
 void fz::dec_call_free(DecCall* d) { delete d; }
 void fz::dec_call_begin(DecCall* d, DevArena& ws, hipStream_t st, const fz_config& c, int dtype, const uint8_t* h_pre,
-                        const uint8_t* d_pre, size_t pre_len, void* d_out, DenseBatch* defer) {
-    decompress_begin(d->D, ws, st, AuxStream{}, c, dtype, h_pre, d_pre, pre_len, d_out, defer, nullptr);
+                        const uint8_t* d_pre, size_t pre_len, void* d_out, FseBatch* defer) {
+    decompress_begin(d->D, ws, st, c, dtype, h_pre, d_pre, pre_len, d_out, defer, nullptr);
+}
+void fz::dec_call_middle(DecCall* d, DevArena& ws, hipStream_t st, DenseBatch* defer) {
+    decompress_middle(d->D, ws, st, AuxStream{}, defer);
 }
 void fz::dec_call_finish(DecCall* d, DevArena& ws, hipStream_t st) {
     decompress_finish(d->D, ws, st);

@@ -55,15 +55,17 @@ size_t compress_resident(fz_ctx* ctx, const fz_config* conf, int dtype, void* d_
 void decompress_to_device(fz_ctx* ctx, const uint8_t* cmp, size_t len, int dtype, void** d_out_io,
                           size_t out_cap_elems, fz_config* conf_out, fz_opts* o);
 
-// fz_decompress_device in two parts around the entropy decoder's sequential chains, for the batched
-// decompress: begin queues the call's work up to those chains on st and hands the chains to defer;
-// finish (once dense_batch_run's launch is done, in stream order) queues the rest and waits for it.
+// fz_decompress_device in three parts around the entropy decoder's shared launches, for the batched
+// decompress: begin queues the call's work up to its FSE blocks on st and hands them to fse; middle
+// (after fse_batch_run's launch, in stream order) queues the work up to the order-1 sequential chains
+// and hands them to dense; finish (after dense_batch_run's launch) queues the rest and waits for it.
 // The call's device memory comes from ws, which must stay untouched from begin to finish.
 struct DecCall;
 DecCall* dec_call_new();
 void dec_call_free(DecCall* d);
 void dec_call_begin(DecCall* d, DevArena& ws, hipStream_t st, const fz_config& c, int dtype, const uint8_t* h_pre,
-                    const uint8_t* d_pre, size_t pre_len, void* d_out, DenseBatch* defer);
+                    const uint8_t* d_pre, size_t pre_len, void* d_out, FseBatch* fse);
+void dec_call_middle(DecCall* d, DevArena& ws, hipStream_t st, DenseBatch* dense);
 void dec_call_finish(DecCall* d, DevArena& ws, hipStream_t st);
 // Device memory such a call takes from a bound arena (less margin than decompress_workspace_bytes,
 // whose arena can grow); a call that needs more throws std::length_error.

@@ -702,7 +702,7 @@ __global__ void __launch_bounds__(256) k_fse_dec_blocks(const FseBlk* blks, int*
     __syncthreads();
     const uint64_t hl = L.hl;
     if (hl == kFseErr || hl >= srcSize) {
-        if (threadIdx.x == 0) *err = 1;
+        if (threadIdx.x == 0) err[B.pad] = 1;
         return;
     }
     const unsigned tl = L.tl;
@@ -713,13 +713,13 @@ __global__ void __launch_bounds__(256) k_fse_dec_blocks(const FseBlk* blks, int*
         for (uint32_t i = threadIdx.x; i < bsz; i += blockDim.x) S.in[i] = src[hl + i];
         __syncthreads();
         if (!spread_parallel(S.tab, tl)) {
-            if (threadIdx.x == 0) *err = 1;
+            if (threadIdx.x == 0) err[B.pad] = 1;
             return;
         }
         dtable_parallel(S.tab, tl, S.dt);
         __syncthreads();
         if (threadIdx.x == 0) {
-            if (S.in[bsz - 1] == 0) *err = 1;
+            if (S.in[bsz - 1] == 0) err[B.pad] = 1;
             else fse_decode_run_lds(lds(S.dt), lds(S.in), bsz, tl, dst, len);
         }
         return;
@@ -728,14 +728,14 @@ __global__ void __launch_bounds__(256) k_fse_dec_blocks(const FseBlk* blks, int*
     G.tab.norm[threadIdx.x] = L.norm[threadIdx.x];
     __syncthreads();
     if (!spread_parallel(G.tab, tl)) {
-        if (threadIdx.x == 0) *err = 1;
+        if (threadIdx.x == 0) err[B.pad] = 1;
         return;
     }
     dtable_parallel(G.tab, tl, G.dt);
     __syncthreads();
     if (threadIdx.x != 0) return;
     if (src[hl + bsz - 1] == 0) {
-        *err = 1;
+        err[B.pad] = 1;
         return;
     }
     fse_decode_run(G.dt, src + hl, bsz, tl, dst, len);
@@ -2947,8 +2947,7 @@ struct FseDecItem {
     uint64_t avail;  // bytes of the framed stream
     uint8_t* out;
 };
-static void fse_dec_multi(const std::vector<FseDecItem>& items, int* err, DevArena& ws, hipStream_t st) {
-    std::vector<FseBlk> blks;
+static void fse_block_list(const std::vector<FseDecItem>& items, std::vector<FseBlk>& blks) {
     std::vector<uint64_t> offs;
     for (const FseDecItem& it : items) {
         if (it.n == 0) continue;
@@ -2959,6 +2958,10 @@ static void fse_dec_multi(const std::vector<FseDecItem>& items, int* err, DevAre
             blks.push_back(FseBlk{it.dsrc + offs[k], it.out + k * kFblk,
                                   (uint32_t)std::min<uint64_t>(kFblk, it.n - k * kFblk), 0});
     }
+}
+static void fse_dec_multi(const std::vector<FseDecItem>& items, int* err, DevArena& ws, hipStream_t st) {
+    std::vector<FseBlk> blks;
+    fse_block_list(items, blks);
     if (blks.empty()) return;
     FseBlk* dblks = ws.take<FseBlk>(blks.size());
     h2d(dblks, blks.data(), blks.size(), st);
@@ -3425,43 +3428,135 @@ void entropy_check(const uint8_t* host, size_t len, uint64_t n) {
     parse_entropy(host, len, n, Lay);
 }
 
-void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
-                    hipStream_t st, const AuxStream& aux) {
-    entropy_decode_end(entropy_decode_begin(host, dev, len, n, codes, ws, st, aux, nullptr), ws, st);
+// FSE blocks of several calls, decoded together (fse_batch_run): each call's blocks flag errors in
+// the call's own flag.
+struct FseBatch {
+    std::mutex mu;
+    std::vector<FseBlk> blks;  // pad: index into errs
+    std::vector<int*> errs;
+};
+FseBatch* fse_batch_create() { return new FseBatch; }
+void fse_batch_destroy(FseBatch* b) { delete b; }
+void fse_batch_clear(FseBatch* b) {
+    std::lock_guard<std::mutex> g(b->mu);
+    b->blks.clear();
+    b->errs.clear();
 }
 
-EntropyTail entropy_decode_begin(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes,
-                                 DevArena& ws, hipStream_t st, const AuxStream& aux, DenseBatch* defer) {
+__global__ void k_or_flags(const int* src, int* const* dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && src[i]) *dst[i] = 1;
+}
+
+void fse_batch_run(FseBatch* b, DevArena& ws, hipStream_t st) {
+    std::lock_guard<std::mutex> g(b->mu);
+    if (b->blks.empty()) return;
+    if (b->blks.size() > 0x7fffffffu) throw std::length_error("too many FSE blocks for one launch");
+    const uint32_t ne = (uint32_t)b->errs.size();
+    FseBlk* dblks = ws.take<FseBlk>(b->blks.size());
+    int* gerr = ws.take<int>(ne);
+    int** derr = ws.take<int*>(ne);
+    // the lists stay in b until fse_batch_clear, after the launches have finished
+    FZ_HIP(hipMemcpyAsync(dblks, b->blks.data(), b->blks.size() * sizeof(FseBlk), hipMemcpyHostToDevice, st));
+    FZ_HIP(hipMemcpyAsync(derr, b->errs.data(), ne * sizeof(int*), hipMemcpyHostToDevice, st));
+    FZ_HIP(hipMemsetAsync(gerr, 0, ne * sizeof(int), st));
+    FZ_LAUNCH("fse_dec_blocks", 0, k_fse_dec_blocks, dim3((unsigned)b->blks.size()), dim3(256), 0, st, dblks, gerr);
+    FZ_LAUNCH("fse_dec_flags", 0, k_or_flags, dim3((ne + 255) / 256), dim3(256), 0, st, gerr, derr, ne);
+    FZ_HIP(hipGetLastError());
+}
+
+// A decode in three parts (entropy_decode_start / middle / end); the state between them.
+struct EntropyCall {
     EntropyLayout Lay;
-    parse_entropy(host, len, n, Lay);
-    const int method = Lay.method, use_delta = Lay.use_delta;
-    int* err = ws.take<int>(1);
-    FZ_HIP(hipMemsetAsync(err, 0, 4, st));
-    int16_t* out = use_delta ? ws.take<int16_t>(n + 1) : codes;
-    if (method == 1) {
-        const uint64_t S = Lay.S, total_ovf = Lay.total_ovf;
+    const uint8_t* host = nullptr;
+    const uint8_t* dev = nullptr;
+    uint64_t n = 0;
+    int16_t* codes = nullptr;
+    int16_t* out = nullptr;
+    int* err = nullptr;
+    int16_t* ovf = nullptr;       // method 1: escaped codes
+    std::vector<uint8_t*> vcls;   // method 1: class bytes of every segment
+    uint8_t* rcl = nullptr;       // method 0: run and value classes
+    uint8_t* vcl = nullptr;
+};
+EntropyCall* entropy_call_new() { return new EntropyCall; }
+void entropy_call_free(EntropyCall* e) { delete e; }
+
+void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
+                    hipStream_t st, const AuxStream& aux) {
+    EntropyCall E;
+    entropy_decode_start(&E, host, dev, len, n, codes, ws, st, nullptr);
+    entropy_decode_middle(&E, ws, st, aux, nullptr);
+    entropy_decode_end(&E, ws, st);
+}
+
+// Parses the stream and decodes its FSE-coded class bytes (escape classes of method 1, run and
+// value classes of method 0), or hands their blocks to defer.
+void entropy_decode_start(EntropyCall* E, const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n,
+                          int16_t* codes, DevArena& ws, hipStream_t st, FseBatch* defer) {
+    parse_entropy(host, len, n, E->Lay);
+    const EntropyLayout& Lay = E->Lay;
+    E->host = host;
+    E->dev = dev;
+    E->n = n;
+    E->codes = codes;
+    E->err = ws.take<int>(1);
+    FZ_HIP(hipMemsetAsync(E->err, 0, 4, st));
+    E->out = Lay.use_delta ? ws.take<int16_t>(n + 1) : codes;
+    std::vector<FseDecItem> fd;
+    if (Lay.method == 1) {
+        E->ovf = ws.take<int16_t>(Lay.total_ovf + 1);
+        E->vcls.assign(Lay.S, nullptr);
+        for (uint64_t g = 0; g < Lay.S; g++) {
+            const OvfPiece& pc = Lay.pieces[g];
+            if (!pc.n || pc.mode == 0) continue;
+            E->vcls[g] = ws.take<uint8_t>(pc.n);
+            fd.push_back(FseDecItem{host + pc.vcl_off, dev + pc.vcl_off, pc.n, pc.svcl, E->vcls[g]});
+        }
+    } else {
+        E->rcl = ws.take<uint8_t>(Lay.nr + 1);
+        E->vcl = ws.take<uint8_t>(Lay.nv + 1);
+        fd = {{host + Lay.o_rcl, dev + Lay.o_rcl, Lay.nr, Lay.s_rcl, E->rcl},
+              {host + Lay.o_vcl, dev + Lay.o_vcl, Lay.nv, Lay.s_vcl, E->vcl}};
+    }
+    if (!defer) {
+        fse_dec_multi(fd, E->err, ws, st);
+        return;
+    }
+    std::vector<FseBlk> blks;
+    fse_block_list(fd, blks);
+    std::lock_guard<std::mutex> g(defer->mu);
+    const uint32_t k = (uint32_t)defer->errs.size();
+    defer->errs.push_back(E->err);
+    for (FseBlk& b : blks) {
+        b.pad = k;
+        defer->blks.push_back(b);
+    }
+}
+
+// Everything after the class bytes up to the decoded codes: method 1's escaped values, tables and
+// chains (the sequential ones handed to defer when given), method 0's runs and values.
+void entropy_decode_middle(EntropyCall* E, DevArena& ws, hipStream_t st, const AuxStream& aux, DenseBatch* defer) {
+    const EntropyLayout& Lay = E->Lay;
+    const uint8_t* host = E->host;
+    const uint8_t* dev = E->dev;
+    const uint64_t n = E->n;
+    int* err = E->err;
+    int16_t* out = E->out;
+    if (Lay.method == 1) {
+        const uint64_t S = Lay.S;
         const std::vector<uint64_t>& chains = Lay.chains;
         const std::vector<uint64_t>& nf_off = Lay.nf_off;
         const std::vector<uint64_t>& ovf_base = Lay.ovf_base;
         const std::vector<OvfPiece>& pieces = Lay.pieces;
-        int16_t* ovf = ws.take<int16_t>(total_ovf + 1);
-        // the class bytes of every segment in one batch
-        std::vector<uint8_t*> vcls(S, nullptr);
-        std::vector<FseDecItem> fd;
-        for (uint64_t g = 0; g < S; g++) {
-            const OvfPiece& pc = pieces[g];
-            if (!pc.n || pc.mode == 0) continue;
-            vcls[g] = ws.take<uint8_t>(pc.n);
-            fd.push_back(FseDecItem{host + pc.vcl_off, dev + pc.vcl_off, pc.n, pc.svcl, vcls[g]});
-        }
-        fse_dec_multi(fd, err, ws, st);
+        int16_t* ovf = E->ovf;
         for (uint64_t g = 0; g < S; g++) {
             const OvfPiece& pc = pieces[g];
             if (!pc.n) continue;
             if (pc.mode == 0) {
                 FZ_HIP(hipMemcpyAsync(ovf + ovf_base[g], dev + pc.nraw_off, pc.n * 2, hipMemcpyDeviceToDevice, st));
             } else {
-                uint8_t* vcl = vcls[g];
+                uint8_t* vcl = E->vcls[g];
                 uint64_t* voff = ws.take<uint64_t>(pc.n + 1);
                 {
                     hipcub::TransformInputIterator<uint64_t, ToU64, const uint8_t*> it(vcl, ToU64{});
@@ -3487,12 +3582,10 @@ EntropyTail entropy_decode_begin(const uint8_t* host, const uint8_t* dev, size_t
         }
         o1_decode_parallel(host, dev, chains, nf_off, dt, ovf, out, err, ws, st, aux, defer);
     } else {
-        const uint64_t nr = Lay.nr, nv = Lay.nv, s_rcl = Lay.s_rcl, s_vcl = Lay.s_vcl, rbp = Lay.rbp, vbp = Lay.vbp;
-        const uint64_t o_rcl = Lay.o_rcl, o_vcl = Lay.o_vcl, o_rman = Lay.o_rman, o_vman = Lay.o_vman;
-        uint8_t* rcl = ws.take<uint8_t>(nr + 1);
-        uint8_t* vcl = ws.take<uint8_t>(nv + 1);
-        fse_dec_multi({{host + o_rcl, dev + o_rcl, nr, s_rcl, rcl}, {host + o_vcl, dev + o_vcl, nv, s_vcl, vcl}}, err, ws,
-                      st);
+        const uint64_t nr = Lay.nr, nv = Lay.nv, rbp = Lay.rbp, vbp = Lay.vbp;
+        const uint64_t o_rman = Lay.o_rman, o_vman = Lay.o_vman;
+        uint8_t* rcl = E->rcl;
+        uint8_t* vcl = E->vcl;
         // bit offsets of the mantissas
         uint64_t* rk = ws.take<uint64_t>(nr + 1);
         uint64_t* vk = ws.take<uint64_t>(nv + 1);
@@ -3528,19 +3621,17 @@ EntropyTail entropy_decode_begin(const uint8_t* host, const uint8_t* dev, size_t
                           vbp * 8, nv, runsum, n, out, err);
         FZ_HIP(hipGetLastError());
     }
-    return EntropyTail{use_delta, out, codes, err, n};
 }
 
-void entropy_decode_end(const EntropyTail& t, DevArena& ws, hipStream_t st) {
-    if (t.use_delta && t.n > 0) {
+void entropy_decode_end(EntropyCall* E, DevArena& ws, hipStream_t st) {
+    if (E->Lay.use_delta && E->n > 0) {
         size_t b = 0;
-        FZ_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, b, t.out, t.codes, Add16{}, (int64_t)t.n, st));
+        FZ_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, b, E->out, E->codes, Add16{}, (int64_t)E->n, st));
         void* tmp = cub_scratch(ws, b);
-        FZ_HIP(hipcub::DeviceScan::InclusiveScan(tmp, b, t.out, t.codes, Add16{}, (int64_t)t.n, st));
+        FZ_HIP(hipcub::DeviceScan::InclusiveScan(tmp, b, E->out, E->codes, Add16{}, (int64_t)E->n, st));
     }
     int herr = 0;
-    d2h(&herr, t.err, 1, st);
+    d2h(&herr, E->err, 1, st);
     if (herr) throw std::runtime_error("corrupt entropy stream");
 }
-
 }  // namespace fz

@@ -155,26 +155,29 @@ void entropy_check(const uint8_t* host, size_t len, uint64_t n);
 void entropy_decode(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes, DevArena& ws,
                     hipStream_t st, const AuxStream& aux);
 
-// The same in two parts around the order-1 decoder's sequential chains. With a DenseBatch, begin
-// hands those chains to it instead of decoding them; dense_batch_run then decodes the chains of
-// every call in the batch in one launch, after each call's begin work (the caller orders the
-// streams), and end runs once that launch is done. Without one, begin decodes them itself.
+// The same in three parts, for the batched decompress, which decodes the FSE blocks and the
+// order-1 decoder's sequential chains of many calls in shared launches: start parses the stream
+// and decodes the FSE-coded class bytes, or hands their blocks to an FseBatch; middle decodes the
+// rest, handing the sequential chains to a DenseBatch when given; end finishes the codes. The caller
+// orders the streams: fse_batch_run and dense_batch_run run after every call's previous part, and
+// the next part after them. Without batches, start and middle decode everything themselves.
+struct FseBatch;
+FseBatch* fse_batch_create();
+void fse_batch_destroy(FseBatch* b);
+void fse_batch_run(FseBatch* b, DevArena& ws, hipStream_t st);
+void fse_batch_clear(FseBatch* b);  // after fse_batch_run's launches have finished
 struct DenseBatch;
 DenseBatch* dense_batch_create();
 void dense_batch_destroy(DenseBatch* b);
 size_t dense_batch_size(DenseBatch* b);
 void dense_batch_run(DenseBatch* b, DevArena& ws, hipStream_t st);
-// forgets the collected chains (after dense_batch_run's launches have finished)
-void dense_batch_clear(DenseBatch* b);
-struct EntropyTail {
-    int use_delta;
-    int16_t* out;
-    int16_t* codes;
-    int* err;
-    uint64_t n;
-};
-EntropyTail entropy_decode_begin(const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n, int16_t* codes,
-                                 DevArena& ws, hipStream_t st, const AuxStream& aux, DenseBatch* defer);
-void entropy_decode_end(const EntropyTail& t, DevArena& ws, hipStream_t st);
+void dense_batch_clear(DenseBatch* b);  // after dense_batch_run's launches have finished
+struct EntropyCall;
+EntropyCall* entropy_call_new();
+void entropy_call_free(EntropyCall* e);
+void entropy_decode_start(EntropyCall* e, const uint8_t* host, const uint8_t* dev, size_t len, uint64_t n,
+                          int16_t* codes, DevArena& ws, hipStream_t st, FseBatch* defer);
+void entropy_decode_middle(EntropyCall* e, DevArena& ws, hipStream_t st, const AuxStream& aux, DenseBatch* defer);
+void entropy_decode_end(EntropyCall* e, DevArena& ws, hipStream_t st);
 
 }  // namespace fz

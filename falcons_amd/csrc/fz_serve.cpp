@@ -187,13 +187,14 @@ namespace {
 constexpr size_t kGroupItems = 1024;
 constexpr size_t kGroupBytes = size_t(16) << 30;
 
-// fz_decompress_batch in groups of items. Each group runs in three steps: the workers run every
-// item's call up to the entropy decoder's sequential chains (dec_call_begin, the call's workspace
-// a slice of one group allocation), handing those chains to one DenseBatch; the chains of the
-// whole group then decode in one launch on the batch's own stream, after every item's first part;
-// and the workers finish every item after that launch. A thread pool of per-item calls keeps at
-// most a few chains per worker on the GPU at a time; one launch holds all of them. An item whose
-// first part does not fit its slice or fails is decompressed by a plain call afterwards.
+// fz_decompress_batch in groups of items. The workers run every item's call in three parts
+// (dec_call_begin, middle, finish; the call's workspace a slice of one group allocation), and
+// between the parts the longest-running kernels of the whole group run as one launch each on the
+// batch's own stream: after the first part, the FSE blocks of every item's class bytes; after the
+// second, every item's order-1 sequential chains. A pool of whole per-item calls keeps only a few
+// kernels on the GPU at a time, each a handful of workgroups running for milliseconds; one launch
+// holds them all. An item whose parts do not fit its slice, or fail, is decompressed by a plain
+// call afterwards.
 void decompress_grouped(fz_ctx* ctx, WorkerSet& wset, fz_batch_dec_item* items, const std::vector<int>& order,
                         FirstError& fe) {
     const size_t n = order.size();
@@ -211,6 +212,7 @@ void decompress_grouped(fz_ctx* ctx, WorkerSet& wset, fz_batch_dec_item* items, 
     FZ_HIP(hipMemGetInfo(&free_b, &total_b));
     const size_t budget = std::max<size_t>(size_t(256) << 20, std::min(kGroupBytes, free_b / 10 * 4));
     struct Res {  // released however the batch ends
+        FseBatch* fse = fse_batch_create();
         DenseBatch* batch = dense_batch_create();
         void* pool = nullptr;
         std::vector<hipEvent_t> ev;
@@ -220,6 +222,7 @@ void decompress_grouped(fz_ctx* ctx, WorkerSet& wset, fz_batch_dec_item* items, 
             for (hipEvent_t e : ev) (void)hipEventDestroy(e);
             if (done) (void)hipEventDestroy(done);
             dense_batch_destroy(batch);
+            fse_batch_destroy(fse);
         }
     } R;
     FZ_HIP(hipEventCreateWithFlags(&R.done, hipEventDisableTiming));
@@ -252,23 +255,39 @@ void decompress_grouped(fz_ctx* ctx, WorkerSet& wset, fz_batch_dec_item* items, 
         }
         std::vector<int> idx(g);
         std::iota(idx.begin(), idx.end(), 0);
+        // a shared launch runs after every item's previous part: the items' events, recorded also
+        // after a failure, since work an item handed over reads what its stream has queued
+        auto shared = [&](auto&& launch) {
+            for (size_t j = 0; j < g; j++)
+                if (rec[j]) FZ_HIP(hipStreamWaitEvent(cst, R.ev[j], 0));
+            launch();
+            FZ_HIP(hipEventRecord(R.done, cst));
+        };
         wset.run(idx, [&](fz_ctx* w, int j) {
             fz_batch_dec_item& it = items[order[k0 + j]];
             if (!it.d_out || it.conf.num > it.out_cap_elems) return;  // the plain call reports it
             try {
                 dec_call_begin(calls[j].get(), *ar[j], w->stream, it.conf, it.dtype, it.h_pre, it.d_pre, it.pre_len,
-                               it.d_out, R.batch);
+                               it.d_out, R.fse);
                 ok[j] = 1;
             } catch (...) {
             }
-            // also after a failure: chains the item handed over read what its stream has queued
             rec[j] = hipEventRecord(R.ev[j], w->stream) == hipSuccess;
         }, wset.size());
-        for (size_t j = 0; j < g; j++)
-            if (rec[j]) FZ_HIP(hipStreamWaitEvent(cst, R.ev[j], 0));
         ctx->arena.reset();
-        dense_batch_run(R.batch, ctx->arena, cst);
-        FZ_HIP(hipEventRecord(R.done, cst));
+        shared([&] { fse_batch_run(R.fse, ctx->arena, cst); });
+        wset.run(idx, [&](fz_ctx* w, int j) {
+            rec[j] = 0;
+            if (!ok[j]) return;
+            try {
+                FZ_HIP(hipStreamWaitEvent(w->stream, R.done, 0));
+                dec_call_middle(calls[j].get(), *ar[j], w->stream, R.batch);
+            } catch (...) {
+                ok[j] = 0;
+            }
+            rec[j] = hipEventRecord(R.ev[j], w->stream) == hipSuccess;
+        }, wset.size());
+        shared([&] { dense_batch_run(R.batch, ctx->arena, cst); });
         wset.run(idx, [&](fz_ctx* w, int j) {
             fz_batch_dec_item& it = items[order[k0 + j]];
             if (ok[j]) {
@@ -287,6 +306,7 @@ void decompress_grouped(fz_ctx* ctx, WorkerSet& wset, fz_batch_dec_item* items, 
             fe.note(it.status, fz_last_error(w));
         }, wset.size());
         FZ_HIP(hipStreamSynchronize(cst));
+        fse_batch_clear(R.fse);
         dense_batch_clear(R.batch);
         k0 = k1;
     }
