@@ -58,6 +58,11 @@ __host__ __device__ inline int o1_sym(int code) {
 #define FZ_LDS __attribute__((address_space(3)))
 template <typename T>
 __device__ __forceinline__ FZ_LDS T* lds(T* p) { return (FZ_LDS T*)p; }
+// Device-memory pointers likewise: global instructions count only in vmcnt, so loads issued ahead
+// stay in flight across the LDS waits of a dependent chain (flat loads count in lgkmcnt as well).
+#define FZ_GLB __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ FZ_GLB T* glb(T* p) { return (FZ_GLB T*)p; }
 
 // ---------------------------------------------------------------- small device helpers
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -352,6 +357,8 @@ struct FseLds {
 // n 16-bit values of scratch (the state before every symbol's step). Returns 0 (not compressible),
 // 1 (one symbol), kFseErr, or the compressed size.
 __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_t* fo, FseLds& L, uint16_t* pre) {
+    const uint8_t* const src_in = src;
+    uint16_t* const pre_in = pre;
     constexpr uint64_t kPending = ~1ull;
     for (int s = threadIdx.x; s < 256; s += blockDim.x) L.count[s] = 0;
     __syncthreads();
@@ -404,26 +411,52 @@ __device__ uint64_t fse_block_compress_wg(const uint8_t* src, uint32_t n, uint8_
     const FZ_LDS uint16_t* stt = lds(L.ct.stateTable);
     if (threadIdx.x < 2) {
         const uint32_t k = threadIdx.x;  // chain k emits e = k, k + 2, ... (symbol n - 3 - e)
+        const FZ_GLB uint8_t* src = glb(src_in);
+        FZ_GLB uint16_t* pre = glb(pre_in);
         uint32_t s = init_state2(&L.ct, src[n - 1 - k]);
         int64_t j = (int64_t)n - 3 - k;
-        // four steps per round; the round's symbols and their table values are read first (they do
-        // not depend on the state)
-        while (j >= 6) {
-            uint32_t sy[4], d[4];
+        // Four steps per round (symbols jj, jj - 2, jj - 4, jj - 6); the round's table values are
+        // read first (they do not depend on the state), and its symbols two rounds earlier, so that
+        // their loads are in flight for the eight steps before them. Three register sets take turns
+        // (no copies between them, which would wait for the loads); reads below symbol 0 are clamped
+        // to it and not used.
+        auto ld4 = [&](uint32_t (&g)[4], int64_t jj) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) g[r] = src[max(jj - 2 * r, (int64_t)0)];
+        };
+        auto round4 = [&](const uint32_t (&cur)[4], uint32_t (&ahead)[4], int64_t jj) {
+            ld4(ahead, jj - 16);
+            uint32_t d[4];
             int32_t f[4];
 #pragma unroll
-            for (int r = 0; r < 4; r++) sy[r] = src[j - 2 * r];
-#pragma unroll
             for (int r = 0; r < 4; r++) {
-                d[r] = dnb[sy[r]];
-                f[r] = dfs[sy[r]];
+                d[r] = dnb[cur[r]];
+                f[r] = dfs[cur[r]];
             }
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                pre[j - 2 * r] = (uint16_t)s;
+                pre[jj - 2 * r] = (uint16_t)s;
                 const uint32_t nb = (s + d[r]) >> 16;
                 s = stt[(s >> nb) + f[r]];
             }
+        };
+        uint32_t sa[4], sb[4], sc[4];
+        ld4(sa, j);
+        ld4(sb, j - 8);
+        // Wait for these first loads here, once (an empty statement that takes them as operands):
+        // the loop's head then needs only the waits of its own iterations; the compiler otherwise
+        // takes the head's wait from this entry path, where the loads were just issued.
+        asm volatile("" ::"v"(sa[0]), "v"(sa[1]), "v"(sa[2]), "v"(sa[3]), "v"(sb[0]), "v"(sb[1]), "v"(sb[2]),
+                     "v"(sb[3]));
+        for (;;) {
+            if (j < 6) break;
+            round4(sa, sc, j);
+            j -= 8;
+            if (j < 6) break;
+            round4(sb, sa, j);
+            j -= 8;
+            if (j < 6) break;
+            round4(sc, sb, j);
             j -= 8;
         }
         for (; j >= 0; j -= 2) {
