@@ -132,6 +132,20 @@ def test_bound_edge_f32(ctx, oracle):
     check(ctx, oracle, d, eb_mode=0, abs_eb=6.1e-5)
 
 
+@pytest.mark.parametrize("bumps", [2, 3, 4, 5, 6, 7, 9, 12, 15, 20, 23, 31, 45, 64, 100, 150, 250, 400])
+def test_rle_fse_short_class_streams(ctx, oracle, bumps):
+    """A flat 1-D field with a few bumps: nearly every code is zero, so the RLE+FSE method codes a
+    run-class and a value-class stream of about `bumps` bytes each. The FSE block encoder reads each
+    state chain's symbols ahead of its steps, in rounds of four with clamped reads near symbol 0;
+    these lengths reach its start, its last rounds and its tail at both parities."""
+    rng = np.random.default_rng(1000 + bumps)
+    d = np.zeros(6000, dtype=np.float32)
+    pos = rng.choice(np.arange(8, d.size - 8), size=bumps, replace=False)
+    d[pos] = (rng.choice([-1.0, 1.0], size=bumps) * 10.0 ** rng.uniform(-2, 1, size=bumps)).astype(np.float32)
+    opts = check(ctx, oracle, d, eb_mode=0, abs_eb=1e-3, interp_algo=0)
+    assert opts.method == 0
+
+
 def test_device_entry_points(ctx, oracle):
     import falcons_amd
     shape = (70, 64, 96)
