@@ -20,6 +20,7 @@
 #include <array>
 #include <exception>
 #include <thread>
+#include <type_traits>
 
 #include "fz_common.h"
 #include "fz_fse.h"
@@ -706,10 +707,113 @@ __device__ __forceinline__ void fse_decode_run_lds(const FZ_LDS DEntry* dt, cons
     if (i < len) out[i] = (uint8_t)(dtw[s0] >> 16);
 }
 
+// The layout for launches whose blocks all have tables up to log 11 (all that FSE_compress writes):
+// the bitstream is staged kFseWin bytes at a time, so that a workgroup takes about 31 KB of LDS and
+// five run on a CU (a launch of 1,036 blocks, the escape streams of a 512^3 REL 1e-4 field, then
+// runs in one wave of workgroups instead of three).
+constexpr uint32_t kFseWin = 16384;
+struct FseDecWin {
+    TableLdsT<kFblkLog> tab;
+    DEntry dt[1u << kFblkLog];
+    alignas(16) uint8_t win[kFseWin + 64];
+};
+struct FseDecLdsWin {
+    FseDecWin w;
+    int16_t norm[256];
+    uint32_t tl;
+    uint64_t hl;
+    int32_t wb;  // the next window's first stream byte
+    uint32_t done;
+};
+// Where the decode of fse_decode_run_lds stands: read position, the two states, symbols done.
+struct FseDecPos {
+    int32_t p;
+    uint32_t s0, s1, i;
+};
+// fse_decode_run_lds over a window: win holds the stream bytes [wb, wb + kFseWin + 12) (wb a multiple
+// of 4; bytes past the stream's end are not used). Decodes while the words of the next refill lie in
+// the window, or to the end when the window starts at byte 0; returns whether all len symbols are
+// done. first: the window holds the stream's end; take the start position and the two states.
+__device__ __forceinline__ bool fse_decode_window(FseDecPos& z, const FZ_LDS DEntry* dt, const FZ_LDS uint8_t* win,
+                                                  int32_t wb, bool first, uint64_t bsz, unsigned tl, uint8_t* out,
+                                                  uint32_t len) {
+    const FZ_LDS uint32_t* w32 = reinterpret_cast<const FZ_LDS uint32_t*>(win);
+    const FZ_LDS uint32_t* dtw = reinterpret_cast<const FZ_LDS uint32_t*>(dt);
+    uint64_t buf = 0;
+    int32_t prel = 0;
+    auto refill = [&]() {  // bits [p - 64, p); bytes below the start read as zero
+        const int32_t lo = z.p - 64;
+        const int32_t lc = max(lo, 0);
+        const int32_t j0 = (lc >> 5) - (wb >> 2);
+        const uint32_t sh = (uint32_t)lc & 31u;
+        uint64_t b = (uint64_t)__builtin_amdgcn_alignbit(w32[j0 + 1], w32[j0], sh) |
+                     ((uint64_t)__builtin_amdgcn_alignbit(w32[j0 + 2], w32[j0 + 1], sh) << 32);
+        if (lo < 0) b = (z.p > 0 ? b << (uint32_t)(-lo) : 0ull);
+        buf = b;
+        prel = 64;
+    };
+    auto read = [&](uint32_t nb) -> uint32_t {
+        prel -= (int32_t)nb;
+        z.p -= (int32_t)nb;
+        return (uint32_t)(buf >> ((uint32_t)prel & 63u)) & ((1u << nb) - 1u);
+    };
+    // How far the symbols can go before a refill would reach below the window: a refill needs
+    // p - 64 >= 8 wb (or wb = 0), and a step of `step` symbols reads at most step * 11 bits, so
+    // room / (step * 11) + 1 steps from here stay in the window (none when room < 0).
+    auto limit = [&](uint32_t step) -> uint32_t {
+        if (wb == 0) return len;
+        const int32_t room = z.p - 64 - 8 * wb;
+        if (room < 0) return z.i;
+        const uint64_t k = (uint64_t)(room / (int32_t)(step * kFblkLog)) + 1;
+        return (uint32_t)min<uint64_t>(len, z.i + step * k);
+    };
+    if (first) {
+        z.p = (int32_t)((bsz - 1) * 8) + (int32_t)highbit32(win[bsz - 1 - (uint64_t)wb]);
+        z.i = 0;
+        refill();
+        z.s0 = read(tl);
+        z.s1 = read(tl);
+    }
+    while (z.i + 4 <= len) {
+        const uint32_t lim = limit(4);
+        if (z.i + 4 > lim) return false;
+        for (; z.i + 4 <= lim; z.i += 4) {
+            refill();
+            const uint32_t e0 = dtw[z.s0], e1 = dtw[z.s1];
+            z.s0 = (e0 & 0xFFFFu) + read(e0 >> 24);
+            z.s1 = (e1 & 0xFFFFu) + read(e1 >> 24);
+            const uint32_t f0 = dtw[z.s0], f1 = dtw[z.s1];
+            z.s0 = (f0 & 0xFFFFu) + read(f0 >> 24);
+            z.s1 = (f1 & 0xFFFFu) + read(f1 >> 24);
+            const uint32_t v = ((e0 >> 16) & 0xFFu) | (((e1 >> 16) & 0xFFu) << 8) | (((f0 >> 16) & 0xFFu) << 16) |
+                               (((f1 >> 16) & 0xFFu) << 24);
+            *reinterpret_cast<uint32_t*>(out + z.i) = v;
+        }
+    }
+    while (z.i + 2 <= len) {
+        const uint32_t lim = limit(2);
+        if (z.i + 2 > lim) return false;
+        for (; z.i + 2 <= lim; z.i += 2) {
+            refill();
+            const uint32_t e0 = dtw[z.s0], e1 = dtw[z.s1];
+            out[z.i] = (uint8_t)(e0 >> 16);
+            out[z.i + 1] = (uint8_t)(e1 >> 16);
+            z.s0 = (e0 & 0xFFFFu) + read(e0 >> 24);
+            z.s1 = (e1 & 0xFFFFu) + read(e1 >> 24);
+        }
+    }
+    if (z.i < len) out[z.i] = (uint8_t)(dtw[z.s0] >> 16);
+    z.i = len;
+    return true;
+}
+
 // Decodes framed class-byte blocks (the blocks of every stream of a call in one launch); the host
 // found where each block starts.
+// kBig: the layout of FseDecLds, for launches with a block whose table log is 12 or 13 (streams
+// other encoders wrote); else FseDecLdsWin.
+template <bool kBig>
 __global__ void __launch_bounds__(256) k_fse_dec_blocks(const FseBlk* blks, int* err) {
-    __shared__ FseDecLds L;
+    __shared__ std::conditional_t<kBig, FseDecLds, FseDecLdsWin> L;
     const FseBlk B = blks[blockIdx.x];
     const uint8_t* in = B.src;
     const uint32_t len = B.len;
@@ -740,38 +844,83 @@ __global__ void __launch_bounds__(256) k_fse_dec_blocks(const FseBlk* blks, int*
     }
     const unsigned tl = L.tl;
     const uint64_t bsz = srcSize - hl;
-    if (tl <= kFblkLog && bsz <= kFblk) {
-        FseDecSmall& S = L.s;
-        S.tab.norm[threadIdx.x] = L.norm[threadIdx.x];
-        for (uint32_t i = threadIdx.x; i < bsz; i += blockDim.x) S.in[i] = src[hl + i];
-        __syncthreads();
-        if (!spread_parallel(S.tab, tl)) {
+    if constexpr (!kBig) {
+        if (tl > kFblkLog) {  // the host launches such blocks with the big layout
             if (threadIdx.x == 0) err[B.pad] = 1;
             return;
         }
-        dtable_parallel(S.tab, tl, S.dt);
+        FseDecWin& W = L.w;
+        const uint8_t* bs = src + hl;
+        auto stage = [&](int32_t wb) {
+            const uint64_t cnt = min<uint64_t>(kFseWin + 12, bsz - (uint64_t)wb);
+            for (uint32_t u = threadIdx.x; u < cnt; u += blockDim.x) W.win[u] = bs[(uint64_t)wb + u];
+        };
+        W.tab.norm[threadIdx.x] = L.norm[threadIdx.x];
+        int32_t wb = bsz > kFseWin ? (int32_t)((bsz - kFseWin) & ~(uint64_t)3) : 0;
+        stage(wb);
         __syncthreads();
-        if (threadIdx.x == 0) {
-            if (S.in[bsz - 1] == 0) err[B.pad] = 1;
-            else fse_decode_run_lds(lds(S.dt), lds(S.in), bsz, tl, dst, len);
+        if (!spread_parallel(W.tab, tl)) {
+            if (threadIdx.x == 0) err[B.pad] = 1;
+            return;
         }
-        return;
+        dtable_parallel(W.tab, tl, W.dt);
+        __syncthreads();
+        // lane 0 decodes a window at a time; between windows the workgroup stages the next one
+        // (each lower than the last, the last one starting at byte 0)
+        FseDecPos z{0, 0, 0, 0};
+        for (bool first = true;; first = false) {
+            if (threadIdx.x == 0) {
+                bool fin;
+                if (first && W.win[bsz - 1 - (uint64_t)wb] == 0) {
+                    err[B.pad] = 1;
+                    fin = true;
+                } else {
+                    fin = fse_decode_window(z, lds(W.dt), lds(W.win), wb, first, bsz, tl, dst, len);
+                }
+                const int32_t cur = max(z.p - 64, 0) >> 3;
+                L.wb = cur > (int32_t)kFseWin - 16 ? (cur - ((int32_t)kFseWin - 16)) & ~3 : 0;
+                L.done = fin ? 1u : 0u;
+            }
+            __syncthreads();
+            if (L.done) return;
+            wb = L.wb;
+            stage(wb);
+            __syncthreads();
+        }
+    } else {
+        if (tl <= kFblkLog && bsz <= kFblk) {
+            FseDecSmall& S = L.s;
+            S.tab.norm[threadIdx.x] = L.norm[threadIdx.x];
+            for (uint32_t i = threadIdx.x; i < bsz; i += blockDim.x) S.in[i] = src[hl + i];
+            __syncthreads();
+            if (!spread_parallel(S.tab, tl)) {
+                if (threadIdx.x == 0) err[B.pad] = 1;
+                return;
+            }
+            dtable_parallel(S.tab, tl, S.dt);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                if (S.in[bsz - 1] == 0) err[B.pad] = 1;
+                else fse_decode_run_lds(lds(S.dt), lds(S.in), bsz, tl, dst, len);
+            }
+            return;
+        }
+        FseDecBig& G = L.g;
+        G.tab.norm[threadIdx.x] = L.norm[threadIdx.x];
+        __syncthreads();
+        if (!spread_parallel(G.tab, tl)) {
+            if (threadIdx.x == 0) err[B.pad] = 1;
+            return;
+        }
+        dtable_parallel(G.tab, tl, G.dt);
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        if (src[hl + bsz - 1] == 0) {
+            err[B.pad] = 1;
+            return;
+        }
+        fse_decode_run(G.dt, src + hl, bsz, tl, dst, len);
     }
-    FseDecBig& G = L.g;
-    G.tab.norm[threadIdx.x] = L.norm[threadIdx.x];
-    __syncthreads();
-    if (!spread_parallel(G.tab, tl)) {
-        if (threadIdx.x == 0) err[B.pad] = 1;
-        return;
-    }
-    dtable_parallel(G.tab, tl, G.dt);
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    if (src[hl + bsz - 1] == 0) {
-        err[B.pad] = 1;
-        return;
-    }
-    fse_decode_run(G.dt, src + hl, bsz, tl, dst, len);
 }
 
 // ---------------------------------------------------------------- method 0 (RleFse:928-982)
@@ -2980,25 +3129,39 @@ struct FseDecItem {
     uint64_t avail;  // bytes of the framed stream
     uint8_t* out;
 };
-static void fse_block_list(const std::vector<FseDecItem>& items, std::vector<FseBlk>& blks) {
+// Appends the blocks of the items; returns whether one of them is FSE-coded with a table log above
+// 11 (its NCount header's first four bits hold the log minus 5), which the decoder's big layout
+// takes.
+static bool fse_block_list(const std::vector<FseDecItem>& items, std::vector<FseBlk>& blks) {
     std::vector<uint64_t> offs;
+    bool big = false;
     for (const FseDecItem& it : items) {
         if (it.n == 0) continue;
         // the block decoder stores four symbols at a time (blocks start 64 KiB apart in out)
         if (((uintptr_t)it.out & 3u) != 0) throw std::logic_error("FSE block decode: output not 4-byte aligned");
         frame_offsets(it.hsrc, it.n, it.avail, offs);
-        for (size_t k = 0; k < offs.size(); k++)
+        for (size_t k = 0; k < offs.size(); k++) {
+            const uint8_t* h = it.hsrc + offs[k];  // an FSE block holds at least 5 + 2 bytes
+            if (h[0] == 1 && (h[5] & 0xFu) + kFseMinTableLog > kFblkLog) big = true;
             blks.push_back(FseBlk{it.dsrc + offs[k], it.out + k * kFblk,
                                   (uint32_t)std::min<uint64_t>(kFblk, it.n - k * kFblk), 0});
+        }
     }
+    return big;
+}
+static void launch_fse_dec(bool big, const FseBlk* dblks, size_t nb, int* err, hipStream_t st) {
+    if (big)
+        FZ_LAUNCH("fse_dec_blocks", 0, k_fse_dec_blocks<true>, dim3((unsigned)nb), dim3(256), 0, st, dblks, err);
+    else
+        FZ_LAUNCH("fse_dec_blocks", 0, k_fse_dec_blocks<false>, dim3((unsigned)nb), dim3(256), 0, st, dblks, err);
 }
 static void fse_dec_multi(const std::vector<FseDecItem>& items, int* err, DevArena& ws, hipStream_t st) {
     std::vector<FseBlk> blks;
-    fse_block_list(items, blks);
+    const bool big = fse_block_list(items, blks);
     if (blks.empty()) return;
     FseBlk* dblks = ws.take<FseBlk>(blks.size());
     h2d(dblks, blks.data(), blks.size(), st);
-    FZ_LAUNCH("fse_dec_blocks", 0, k_fse_dec_blocks, dim3((unsigned)blks.size()), dim3(256), 0, st, dblks, err);
+    launch_fse_dec(big, dblks, blks.size(), err, st);
     FZ_HIP(hipGetLastError());
     sync(st);  // blks is read by the copy above
 }
@@ -3467,6 +3630,7 @@ struct FseBatch {
     std::mutex mu;
     std::vector<FseBlk> blks;  // pad: index into errs
     std::vector<int*> errs;
+    bool big = false;  // a block takes the decoder's big layout
 };
 FseBatch* fse_batch_create() { return new FseBatch; }
 void fse_batch_destroy(FseBatch* b) { delete b; }
@@ -3474,6 +3638,7 @@ void fse_batch_clear(FseBatch* b) {
     std::lock_guard<std::mutex> g(b->mu);
     b->blks.clear();
     b->errs.clear();
+    b->big = false;
 }
 
 __global__ void k_or_flags(const int* src, int* const* dst, uint32_t n) {
@@ -3493,7 +3658,7 @@ void fse_batch_run(FseBatch* b, DevArena& ws, hipStream_t st) {
     FZ_HIP(hipMemcpyAsync(dblks, b->blks.data(), b->blks.size() * sizeof(FseBlk), hipMemcpyHostToDevice, st));
     FZ_HIP(hipMemcpyAsync(derr, b->errs.data(), ne * sizeof(int*), hipMemcpyHostToDevice, st));
     FZ_HIP(hipMemsetAsync(gerr, 0, ne * sizeof(int), st));
-    FZ_LAUNCH("fse_dec_blocks", 0, k_fse_dec_blocks, dim3((unsigned)b->blks.size()), dim3(256), 0, st, dblks, gerr);
+    launch_fse_dec(b->big, dblks, b->blks.size(), gerr, st);
     FZ_LAUNCH("fse_dec_flags", 0, k_or_flags, dim3((ne + 255) / 256), dim3(256), 0, st, gerr, derr, ne);
     FZ_HIP(hipGetLastError());
 }
@@ -3557,8 +3722,9 @@ void entropy_decode_start(EntropyCall* E, const uint8_t* host, const uint8_t* de
         return;
     }
     std::vector<FseBlk> blks;
-    fse_block_list(fd, blks);
+    const bool big = fse_block_list(fd, blks);
     std::lock_guard<std::mutex> g(defer->mu);
+    defer->big = defer->big || big;
     const uint32_t k = (uint32_t)defer->errs.size();
     defer->errs.push_back(E->err);
     for (FseBlk& b : blks) {

@@ -146,6 +146,20 @@ def test_rle_fse_short_class_streams(ctx, oracle, bumps):
     assert opts.method == 0
 
 
+@pytest.mark.parametrize("density", [0.02, 0.04])
+def test_rle_fse_long_class_streams(ctx, oracle, density):
+    """A long 1-D field with bumps at a few percent of its points: the RLE+FSE method's run-class and
+    value-class streams run to several 64 K-symbol blocks whose FSE bytes exceed the decoder's
+    16 KiB LDS window, so each block decodes over several windows."""
+    rng = np.random.default_rng(77)
+    d = np.zeros(3_000_000, dtype=np.float32)
+    k = int(d.size * density)
+    pos = rng.choice(d.size, size=k, replace=False)
+    d[pos] = (rng.choice([-1.0, 1.0], size=k) * 10.0 ** rng.uniform(-2, 1, size=k)).astype(np.float32)
+    opts = check(ctx, oracle, d, eb_mode=0, abs_eb=1e-3, interp_algo=0)
+    assert opts.method == 0
+
+
 def test_device_entry_points(ctx, oracle):
     import falcons_amd
     shape = (70, 64, 96)
