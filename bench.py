@@ -41,12 +41,16 @@ def parse():
                     help="field: one 512^3 field (the default line); batched: the 4096-chunk mix through the "
                          "batched entry points against plain per-call calls; serving: Poisson request mix")
     ap.add_argument("--chunks", type=int, default=4096, help="batched: number of chunks")
-    ap.add_argument("--workers", type=int, default=24, help="batched/serving: worker sub-contexts (slots)")
+    ap.add_argument("--workers", type=int, default=None,
+                    help="batched: worker sub-contexts (default 16); serving: slots (default 24)")
     ap.add_argument("--no-per-call", action="store_true", help="batched: skip the per-call comparison run")
     ap.add_argument("--rate", type=float, default=200.0, help="serving: requests per second (Poisson)")
     ap.add_argument("--requests", type=int, default=2000, help="serving: requests measured")
     ap.add_argument("--warm-requests", type=int, default=200, help="serving: requests before the measurement")
-    return ap.parse_args()
+    a = ap.parse_args()
+    if a.workers is None:
+        a.workers = 24 if a.workload == "serving" else 16
+    return a
 
 
 # launch-profiler name -> kernel symbol, for the counter data of profiles/

@@ -87,10 +87,13 @@ void destroy_workers(WorkerSet* w) { delete w; }
 
 namespace {
 
+// 16: one worker stream to each of the 16 hardware queues the Python package asks the runtime for
+// (4096 uneven chunks: compress 2.85 s and decompress 1.48 s at 16 workers, 3.02 s and 1.51 s at 24,
+// 3.72 s and 1.71 s at 8).
 int default_workers() {
     const char* e = getenv("FALCONS_WORKERS");
     const int v = e ? atoi(e) : 0;
-    return v > 0 ? std::min(v, 64) : 24;
+    return v > 0 ? std::min(v, 64) : 16;
 }
 
 // Workers that may run at once: each holds a workspace for the largest item, and together they stay
