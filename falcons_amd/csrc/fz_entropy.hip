@@ -1007,11 +1007,17 @@ __global__ void __launch_bounds__(256) k_o1_hist_exact(const int16_t* bins, uint
         if (h[i]) atomicAdd(&cnt[i], h[i]);
 }
 
-// Windowed sample histogram (RleFse:641-658): 512-code windows every `step` codes, the context of
-// each code taken from its true predecessor (0 at the segment start).
-__global__ void __launch_bounds__(256) k_o1_hist_sample(const int16_t* bins, uint64_t n, uint64_t step, uint64_t wlen,
+// Windowed sample histogram (RleFse:641-658) of segment blockIdx.y (codes [g seg, g seg + seg) of the
+// n): wlen-code windows every step = max(segment length / 2048, 512) codes, the context of each code
+// taken from its true predecessor (0 at the segment start). The segments of a call in one launch.
+__global__ void __launch_bounds__(256) k_o1_hist_sample(const int16_t* bins, uint64_t n, uint64_t seg, uint64_t wlen,
                                                         uint32_t* cnt) {
     __shared__ uint32_t h[kO1Ctx * kO1Sym];
+    const uint64_t lo = (uint64_t)blockIdx.y * seg;
+    bins += lo;
+    cnt += (uint64_t)blockIdx.y * kO1Ctx * kO1Sym;
+    n = min(seg, n - lo);
+    const uint64_t step = max(n / 2048, (uint64_t)512);
     for (int i = threadIdx.x; i < kO1Ctx * kO1Sym; i += blockDim.x) h[i] = 0;
     __syncthreads();
     const uint64_t w = (uint64_t)blockIdx.x * step;
@@ -2851,6 +2857,9 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
     std::vector<uint64_t> novf(S * kO1K), ovf_off(S + 1);
     std::vector<uint64_t> seg_n(S), ends;
     uint64_t e = 0;
+    // segments of 2^20 codes or more are sampled, in one launch (they are the first ones: every
+    // segment but the last is seg long)
+    uint64_t s_sampled = 0, nw_max = 0, sampled_codes = 0;
     for (uint64_t g = 0; g < S; g++) {
         const uint64_t lo = g * seg, hi = std::min(lo + seg, n), sn = hi - lo;
         seg_n[g] = sn;
@@ -2860,11 +2869,12 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
         if (sn < (1u << 20)) {
             FZ_LAUNCH("o1_hist", sn * 2, k_o1_hist_exact, dim3(std::min<uint64_t>(1024, nblk(sn))), dim3(256), 0, st, sb, sn, cs, c);
         } else {
-            uint64_t step = sn / 2048;
-            if (step < 512) step = 512;
+            if (s_sampled != g) throw std::logic_error("order-1 histogram: sampled segments out of order");
+            const uint64_t step = std::max<uint64_t>(sn / 2048, 512);
             const uint64_t nw = (sn + step - 1) / step;
-            FZ_LAUNCH("o1_hist", nw * 1024, k_o1_hist_sample, dim3((unsigned)nw), dim3(256), 0, st, sb, sn, step, (uint64_t)512, c);
-            FZ_LAUNCH("o1_hist", 0, k_add_one, dim3(nblk(kO1Ctx * kO1Sym)), dim3(256), 0, st, c, kO1Ctx * kO1Sym);
+            nw_max = std::max(nw_max, nw);
+            sampled_codes += nw * 512;
+            s_sampled = g + 1;
         }
         FZ_HIP(hipGetLastError());
         for (int k = 0; k < kO1K; k++) {
@@ -2872,6 +2882,13 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
             chains.insert(chains.end(), {lo, clo, std::max(chi, clo), g});
             ends.push_back(chi);  // below clo for the empty chunks at the end of a short segment
         }
+    }
+    if (s_sampled) {
+        FZ_LAUNCH("o1_hist", sampled_codes * 2, k_o1_hist_sample, dim3((unsigned)nw_max, (unsigned)s_sampled), dim3(256), 0,
+                  st, bins, n, seg, (uint64_t)512, cnt);
+        FZ_LAUNCH("o1_hist", 0, k_add_one, dim3(nblk(s_sampled * kO1Ctx * kO1Sym)), dim3(256), 0, st, cnt,
+                  (int)(s_sampled * kO1Ctx * kO1Sym));
+        FZ_HIP(hipGetLastError());
     }
     {
         // escapes below each chain's end: binary searches in the sorted escape positions
