@@ -2985,18 +2985,23 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
         uint8_t* chg_next = chg_a;
         uint64_t max_rounds = 1;
         for (int q = 0; q < nch; q++) max_rounds = std::max<uint64_t>(max_rounds, pfirst[q + 1] - pfirst[q] + 1);
-        for (uint64_t round = 0;; round++) {
-            if (round > max_rounds) throw std::runtime_error("tANS piece fix-up did not settle");
-            FZ_HIP(hipMemsetAsync(any, 0, sizeof(unsigned int), st));
-            FZ_LAUNCH("o1_encode_fix", 0, k_o1_fix, dim3(nblk(np)), dim3(256), 0, st, bins, CH, np, ct, slog, spec_end,
-                               spec_bits, end_prev, end_next, nbits, chg_prev, chg_next, any);
-            FZ_HIP(hipGetLastError());
+        // Two rounds per host check: `any` counts the changes of the second only. A round after one
+        // with no change re-runs no piece and changes nothing, so the check ends the loop at the
+        // first pair whose second round saw no change.
+        for (uint64_t round = 0;;) {
+            if (round > max_rounds + 1) throw std::runtime_error("tANS piece fix-up did not settle");
+            for (int k = 0; k < 2; k++, round++) {
+                if (k == 1) FZ_HIP(hipMemsetAsync(any, 0, sizeof(unsigned int), st));
+                FZ_LAUNCH("o1_encode_fix", 0, k_o1_fix, dim3(nblk(np)), dim3(256), 0, st, bins, CH, np, ct, slog,
+                          spec_end, spec_bits, end_prev, end_next, nbits, chg_prev, chg_next, any);
+                FZ_HIP(hipGetLastError());
+                end_prev = end_next;
+                end_next = (end_next == end_a) ? end_b : end_a;
+                chg_prev = chg_next;
+                chg_next = (chg_next == chg_a) ? chg_b : chg_a;
+            }
             unsigned int h_any = 0;
             d2h(&h_any, any, 1, st);
-            end_prev = end_next;
-            end_next = (end_next == end_a) ? end_b : end_a;
-            chg_prev = chg_next;
-            chg_next = (chg_next == chg_a) ? chg_b : chg_a;
             if (!h_any) break;
         }
         exclusive_scan_u64(nbits, scan, np, ws, st);
