@@ -2620,6 +2620,31 @@ static void fse_enc_multi(std::vector<FseEncItem>& items, DevArena& ws, hipStrea
 
 static void put64(uint8_t*& p, uint64_t v) { memcpy(p, &v, 8); p += 8; }
 
+// One part of a stream assembly: len bytes (at most kPieceBytes) from src to dst, device memory.
+struct CopyPiece {
+    const uint8_t* src;
+    uint8_t* dst;
+    uint64_t len;
+};
+constexpr uint64_t kPieceBytes = 64 << 10;
+// A workgroup a part; consecutive threads take consecutive bytes, 16 loads in flight per thread.
+__global__ void __launch_bounds__(256) k_copy_pieces(const CopyPiece* pieces) {
+    const CopyPiece c = pieces[blockIdx.x];
+    for (uint64_t i0 = threadIdx.x; i0 < c.len; i0 += 16ull * blockDim.x) {
+        uint8_t t[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint64_t i = i0 + (uint64_t)k * blockDim.x;
+            t[k] = i < c.len ? c.src[i] : (uint8_t)0;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint64_t i = i0 + (uint64_t)k * blockDim.x;
+            if (i < c.len) c.dst[i] = t[k];
+        }
+    }
+}
+
 static size_t encode_method0(const int16_t* bins, uint64_t n, uint8_t* out, uint64_t cap, DevArena& ws,
                              hipStream_t st) {
     need_room(48, cap);
@@ -3037,33 +3062,43 @@ static size_t encode_method1(const int16_t* bins, uint64_t n, uint8_t* out, uint
         for (int k = 0; k < kO1K; k++) total += hblen[g * kO1K + k];
     }
     need_room(total, cap);
-    uint8_t* p = out;
+    // The headers go to the device in one copy; then one launch moves every piece (headers, tables,
+    // chain streams, escape blocks) to its place, in parts of at most kPieceBytes.
+    std::vector<uint8_t> heads(8 + S * 80);
     {
-        uint64_t s = S;
-        FZ_HIP(hipMemcpyAsync(p, &s, 8, hipMemcpyHostToDevice, st));
-        sync(st);
-        p += 8;
+        uint8_t* h = heads.data();
+        put64(h, S);
+        for (uint64_t g = 0; g < S; g++) {
+            put64(h, seg_n[g]);
+            put64(h, kO1K);
+            for (int k = 0; k < kO1K; k++) put64(h, hblen[g * kO1K + k]);
+            for (int k = 0; k < kO1K; k++) put64(h, novf[g * kO1K + k]);
+        }
     }
+    uint8_t* dheads = ws.take<uint8_t>(heads.size());
+    h2d(dheads, heads.data(), heads.size(), st);
+    std::vector<CopyPiece> pieces;
+    uint8_t* p = out;
+    auto piece = [&](const uint8_t* src, uint64_t len) {
+        for (uint64_t o = 0; o < len; o += kPieceBytes)
+            pieces.push_back(CopyPiece{src + o, p + o, std::min<uint64_t>(kPieceBytes, len - o)});
+        p += len;
+    };
+    piece(dheads, 8);
     for (uint64_t g = 0; g < S; g++) {
-        uint8_t head[8 * 10], *h = head;
-        put64(h, seg_n[g]);
-        put64(h, kO1K);
-        for (int k = 0; k < kO1K; k++) put64(h, hblen[g * kO1K + k]);
-        for (int k = 0; k < kO1K; k++) put64(h, novf[g * kO1K + k]);
-        FZ_HIP(hipMemcpyAsync(p, head, sizeof head, hipMemcpyHostToDevice, st));
-        sync(st);
-        p += sizeof head;
-        FZ_HIP(hipMemcpyAsync(p, nf + g * kO1Ctx * kO1Sym, kO1Ctx * kO1Sym * 2, hipMemcpyDeviceToDevice, st));
-        p += kO1Ctx * kO1Sym * 2;
+        piece(dheads + 8 + g * 80, 80);
+        piece(reinterpret_cast<const uint8_t*>(nf + g * kO1Ctx * kO1Sym), kO1Ctx * kO1Sym * 2);
         for (int k = 0; k < kO1K; k++) {
             const uint64_t q = g * kO1K + k;
-            FZ_HIP(hipMemcpyAsync(p, bufs + q * bstride, hblen[q], hipMemcpyDeviceToDevice, st));
-            p += hblen[q];
+            piece(bufs + q * bstride, hblen[q]);
         }
-        FZ_HIP(hipMemcpyAsync(p, oblock[g], holen[g], hipMemcpyDeviceToDevice, st));
-        p += holen[g];
+        piece(oblock[g], holen[g]);
     }
-    sync(st);
+    CopyPiece* dpieces = ws.take<CopyPiece>(pieces.size());
+    h2d(dpieces, pieces.data(), pieces.size(), st);
+    FZ_LAUNCH("o1_assemble", 2 * (p - out), k_copy_pieces, dim3((unsigned)pieces.size()), dim3(256), 0, st, dpieces);
+    FZ_HIP(hipGetLastError());
+    sync(st);  // heads and pieces (host) are read by the copies above
     return (size_t)(p - out);
 }
 
